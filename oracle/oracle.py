@@ -1,0 +1,261 @@
+"""ctypes binding of the CPU oracle (liboracle_z4c.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Imported by tests/,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of bench.py -- never by
+the product path.
+
+Arrays are numpy float64, C-contiguous, shape (nvar, nn3, nn2, nn1) in the
+reference's AthenaArray order u(n,k,j,i) (src/athena_arrays.hpp:192-193).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_z4c.so")
+
+NZ4C, NADM, NCON, NMAT = 22, 17, 7, 10
+ND1, NDDCHI, NAUX, NAUXEXT = 33, 6, 30, 7
+NGHOST = 4
+
+# Z4c variable indices (src/z4c/z4c.hpp:54-88)
+I_CHI, I_GXX, I_KHAT, I_AXX, I_GAMX, I_THETA, I_ALPHA, I_BETAX = 0, 1, 7, 8, 14, 17, 18, 19
+Z4C_NAMES = ["chi", "gxx", "gxy", "gxz", "gyy", "gyz", "gzz", "Khat",
+             "Axx", "Axy", "Axz", "Ayy", "Ayz", "Azz", "Gamx", "Gamy", "Gamz",
+             "Theta", "alpha", "betax", "betay", "betaz"]
+# ADM indices (src/z4c/z4c.hpp:99-135)
+I_ADM_ALPHA, I_ADM_BETAX, I_ADM_GXX, I_ADM_KXX, I_ADM_PSI4 = 0, 1, 4, 10, 16
+
+
+class Params(C.Structure):
+    """Z4c::opt, defaults of src/z4c/z4c.cpp:318-472."""
+    _fields_ = [(n, C.c_double) for n in (
+        "chi_psi_power", "chi_div_floor", "diss", "eps_floor", "damp_kappa1",
+        "damp_kappa2", "lapse_oplog", "lapse_harmonicf", "lapse_harmonic",
+        "lapse_advect", "lapse_K", "shift_Gamma", "shift_alpha2Gamma",
+        "shift_H", "shift_advect", "shift_eta", "r_max_con")] + [
+        ("cowling", C.c_int), ("tf_force_regularization", C.c_int),
+        ("force_regularization", C.c_int), ("pad_", C.c_int)]
+
+    def __init__(self, **kw):
+        d = dict(chi_psi_power=-4.0, chi_div_floor=-1000.0, diss=0.0,
+                 eps_floor=1e-12, damp_kappa1=0.0, damp_kappa2=0.0,
+                 lapse_oplog=2.0, lapse_harmonicf=1.0, lapse_harmonic=0.0,
+                 lapse_advect=1.0, lapse_K=0.0, shift_Gamma=1.0,
+                 shift_alpha2Gamma=0.0, shift_H=0.0, shift_advect=1.0,
+                 shift_eta=2.0, r_max_con=float("inf"), cowling=0,
+                 tf_force_regularization=1, force_regularization=0, pad_=0)
+        d.update(kw)
+        super().__init__(**d)
+
+
+class Geom(C.Structure):
+    _fields_ = [("nn1", C.c_int), ("nn2", C.c_int), ("nn3", C.c_int),
+                ("ng", C.c_int), ("il", C.c_int), ("iu", C.c_int),
+                ("jl", C.c_int), ("ju", C.c_int), ("kl", C.c_int),
+                ("ku", C.c_int), ("dx", C.c_double * 3), ("x0", C.c_double * 3)]
+
+    @classmethod
+    def block(cls, nx, dx, xmin):
+        """VC block with meshblock/nx = (nx1,nx2,nx3) cells: nx+1 owned
+        vertices per axis plus 4 ghosts each side (src/mesh/meshblock.cpp:869-889).
+        ``xmin`` is the coordinate of the first interior vertex."""
+        nx = np.broadcast_to(np.asarray(nx, dtype=int), (3,))
+        dx = np.broadcast_to(np.asarray(dx, dtype=float), (3,))
+        xmin = np.broadcast_to(np.asarray(xmin, dtype=float), (3,))
+        nn = nx + 1 + 2 * NGHOST
+        g = cls()
+        g.nn1, g.nn2, g.nn3 = (int(v) for v in nn)
+        g.ng = NGHOST
+        g.il, g.iu = NGHOST, NGHOST + int(nx[0])
+        g.jl, g.ju = NGHOST, NGHOST + int(nx[1])
+        g.kl, g.ku = NGHOST, NGHOST + int(nx[2])
+        for d in range(3):
+            g.dx[d] = float(dx[d])
+            g.x0[d] = float(xmin[d]) - NGHOST * float(dx[d])
+        return g
+
+    @property
+    def shape(self):
+        return (self.nn3, self.nn2, self.nn1)
+
+    def coords(self):
+        """x, y, z arrays of the padded vertices (x = x0 + i*dx)."""
+        return tuple(self.x0[d] + np.arange(n) * self.dx[d]
+                     for d, n in enumerate((self.nn1, self.nn2, self.nn3)))
+
+    def mesh(self):
+        x, y, z = self.coords()
+        Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+        return X, Y, Z
+
+    def interior(self):
+        return (slice(self.kl, self.ku + 1), slice(self.jl, self.ju + 1),
+                slice(self.il, self.iu + 1))
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "all"],
+                       check=True, capture_output=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        P = C.POINTER
+        d = C.c_double
+        pd = P(C.c_double)
+        pg = P(Geom)
+        pp = P(Params)
+        L = _lib
+        L.oracle_fd_point.restype = d
+        L.oracle_fd_point.argtypes = [C.c_int, pg, pd, C.c_long, C.c_int]
+        L.oracle_lx_point.restype = d
+        L.oracle_lx_point.argtypes = [pg, pd, C.c_long, C.c_int, d]
+        L.oracle_prep_derivs.argtypes = [pp, pg, pd, pd, pd, pd]
+        L.oracle_rhs.argtypes = [pp, pg, pd, pd, pd, pd, pd]
+        L.oracle_boundary_rhs.argtypes = [pp, pg, pd, P(C.c_int), pd]
+        L.oracle_algconstr.argtypes = [pp, pg, pd]
+        L.oracle_algconstr_range.argtypes = [pp, pg, pd] + [C.c_int] * 7
+        L.oracle_adm_to_z4c.argtypes = [pp, pg, pd, pd, pd]
+        L.oracle_z4c_to_adm.argtypes = [pp, pg, pd, pd, pd]
+        L.oracle_adm_constraints.argtypes = [pp, pg, pd, pd, pd, pd, pd, pd]
+        L.oracle_bt_sum.argtypes = [pg, C.c_int, pd, pd, C.c_int, P(pd), pd, d]
+        L.oracle_weighted_ave.argtypes = [pg, C.c_int, pd, pd, pd, pd]
+        L.oracle_add_rhs.argtypes = [pg, C.c_int, pd, d, pd]
+        L.oracle_new_dt.restype = d
+        L.oracle_new_dt.argtypes = [pg, d]
+        L.oracle_periodic_fill.argtypes = [pg, C.c_int, pd]
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def zeros(n, g):
+    return np.zeros((n,) + g.shape)
+
+
+def fd_point(kind, g, u, p, d):
+    return lib().oracle_fd_point(kind, C.byref(g), _p(u), p, d)
+
+
+def prep_derivs(o, g, u, want_aux=False):
+    d1, dd = zeros(ND1, g), zeros(NDDCHI, g)
+    aux = zeros(NAUX, g) if want_aux else None
+    lib().oracle_prep_derivs(C.byref(o), C.byref(g), _p(u), _p(d1), _p(dd), _p(aux))
+    return (d1, dd, aux) if want_aux else (d1, dd)
+
+
+def rhs(o, g, u, mat=None, derivs=None):
+    """Z4cRHS of ``u`` (prepares derivatives first unless given)."""
+    d1, dd = derivs if derivs is not None else prep_derivs(o, g, u)
+    out = zeros(NZ4C, g)
+    lib().oracle_rhs(C.byref(o), C.byref(g), _p(u), _p(mat), _p(d1), _p(dd), _p(out))
+    return out
+
+
+def boundary_rhs(o, g, u, faces, out):
+    f = (C.c_int * 6)(*[int(bool(v)) for v in faces])
+    lib().oracle_boundary_rhs(C.byref(o), C.byref(g), _p(u), f, _p(out))
+    return out
+
+
+def algconstr(o, g, u):
+    lib().oracle_algconstr(C.byref(o), C.byref(g), _p(u))
+    return u
+
+
+def algconstr_range(o, g, u, il, iu, jl, ju, kl, ku, skip_physical):
+    lib().oracle_algconstr_range(C.byref(o), C.byref(g), _p(u), il, iu, jl, ju,
+                                 kl, ku, int(skip_physical))
+    return u
+
+
+def adm_to_z4c(o, g, adm, want_aux_ext=False):
+    u = zeros(NZ4C, g)
+    ax = zeros(NAUXEXT, g) if want_aux_ext else None
+    lib().oracle_adm_to_z4c(C.byref(o), C.byref(g), _p(adm), _p(u), _p(ax))
+    return (u, ax) if want_aux_ext else u
+
+
+def z4c_to_adm(o, g, u, want_aux_ext=False):
+    adm = zeros(NADM, g)
+    ax = zeros(NAUXEXT, g) if want_aux_ext else None
+    lib().oracle_z4c_to_adm(C.byref(o), C.byref(g), _p(u), _p(adm), _p(ax))
+    return (adm, ax) if want_aux_ext else adm
+
+
+def adm_constraints(o, g, u, adm, mat=None, derivs=None):
+    d1, dd = derivs if derivs is not None else prep_derivs(o, g, u)
+    con = zeros(NCON, g)
+    lib().oracle_adm_constraints(C.byref(o), C.byref(g), _p(u), _p(adm), _p(mat),
+                                 _p(d1), _p(dd), _p(con))
+    return con
+
+
+def bt_sum(g, u0, ks, ws, dt):
+    out = np.empty_like(u0)
+    arr = (C.POINTER(C.c_double) * len(ks))(*[_p(k) for k in ks])
+    w = (C.c_double * len(ws))(*ws)
+    lib().oracle_bt_sum(C.byref(g), u0.shape[0], _p(out), _p(u0), len(ks), arr, w, dt)
+    return out
+
+
+def weighted_ave(g, u_out, u1, u2, w):
+    ww = (C.c_double * 3)(*w)
+    lib().oracle_weighted_ave(C.byref(g), u_out.shape[0], _p(u_out), _p(u1), _p(u2), ww)
+    return u_out
+
+
+def add_rhs(g, rhs_, wght, u_out):
+    lib().oracle_add_rhs(C.byref(g), u_out.shape[0], _p(rhs_), wght, _p(u_out))
+    return u_out
+
+
+def new_dt(g, cfl):
+    return lib().oracle_new_dt(C.byref(g), cfl)
+
+
+def periodic_fill(g, u):
+    lib().oracle_periodic_fill(C.byref(g), u.shape[0], _p(u))
+    return u
+
+
+# Butcher tableau of bt_rk4 (src/task_list/task_list_integrators_butcher.cpp:37-49)
+BT_RK4_A = [[0, 0, 0, 0], [0.5, 0, 0, 0], [0, 0.5, 0, 0], [0, 0, 1, 0]]
+BT_RK4_B = [1. / 6., 1. / 3., 1. / 3., 1. / 6.]
+
+
+def rk4_step(o, g, u, dt, mat=None, between_stages=None):
+    """One bt_rk4 step in the GR_Z4c task order (task_list_gr_z4c.cpp:45-89):
+    k_s = RHS(u); u = u_n + sum a dt k (SumBT_ak(stage+1) / SumBT_bk);
+    optional ghost update hook; AlgConstr; derivatives re-prepared by the
+    next RHS call.  Ghosts of k are zero, so without a hook ghosts stay u_n."""
+    un = u.copy()
+    ks = []
+    cur = u
+    for s in range(4):
+        ks.append(rhs(o, g, cur, mat))
+        if s < 3:
+            cur = bt_sum(g, un, ks, BT_RK4_A[s + 1][:len(ks)], dt)
+        else:
+            cur = bt_sum(g, un, ks, BT_RK4_B, dt)
+        if between_stages is not None:
+            between_stages(cur)
+        algconstr(o, g, cur)
+    return cur, ks
