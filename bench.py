@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the batched FP64 Z4c RK4 step on one MI355X.
+
+Workload (the batched uneven-block configuration): 2048 vertex-centred
+MeshBlocks from a seeded octree refined toward two punctures (levels 0-7,
+spacing 2^-level), interior sizes 70% 16^3 / 20% 24^3 / 10% 32^3 cells,
+Brill-Lindquist data (m1 = m2 = 0.5, separation 10, pre-collapsed lapse,
+zero shift, K = 0), moving-puncture gauge.  A step is one classical RK4 step
+(bt_rk4) in the GR_Z4c stage order on every block: Z4cRHS (+KO), SumBT_ak /
+SumBT_bk, AlgConstr, PrepareZ4cDerivatives -- one launch per kernel per
+stage for all blocks.  Ghost zones stay at u_n through the step (no inter-
+block exchange between blocks of different sizes).
+
+value = owned point-updates per second: sum over blocks of nx^3 (the
+reference's zone count, src/main.hpp:729-731) per RK4 step, divided by the
+wall time of the timed steps.  Inputs are resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gr-athena_amd"))
+
+import numpy as np  # noqa: E402
+
+# Algorithmic cost per interior point of one Z4cRHS evaluation (FP64 flops)
+# and of one RK4 point-step (bytes), the figures the roofline is priced with.
+F_RHS = 8.5e3
+B_STEP = 2992.0
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nblocks", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-perblock", action="store_true", help="skip the per-block launch comparison")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_workload(nblocks):
+    from z4c_amd import Params
+    from z4c_amd.initial_data import brill_lindquist_adm, octree_blocks, owned_points
+    blocks = octree_blocks(nblocks=nblocks, seed=0)
+    params = Params(lapse_oplog=2.0, lapse_harmonicf=1.0, lapse_advect=1.0, shift_Gamma=1.0,
+                    shift_advect=1.0, shift_eta=2.0, diss=0.02, damp_kappa1=0.02,
+                    chi_div_floor=1e-5)
+    return blocks, params, owned_points(blocks), brill_lindquist_adm
+
+
+def cpu_baseline(blocks, params, adm_fn, budget_s):
+    """The CPU oracle (single-threaded C, one block per host thread) on a
+    sample of the same workload: whole blocks, one RK4 step each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from concurrent.futures import ThreadPoolExecutor
+    op = orc.Params(**params.as_dict())
+    sample = blocks[:: max(1, len(blocks) // 64)]
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+
+    def one(g):
+        og = orc.Geom.block(g.nx, g.dx, g.xmin)
+        u = orc.adm_to_z4c(op, og, np.ascontiguousarray(adm_fn(g)))
+        dt = 0.25 * min(g.dx)
+        t0 = time.perf_counter()
+        orc.rk4_step(op, og, u, dt)
+        return time.perf_counter() - t0, g.nx[0] * g.nx[1] * g.nx[2]
+
+    done_pts, t_wall, used = 0, 0.0, 0
+    t_start = time.perf_counter()
+    with ThreadPoolExecutor(nthreads) as ex:
+        futs = [ex.submit(one, g) for g in sample]
+        for f in futs:
+            _, pts = f.result()
+            done_pts += pts
+            used += 1
+            if time.perf_counter() - t_start > budget_s:
+                break
+        for f in futs:
+            f.cancel()
+    t_wall = time.perf_counter() - t_start
+    return {"value": done_pts / t_wall, "unit": "point-updates/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"{used} blocks (every {max(1, len(blocks) // 64)}th of the workload), "
+                      f"one bt_rk4 step each, oracle C code, {nthreads} host threads, "
+                      f"{t_wall:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    import torch
+    from z4c_amd.z4c import Z4c
+
+    blocks, params, owned, adm_fn = make_workload(args.nblocks)
+    interior_pts = int(sum((g.nx[0] + 1) * (g.nx[1] + 1) * (g.nx[2] + 1) for g in blocks))
+    log(f"{len(blocks)} blocks, owned points {owned}, interior points {interior_pts}")
+    z = Z4c(blocks, params)
+    log(f"padded points {z.total_points}")
+    # initial data: Brill-Lindquist ADM -> Z4c on the device
+    adm = z.empty(17)
+    for b, g in enumerate(blocks):
+        z.block_view(adm, b, 17).copy_(torch.from_numpy(adm_fn(g)))
+    u = z.zeros()
+    z.ADMToZ4c(adm, u)
+    del adm
+    un, k = z.make_rk_registers()
+    z.PrepareZ4cDerivatives(u)
+    dt = min(z.NewBlockTimeStep(0.25), 1.0)
+    torch.cuda.synchronize()
+    log("initial data ready")
+
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def step(record):
+        un.copy_(u)
+        for s in range(1, 5):
+            if record:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                z.Z4cRHS(u, None, k[s - 1])
+                e1.record(stream)
+                ev.append((e0, e1))
+                # the rest of the stage (update, AlgConstr, derivative prep)
+                from z4c_amd.z4c import BT_RK4_A, BT_RK4_B
+                if s < 4:
+                    z.bt_sum(u, un, k[:s], BT_RK4_A[s][:s], dt)
+                else:
+                    z.bt_sum(u, un, k, BT_RK4_B, dt)
+                z.AlgConstr(u)
+                z.PrepareZ4cDerivatives(u)
+            else:
+                z.rk4_stage(s, dt, u, un, k)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    rate = owned * args.steps / T
+    t_rhs = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
+    if not torch.isfinite(u).all():
+        raise RuntimeError("non-finite state after the timed steps")
+    log(f"{args.steps} steps in {T:.3f} s -> {rate:.4e} point-updates/s; Z4cRHS avg {t_rhs*1e3:.3f} ms")
+
+    out = {
+        "metric": "FP64 Z4c point-updates/s, batched vs per-block launch; adjoint/forward cost",
+        "value": rate, "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": T / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched uneven AMR blocks: 2048 VC MeshBlocks (70% 16^3, "
+                               "20% 24^3, 10% 32^3), levels 0-7, Brill-Lindquist binary, "
+                               "bt_rk4, one launch per kernel per stage",
+                   "nblocks": len(blocks), "owned_points": owned,
+                   "interior_points": interior_pts, "padded_points": z.total_points,
+                   "parallelism": "single GPU"},
+    }
+    ach = F_RHS * interior_pts / t_rhs
+    out["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
+                       "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
+                       "kernel": "Z4cRHS (curvature + assembly passes)",
+                       "note": "FP64 vector peak; no MFMA is used (stencil + pointwise algebra)",
+                       "avg_ms": t_rhs * 1e3,
+                       "frac_hbm_step": rate * B_STEP / PEAK_HBM}
+
+    if not args.no_perblock:
+        out["per_block_launch"] = per_block(z, blocks, params, u, un, k, dt, owned)
+        out["batched_speedup_vs_per_block"] = rate / out["per_block_launch"]["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def per_block(z, blocks, params, u, un, k, dt, owned):
+    """The same RK4 step issued block by block (one launch per kernel per
+    stage per block, a single-block context each, sharing the batched
+    registers): the per-MeshBlock loop of the reference's task list on the
+    same GPU."""
+    import torch
+    from z4c_amd.z4c import Z4c
+    ctxs = [Z4c([g], params) for g in blocks]
+
+    def view(buf, b, g):
+        o = z.offsets[b] * 22
+        return buf[o:o + g.npts * 22]
+    regs = [(view(u, b, g), view(un, b, g), [view(x, b, g) for x in k])
+            for b, g in enumerate(blocks)]
+    for c, (v, _, _) in zip(ctxs, regs):
+        c.PrepareZ4cDerivatives(v)
+
+    def step():
+        un.copy_(u)
+        for s in range(1, 5):
+            for c, (v, vn, vk) in zip(ctxs, regs):
+                c.rk4_stage(s, dt, v, vn, vk)
+    step()
+    torch.cuda.synchronize()
+    n = 2
+    t0 = time.perf_counter()
+    for _ in range(n):
+        step()
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return {"value": owned * n / T, "unit": "point-updates/s", "steps": n,
+            "ms_per_step": T / n * 1e3,
+            "form": "per-block launches: 2048 single-block contexts, one rk4_stage call "
+                    "(5 kernel launches) per block per stage"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,255 @@
+// z4c_device.hpp -- gfx950 device building blocks of the Z4c hot path:
+// block table, finite-difference stencils and 3x3 symmetric algebra.
+//
+// Arithmetic is FP64 throughout.  Every stencil and algebraic expression is
+// evaluated in the operation order of the reference so results agree with
+// the CPU path to rounding (the build disables FMA contraction):
+//   stencils   src/utils/finite_differencing.tpp (DBG_SYMMETRIZE_FD branches,
+//              on by src/defs.hpp.in:18), NGHOST = 4
+//   algebra    src/utils/linear_algebra.hpp:47-118,271-290,
+//              src/utils/floating_point.hpp:110-131
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace z4c {
+
+constexpr int NG = 4;
+constexpr int NVAR = 22;
+constexpr int ND1 = 33;  // d(alpha, chi, beta^c, g~_cd)/dx^b, index b*11 + f
+constexpr int NCV = 12;  // curvature pass: R~_ab + R^phi_ab, D_a D_b alpha
+
+// Z4c variable indices (src/z4c/z4c.hpp:54-88)
+enum : int {
+  V_CHI = 0, V_GXX = 1, V_KHAT = 7, V_AXX = 8, V_GAMX = 14, V_THETA = 17,
+  V_ALPHA = 18, V_BETAX = 19
+};
+// ADM / constraint / matter indices (src/z4c/z4c.hpp:99-190)
+enum : int { A_ALPHA = 0, A_BETAX = 1, A_GXX = 4, A_KXX = 10, A_PSI4 = 16, NADM = 17 };
+enum : int { C_C = 0, C_H = 1, C_M = 2, C_Z = 3, C_MX = 4, NCON = 7 };
+enum : int { M_RHO = 0, M_SX = 1, M_SXX = 4, NMAT = 10 };
+// prepared first-derivative fields, per direction b: f = 0 alpha, 1 chi,
+// 2..4 beta^x..z, 5..10 g~_xx..g~_zz
+enum : int { F_ALPHA = 0, F_CHI = 1, F_BETA = 2, F_G = 5 };
+
+// SYM2 component order xx,xy,xz,yy,yz,zz (src/athena_tensor.hpp:57-179)
+__host__ __device__ constexpr int sym(int a, int b) {
+  return (a == b) ? (a == 0 ? 0 : (a == 1 ? 3 : 5))
+                  : ((a + b == 1) ? 1 : ((a + b == 2) ? 2 : 4));
+}
+
+// Device-side description of one MeshBlock.
+struct DevBlock {
+  long long off;     // element offset in state buffers (per-block [nvar][npts])
+  long long d1off;   // element offset in the prepared-derivative buffer
+  int nn1, nn2, nn3;
+  int il, iu, jl, ju, kl, ku;
+  int npts;          // nn1*nn2*nn3
+  int faces;         // Sommerfeld face bits
+  double dx[3];
+  double x0[3];
+  double cidx1[3];   // 1/(dx*60)        Dx
+  double cidx2[3];   // (1/dx)^2/180     Dxx
+  double cidx1lo[3]; // 1/(dx*2)         Ds
+  double cidxd[3];   // -2^-8/(dx*1)     Diss (times diss)
+  double lidx[3];    // 1/(dx*60)        Lx
+};
+
+// A tile = up to TILE consecutive points of one box of one block.
+struct Tile {
+  int block;
+  int start;
+};
+
+constexpr int TILE = 256;
+
+// Box kinds used by the launch tables.
+enum BoxKind : int { BOX_INTERIOR = 0, BOX_PREP = 1, BOX_WHOLE = 2, NBOX = 3 };
+
+struct Box {
+  int lo[3];   // i, j, k
+  int n[3];    // extents
+};
+
+// ---------------------------------------------------------------------------
+// Buffer addressing.  A workgroup works on one block, so the block's field
+// buffer is a wave-uniform buffer resource (SGPRs).  Each thread keeps one
+// 32-bit byte offset for its point (v0) and one per axis shifted four points
+// back (vd[d] = (p - 4*stride_d)*8), so every stencil access is a
+// buffer_load with a VGPR offset and a uniform SGPR offset
+// (field*N + (m+4)*stride_d)*8 >= 0 -- no 64-bit per-lane addresses.
+// Stencil points stay inside the block for interior points (index >= 4).
+// ---------------------------------------------------------------------------
+constexpr int RSRC_WORD3 = 0x00020000;  // gfx9 family (incl. gfx950) raw buffer
+
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+
+__device__ __forceinline__ Buf make_buf(const double* base, long long nbytes) {
+  const int nr = (nbytes > 0x7fffffffLL) ? 0x7fffffff : (int)nbytes;
+  return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, nr,
+                                               RSRC_WORD3)};
+}
+
+__device__ __forceinline__ double bload(const Buf& b, unsigned vbyte, int sbyte) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(b.r, vbyte, sbyte, 0));
+}
+
+// Per-thread addressing of one point.
+struct Pt {
+  unsigned v0;     // p*8
+  unsigned vd[3];  // (p - 4*st[d])*8
+  int sd[3];       // st[d]*8
+  int fb;          // field stride N*8
+};
+
+__device__ __forceinline__ Pt make_pt(int p, int nn1, int nn2, int N) {
+  Pt P;
+  const int st[3] = {1, nn1, nn1 * nn2};
+  P.v0 = (unsigned)p * 8u;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    P.vd[d] = (unsigned)(p - 4 * st[d]) * 8u;
+    P.sd[d] = st[d] * 8;
+  }
+  P.fb = N * 8;
+  return P;
+}
+
+// value of field n at the point
+__device__ __forceinline__ double ldc(const Buf& b, const Pt& P, int n) {
+  return bload(b, P.v0, n * P.fb);
+}
+// value of field n at offset m (-4..4) along axis d
+__device__ __forceinline__ double ldm(const Buf& b, const Pt& P, int n, int d, int m) {
+  return bload(b, P.vd[d], n * P.fb + (m + 4) * P.sd[d]);
+}
+
+// ---------------------------------------------------------------------------
+// Finite differences of field n along axis d at the point.
+// ---------------------------------------------------------------------------
+// FDStencilCenteredDegreeOdd<1,3> {-1, 9, -45}/60
+__device__ __forceinline__ double fd_dx(const Buf& b, const Pt& P, int n, int d,
+                                        double cidx) {
+  double out = 0.0;
+  out += -1.0 * (ldm(b, P, n, d, -3) - ldm(b, P, n, d, 3));
+  out += 9.0 * (ldm(b, P, n, d, -2) - ldm(b, P, n, d, 2));
+  out += -45.0 * (ldm(b, P, n, d, -1) - ldm(b, P, n, d, 1));
+  return out * cidx;
+}
+
+// FDStencilCenteredDegreeEven<2,3> {2, -27, 270, -490}/180
+__device__ __forceinline__ double fd_dxx(const Buf& b, const Pt& P, int n, int d,
+                                         double cidx2) {
+  double out = 0.0;
+  out += 2.0 * (ldm(b, P, n, d, -3) + ldm(b, P, n, d, 3));
+  out += -27.0 * (ldm(b, P, n, d, -2) + ldm(b, P, n, d, 2));
+  out += 270.0 * (ldm(b, P, n, d, -1) + ldm(b, P, n, d, 1));
+  out += -490.0 * ldm(b, P, n, d, 0);
+  return out * cidx2;
+}
+
+// FDStencilCenteredDegreeOdd<1,1> {-1}/2
+__device__ __forceinline__ double fd_ds(const Buf& b, const Pt& P, int n, int d,
+                                        double cidxlo) {
+  double out = 0.0;
+  out += -1.0 * (ldm(b, P, n, d, -1) - ldm(b, P, n, d, 1));
+  return out * cidxlo;
+}
+
+// FDStencilCenteredDegreeEven<8,4> {1,-8,28,-56,70}, scaled -2^-8/dx * diss
+__device__ __forceinline__ double fd_diss(const Buf& b, const Pt& P, int n, int d,
+                                          double cidxd, double diss) {
+  double out = 0.0;
+  out += 1.0 * (ldm(b, P, n, d, -4) + ldm(b, P, n, d, 4));
+  out += -8.0 * (ldm(b, P, n, d, -3) + ldm(b, P, n, d, 3));
+  out += 28.0 * (ldm(b, P, n, d, -2) + ldm(b, P, n, d, 2));
+  out += -56.0 * (ldm(b, P, n, d, -1) + ldm(b, P, n, d, 1));
+  out += 70.0 * ldm(b, P, n, d, 0);
+  return out * cidxd * diss;
+}
+
+// Lop-sided advective derivative: FDStencilBiasedLeft<1,4,2> on -4..+2 if
+// vx < 0, FDStencilBiasedRight<1,4,2> on -2..+4 otherwise; both sums are
+// formed (as the reference does) and one is selected.
+__device__ __forceinline__ double fd_lx(const Buf& b, const Pt& P, int n, int d,
+                                        double lidx, double vx) {
+  const double um4 = ldm(b, P, n, d, -4), um3 = ldm(b, P, n, d, -3),
+               um2 = ldm(b, P, n, d, -2), um1 = ldm(b, P, n, d, -1),
+               u0 = ldm(b, P, n, d, 0), up1 = ldm(b, P, n, d, 1),
+               up2 = ldm(b, P, n, d, 2), up3 = ldm(b, P, n, d, 3),
+               up4 = ldm(b, P, n, d, 4);
+  double dl = 0.0;
+  dl += 1.0 * um4;
+  dl += -8.0 * um3;
+  dl += 30.0 * um2;
+  dl += -80.0 * um1;
+  dl += 35.0 * u0;
+  dl += 24.0 * up1;
+  dl += -2.0 * up2;
+  double dr = 0.0;
+  dr += -1.0 * up4;
+  dr += 8.0 * up3;
+  dr += -30.0 * up2;
+  dr += 80.0 * up1;
+  dr += -35.0 * u0;
+  dr += -24.0 * um1;
+  dr += 2.0 * um2;
+  return ((vx < 0) ? (vx * dl) : (vx * dr)) * lidx;
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 symmetric algebra (SYM2 order)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double det3(const double* g) {
+  const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
+               gzz = g[5];
+  return (-(gxz * gxz) * gyy + 2 * gxy * gxz * gyz - gxx * (gyz * gyz) -
+          (gxy * gxy) * gzz + gxx * gyy * gzz);
+}
+
+__device__ __forceinline__ void inv3(double oodetg, const double* g, double* u) {
+  const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
+               gzz = g[5];
+  u[0] = (-(gyz * gyz) + gyy * gzz) * oodetg;
+  u[1] = (gxz * gyz - gxy * gzz) * oodetg;
+  u[3] = (-(gxz * gxz) + gxx * gzz) * oodetg;
+  u[2] = (-gxz * gyy + gxy * gyz) * oodetg;
+  u[4] = (gxy * gxz - gxx * gyz) * oodetg;
+  u[5] = (-(gxy * gxy) + gxx * gyy) * oodetg;
+}
+
+__device__ __forceinline__ double trace_rank2(double detginv, const double* g,
+                                              const double* A) {
+  const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
+               gzz = g[5];
+  const double Axx = A[0], Axy = A[1], Axz = A[2], Ayy = A[3], Ayz = A[4],
+               Azz = A[5];
+  return detginv * (-2. * Ayz * gxx * gyz + Axx * gyy * gzz +
+                    gxx * (Azz * gyy + Ayy * gzz) +
+                    2. * (gxz * (Ayz * gxy - Axz * gyy + Axy * gyz) +
+                          gxy * (Axz * gyz - Axy * gzz)) -
+                    Azz * (gxy * gxy) - Ayy * (gxz * gxz) - Axx * (gyz * gyz));
+}
+
+__device__ __forceinline__ double dmax(double a, double b) {
+  return (a < b) ? b : a;  // std::max semantics
+}
+
+__device__ __forceinline__ double regularize_nth_root(double value,
+                                                      double value_reg,
+                                                      double eps_floor,
+                                                      double npow) {
+  const double eps = value - value_reg;
+  return (eps < eps_floor) ? (1.0 + eps_floor / npow) : pow(value, 1.0 / npow);
+}
+
+__device__ __forceinline__ double regularize_near_zero(double v, double e) {
+  return (v >= 0) ? ((v < e) ? e : v) : ((v > -e) ? -e : v);
+}
+
+}  // namespace z4c

@@ -1,0 +1,1305 @@
+// z4c_kernels.hip -- gfx950 kernels and the C-ABI of the batched Z4c hot path.
+//
+// Data layout (HBM): a buffer of `nvar` fields holds the registered blocks
+// back to back; block b with point offset off_b (= sum of npts of earlier
+// blocks) stores field n at element (off_b*nvar + n*npts_b + p), p the
+// AthenaArray flat index i + nn1*(j + nn2*k).  One launch covers every
+// block: a host-built tile table maps 256-thread workgroups to (block,
+// first point) pairs of a per-block box (interior, derivative-prep region,
+// or whole padded block).  Workgroup IDs are remapped so that the tiles one
+// XCD receives are contiguous in the table, keeping a block's halo in one
+// L2 (workgroups are dealt to the 8 XCDs round robin).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "z4c_device.hpp"
+#include "z4c_mi355x.h"
+#include "z4c_rhs_point.hpp"
+
+using namespace z4c;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCALL(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(Z4C_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Tile decode: XCD-contiguous remap of the workgroup id, then tile -> point.
+// ---------------------------------------------------------------------------
+struct Launch {
+  const Tile* tiles;
+  const Box* boxes;
+  int ntiles;
+};
+
+__device__ __forceinline__ bool tile_point(const Launch& L, int& blk, int& i,
+                                           int& j, int& k) {
+  const int G = gridDim.x;  // multiple of 8
+  const int w = blockIdx.x;
+  const int t = (w & 7) * (G >> 3) + (w >> 3);
+  if (t >= L.ntiles) return false;
+  const Tile T = L.tiles[t];
+  blk = T.block;
+  const Box bx = L.boxes[blk];
+  const int q = T.start + (int)threadIdx.x;
+  const int n01 = bx.n[0] * bx.n[1];
+  if (q >= n01 * bx.n[2]) return false;
+  const int kk = q / n01;
+  const int r = q - kk * n01;
+  const int jj = r / bx.n[0];
+  i = bx.lo[0] + (r - jj * bx.n[0]);
+  j = bx.lo[1] + jj;
+  k = bx.lo[2] + kk;
+  return true;
+}
+
+__device__ __forceinline__ int flat(const DevBlock& B, int i, int j, int k) {
+  return i + B.nn1 * (j + B.nn2 * k);
+}
+
+// ---------------------------------------------------------------------------
+// PrepareZ4cDerivatives (src/z4c/adm_z4c.cpp:977-1013, aux fill :1044-1119)
+// ---------------------------------------------------------------------------
+struct PrepArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* u;
+  double* d1;
+  double* aux;  // may be null
+  double chi_psi_power, chi_div_floor;
+};
+
+__global__ void __launch_bounds__(TILE) k_prep(PrepArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const double* u = a.u + B.off * NVAR;
+  double* d1 = a.d1 + B.off * ND1;
+  const int p = flat(B, i, j, k);
+  const Buf U = make_buf(u, (long long)NVAR * N * 8);
+  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  const bool in_i = (i >= B.il && i <= B.iu), in_j = (j >= B.jl && j <= B.ju),
+             in_k = (k >= B.kl && k <= B.ku);
+  const bool inside[3] = {in_i, in_j, in_k};
+#pragma unroll
+  for (int dir = 0; dir < 3; ++dir) {
+    if (!inside[dir]) continue;  // along `dir` only the interior is needed
+    const double c = B.cidx1[dir];
+    double* o = d1 + dir * 11 * N + p;
+    o[F_ALPHA * N] = fd_dx(U, P, V_ALPHA, dir, c);
+    o[F_CHI * N] = fd_dx(U, P, V_CHI, dir, c);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) o[(F_BETA + m) * N] = fd_dx(U, P, V_BETAX + m, dir, c);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) o[(F_G + m) * N] = fd_dx(U, P, V_GXX + m, dir, c);
+  }
+  if (a.aux == nullptr || !(in_i && in_j && in_k)) return;
+  // storage.aux: physical derivatives by the chain rule (interior only)
+  double* aux = a.aux + B.off * 30;
+  const double pw = a.chi_psi_power;
+  const double chi_g = dmax(u[V_CHI * N + p], a.chi_div_floor);
+  const double oochi = 1.0 / chi_g;
+  const double fac = 4.0 / pw;
+  const double chi_pow = (pw == -4.) ? 0.0 : pow(chi_g, fac);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double dchi_c = d1[(c * 11 + F_CHI) * N + p];
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      const double dg = d1[(c * 11 + F_G + ab) * N + p];
+      const double gab = u[(V_GXX + ab) * N + p];
+      const double v = (pw == -4.) ? oochi * (dg - oochi * gab * dchi_c)
+                                   : chi_pow * (dg + fac * oochi * gab * dchi_c);
+      aux[(12 + c * 6 + ab) * N + p] = v;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) aux[(3 + c * 3 + m) * N + p] = d1[(c * 11 + F_BETA + m) * N + p];
+    aux[c * N + p] = d1[(c * 11 + F_ALPHA) * N + p];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Z4cRHS (+ optional fused Sommerfeld faces)
+// ---------------------------------------------------------------------------
+struct RhsArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* u;
+  const double* d1;
+  const double* mat;  // may be null
+  double* rhs;
+  double* cv;         // curvature-pass output, 12 fields
+  RhsParams o;
+};
+
+__device__ void sommerfeld_point(const DevBlock& B, const Buf& U, const Pt& P,
+                                 const double* __restrict__ u, double* __restrict__ rhs,
+                                 int i, int j, int k, int N) {
+  const int p = flat(B, i, j, k);
+  const double sq2 = sqrt(2.);
+  const double x1 = B.x0[0] + i * B.dx[0];
+  const double x2 = B.x0[1] + j * B.dx[1];
+  const double x3 = B.x0[2] + k * B.dx[2];
+  const double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+  const double su[3] = {x1 / r, x2 / r, x3 / r};
+  double th = -u[V_THETA * N + p] / r;
+  double kh = -sq2 * u[V_KHAT * N + p] / r;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    th -= su[a] * fd_ds(U, P, V_THETA, a, B.cidx1lo[a]);
+    kh -= sq2 * su[a] * fd_ds(U, P, V_KHAT, a, B.cidx1lo[a]);
+  }
+  rhs[V_THETA * N + p] = th;
+  rhs[V_KHAT * N + p] = kh;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double t = -u[(V_GAMX + a) * N + p] / r;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) t -= su[b] * fd_ds(U, P, V_GAMX + a, b, B.cidx1lo[b]);
+    rhs[(V_GAMX + a) * N + p] = t;
+  }
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    double t = -u[(V_AXX + ab) * N + p] / r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t -= su[c] * fd_ds(U, P, V_AXX + ab, c, B.cidx1lo[c]);
+    rhs[(V_AXX + ab) * N + p] = t;
+  }
+}
+
+__device__ __forceinline__ bool on_sommerfeld_face(const DevBlock& B, int i,
+                                                   int j, int k) {
+  const int f = B.faces;
+  return ((f & 1) && i == B.il) || ((f & 2) && i == B.iu) ||
+         ((f & 4) && j == B.jl) || ((f & 8) && j == B.ju) ||
+         ((f & 16) && k == B.kl) || ((f & 32) && k == B.ku);
+}
+
+__global__ void __launch_bounds__(TILE) k_rhs_curv(RhsArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const Buf U = make_buf(a.u + B.off * NVAR, (long long)NVAR * N * 8);
+  const Buf D = make_buf(a.d1 + B.off * ND1, (long long)ND1 * N * 8);
+  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  rhs_curv_point(a.o, B, U, D, P, a.cv + B.off * NCV, p, N);
+}
+
+template <bool WITH_SOMMERFELD>
+__global__ void __launch_bounds__(TILE) k_rhs_asm(RhsArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const double* u = a.u + B.off * NVAR;
+  const Buf U = make_buf(u, (long long)NVAR * N * 8);
+  const Buf D = make_buf(a.d1 + B.off * ND1, (long long)ND1 * N * 8);
+  const Buf CV = make_buf(a.cv + B.off * NCV, (long long)NCV * N * 8);
+  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  double* rhs = a.rhs + B.off * NVAR;
+  if (a.mat) {
+    const Buf MT = make_buf(a.mat + B.off * NMAT, (long long)NMAT * N * 8);
+    rhs_assemble_point(a.o, B, U, D, CV, &MT, P, rhs, p, N);
+  } else {
+    rhs_assemble_point(a.o, B, U, D, CV, nullptr, P, rhs, p, N);
+  }
+  if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k))
+    sommerfeld_point(B, U, P, u, rhs, i, j, k, N);
+}
+
+__global__ void __launch_bounds__(TILE) k_sommerfeld(RhsArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  if (!B.faces || !on_sommerfeld_face(B, i, j, k)) return;
+  const int N = B.npts;
+  const double* u = a.u + B.off * NVAR;
+  const Buf U = make_buf(u, (long long)NVAR * N * 8);
+  const Pt P = make_pt(flat(B, i, j, k), B.nn1, B.nn2, N);
+  sommerfeld_point(B, U, P, u, a.rhs + B.off * NVAR, i, j, k, N);
+}
+
+// ---------------------------------------------------------------------------
+// AlgConstr (src/z4c/z4c.cpp:677-991)
+// ---------------------------------------------------------------------------
+struct AlgArgs {
+  const DevBlock* blocks;
+  Launch L;
+  double* u;
+  double eps_floor;
+  int tf_force, force_reg;
+  int range[6];  // il,iu,jl,ju,kl,ku (ranged form) or all -1 = whole block
+  int skip_physical;
+};
+
+__device__ __forceinline__ void algconstr_point(double* __restrict__ u, int N,
+                                                int p, double eps_floor,
+                                                int tf_force, int force_reg) {
+  double g[6], A[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    g[ab] = u[(V_GXX + ab) * N + p];
+    A[ab] = u[(V_AXX + ab) * N + p];
+  }
+  double detg = det3(g);
+  bool project = true;
+  if (tf_force)
+    detg = detg > 0. ? detg : 1.;
+  else
+    project = detg > eps_floor;
+  if (project) {
+    const double oopsi4 = regularize_nth_root(detg, 1.0, eps_floor, -3.0);
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) g[ab] *= oopsi4;
+    const double tr = trace_rank2(1.0, g, A);
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) A[ab] -= (1.0 / 3.0) * tr * g[ab];
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      u[(V_GXX + ab) * N + p] = g[ab];
+      u[(V_AXX + ab) * N + p] = A[ab];
+    }
+  }
+  if (force_reg) {
+    u[V_CHI * N + p] = dmax(u[V_CHI * N + p], eps_floor);
+    u[V_ALPHA * N + p] = regularize_near_zero(u[V_ALPHA * N + p], eps_floor);
+  }
+}
+
+__global__ void __launch_bounds__(TILE) k_algconstr(AlgArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  if (a.range[0] >= 0) {
+    if (i < a.range[0] || i > a.range[1] || j < a.range[2] || j > a.range[3] ||
+        k < a.range[4] || k > a.range[5])
+      return;
+    if (a.skip_physical && i >= B.il && i <= B.iu && j >= B.jl && j <= B.ju &&
+        k >= B.kl && k <= B.ku)
+      return;
+  }
+  algconstr_point(a.u + B.off * NVAR, B.npts, flat(B, i, j, k), a.eps_floor,
+                  a.tf_force, a.force_reg);
+}
+
+// ---------------------------------------------------------------------------
+// ADMToZ4c (src/z4c/adm_z4c.cpp:46-192), Z4cToADM (:200-260),
+// PrepareAuxExtended (:1139-1231)
+// ---------------------------------------------------------------------------
+struct AdmArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* adm_in;
+  double* u;
+  double* adm_out;
+  double* aux_ext;
+  double chi_psi_power, chi_div_floor;
+};
+
+__device__ __forceinline__ void aux_ext_point(double pw, double floor,
+                                              const double* __restrict__ u,
+                                              const double* __restrict__ adm,
+                                              double* __restrict__ ax, int N,
+                                              int p) {
+  const double chi = fabs(u[V_CHI * N + p]);
+  const double chi_g = dmax(chi, floor);
+  const double sdg = (pw == -4.) ? 1.0 / (chi_g * sqrt(chi_g))
+                                 : pow(chi_g, (12.0 / pw) / 2.0);
+  ax[p] = sdg;
+  double g[6], gi[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) g[ab] = adm[(A_GXX + ab) * N + p];
+  inv3(1.0 / (sdg * sdg), g, gi);
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) ax[(1 + ab) * N + p] = gi[ab];
+}
+
+__global__ void __launch_bounds__(TILE) k_adm2z4c_point(AdmArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const double* adm = a.adm_in + B.off * NADM;
+  double* u = a.u + B.off * NVAR;
+  const double pw = a.chi_psi_power;
+  u[V_ALPHA * N + p] = adm[A_ALPHA * N + p];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) u[(V_BETAX + m) * N + p] = adm[(A_BETAX + m) * N + p];
+  double G[6], gt[6], Kt[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) G[ab] = adm[(A_GXX + ab) * N + p];
+  double detg = det3(G);
+  double oopsi4;
+  if (pw == -4.) {
+    oopsi4 = cbrt(1.0 / detg);
+    u[V_CHI * N + p] = oopsi4;
+  } else {
+    oopsi4 = pow(detg, -1. / 3.);
+    u[V_CHI * N + p] = pow(detg, 1. / 12. * pw);
+  }
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    gt[ab] = oopsi4 * G[ab];
+    Kt[ab] = oopsi4 * adm[(A_KXX + ab) * N + p];
+    u[(V_GXX + ab) * N + p] = gt[ab];
+  }
+  detg = det3(gt);
+  const double Khat = trace_rank2(1.0 / detg, gt, Kt);
+  u[V_KHAT * N + p] = Khat;
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) u[(V_AXX + ab) * N + p] = Kt[ab] - (1. / 3.) * Khat * gt[ab];
+}
+
+// Gamma~^a = -sum_b Dx_b g~^{ba} on the interior (zero in ghosts), Theta = 0.
+__global__ void __launch_bounds__(TILE) k_adm2z4c_gamma(AdmArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  double* u = a.u + B.off * NVAR;
+  // Only the Gamma and Theta slots are written here and they are not read,
+  // so neighbouring threads see a stable g~.
+  const double* ug = u;
+  const bool interior = i >= B.il && i <= B.iu && j >= B.jl && j <= B.ju &&
+                        k >= B.kl && k <= B.ku;
+  double gam[3] = {0.0, 0.0, 0.0};
+  if (interior) {
+    const Buf U = make_buf(ug, (long long)NVAR * N * 8);
+    const Pt P = make_pt(p, B.nn1, B.nn2, N);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        // g~^{dc} at the points -3..3 along d, then the 6th-order Dx
+        double v[7];
+#pragma unroll
+        for (int m = 0; m < 7; ++m) {
+          if (m == 3) { v[m] = 0.0; continue; }
+          double g[6], gi[6];
+#pragma unroll
+          for (int ab = 0; ab < 6; ++ab) g[ab] = ldm(U, P, V_GXX + ab, d, m - 3);
+          inv3(1.0 / det3(g), g, gi);
+          v[m] = gi[sym(d, c)];
+        }
+        double out = 0.0;
+        out += -1.0 * (v[0] - v[6]);
+        out += 9.0 * (v[1] - v[5]);
+        out += -45.0 * (v[2] - v[4]);
+        gam[c] -= out * B.cidx1[d];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[(V_GAMX + c) * N + p] = gam[c];
+  u[V_THETA * N + p] = 0.0;
+}
+
+__global__ void __launch_bounds__(TILE) k_aux_ext(AdmArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  aux_ext_point(a.chi_psi_power, a.chi_div_floor, a.u + B.off * NVAR,
+                a.adm_in + B.off * NADM, a.aux_ext + B.off * 7, B.npts,
+                flat(B, i, j, k));
+}
+
+__global__ void __launch_bounds__(TILE) k_z4c2adm(AdmArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const double* u = a.u + B.off * NVAR;
+  double* adm = a.adm_out + B.off * NADM;
+  adm[A_ALPHA * N + p] = u[V_ALPHA * N + p];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) adm[(A_BETAX + m) * N + p] = u[(V_BETAX + m) * N + p];
+  const double chi = u[V_CHI * N + p];
+  const double psi4 = (a.chi_psi_power == -4.) ? 1.0 / chi : pow(chi, 4. / a.chi_psi_power);
+  adm[A_PSI4 * N + p] = psi4;
+  const double th = u[V_THETA * N + p], kh = u[V_KHAT * N + p];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    const double G = psi4 * u[(V_GXX + ab) * N + p];
+    adm[(A_GXX + ab) * N + p] = G;
+    adm[(A_KXX + ab) * N + p] = psi4 * u[(V_AXX + ab) * N + p] + (1. / 3.) * (kh + 2. * th) * G;
+  }
+  if (a.aux_ext)
+    aux_ext_point(a.chi_psi_power, a.chi_div_floor, u, adm, a.aux_ext + B.off * 7, N, p);
+}
+
+// ---------------------------------------------------------------------------
+// ADMConstraints (src/z4c/adm_z4c.cpp:382-930), p = -4 and general branch
+// ---------------------------------------------------------------------------
+struct ConArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* u;
+  const double* adm;
+  const double* mat;
+  const double* d1;
+  double* con;
+  double chi_psi_power, chi_div_floor, r_max_con;
+};
+
+__global__ void __launch_bounds__(TILE) k_constraints(ConArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const double* u = a.u + B.off * NVAR;
+  const double* adm = a.adm + B.off * NADM;
+  const double* d1 = a.d1 + B.off * ND1;
+  double* con = a.con + B.off * NCON;
+#define DP(bb, f) (d1 + ((bb) * 11 + (f)) * N + p)
+  const Buf U = make_buf(u, (long long)NVAR * N * 8);
+  const Buf D = make_buf(d1, (long long)ND1 * N * 8);
+  const Buf AD = make_buf(adm, (long long)NADM * N * 8);
+  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  const double pw = a.chi_psi_power;
+  double gt[6], G[6], Kdd[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    gt[ab] = u[(V_GXX + ab) * N + p];
+    G[ab] = adm[(A_GXX + ab) * N + p];
+    Kdd[ab] = adm[(A_KXX + ab) * N + p];
+  }
+  double rho = 0.0, Sd[3] = {0, 0, 0};
+  if (a.mat) {
+    const double* mat = a.mat + B.off * NMAT;
+    rho = mat[M_RHO * N + p];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) Sd[m] = mat[(M_SX + m) * N + p];
+  }
+  const double chi_g = dmax(u[V_CHI * N + p], a.chi_div_floor);
+  const double oochi = 1.0 / chi_g;
+  const bool p4 = (pw == -4.);
+  const double fac = 4.0 / pw, fac2 = fac - 1.0;
+  const double psi4 = p4 ? 0.0 : pow(chi_g, fac);
+  double dchi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dchi[c] = *DP(c, F_CHI);
+  double dg[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab)
+      dg[c][ab] = p4 ? oochi * (*DP(c, F_G + ab) - oochi * gt[ab] * dchi[c])
+                     : psi4 * (*DP(c, F_G + ab) + fac * oochi * gt[ab] * dchi[c]);
+  double ddchi[6];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y)
+      ddchi[sym(x, y)] = (x == y) ? fd_dxx(U, P, V_CHI, x, B.cidx2[x])
+                                  : fd_dx(D, P, d1f(y, F_CHI), x, B.cidx1[x]);
+  double ddg[6][6];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y)
+#pragma unroll
+      for (int cd = 0; cd < 6; ++cd) {
+        const double dchi_a = dchi[x];
+        const double gtil = gt[cd];
+        double v;
+        if (x == y) {
+          const double ddg_conf = fd_dxx(U, P, V_GXX + cd, x, B.cidx2[x]);
+          v = p4 ? oochi * (ddg_conf -
+                            oochi * (2.0 * dchi_a * *DP(x, F_G + cd) + ddchi[sym(x, x)] * gtil) +
+                            2.0 * oochi * oochi * dchi_a * dchi_a * gtil)
+                 : psi4 * (ddg_conf +
+                           fac * oochi * (2.0 * dchi_a * *DP(x, F_G + cd) + ddchi[sym(x, x)] * gtil) +
+                           fac * fac2 * oochi * oochi * dchi_a * dchi_a * gtil);
+        } else {
+          const double dchi_b = dchi[y];
+          const double ddg_conf = fd_dx(D, P, d1f(y, F_G + cd), x, B.cidx1[x]);
+          v = p4 ? oochi * (ddg_conf -
+                            oochi * (dchi_a * *DP(y, F_G + cd) + dchi_b * *DP(x, F_G + cd) +
+                                     ddchi[sym(x, y)] * gtil) +
+                            2.0 * oochi * oochi * dchi_a * dchi_b * gtil)
+                 : psi4 * (ddg_conf +
+                           fac * oochi * (dchi_a * *DP(y, F_G + cd) + dchi_b * *DP(x, F_G + cd) +
+                                          ddchi[sym(x, y)] * gtil) +
+                           fac * fac2 * oochi * oochi * dchi_a * dchi_b * gtil);
+        }
+        ddg[sym(x, y)][cd] = v;
+      }
+  double guu[6];
+  inv3(1. / det3(G), G, guu);
+  double Gddd[3][6], Gudd[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = x; y < 3; ++y)
+        Gddd[c][sym(x, y)] = 0.5 * (dg[x][sym(y, c)] + dg[y][sym(x, c)] - dg[c][sym(x, y)]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      double t = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) t += guu[sym(c, d)] * Gddd[d][ab];
+      Gudd[c][ab] = t;
+    }
+  double R = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y) {
+      double r = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+            r += guu[sym(d, e)] * (Gudd[c][sym(x, e)] * Gddd[c][sym(y, d)] -
+                                   Gudd[c][sym(x, y)] * Gddd[c][sym(e, d)]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          r += 0.5 * guu[sym(c, d)] *
+               (-ddg[sym(y, x)][sym(c, d)] + ddg[sym(d, x)][sym(y, c)] +
+                ddg[sym(d, y)][sym(x, c)] - ddg[sym(d, c)][sym(x, y)]);
+      R += ((x == y) ? 1.0 : 2.0) * guu[sym(x, y)] * r;
+    }
+  double Kud[3][3], K = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+      double t = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) t += guu[sym(x, c)] * Kdd[sym(c, y)];
+      Kud[x][y] = t;
+    }
+    K += Kud[x][x];
+  }
+  double KK = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) KK += Kud[x][y] * Kud[y][x];
+  double DKudd[3][6];
+  {
+    double DKddd[3][6];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int c = y; c < 3; ++c) {
+          double t = fd_dx(AD, P, A_KXX + sym(y, c), x, B.cidx1[x]);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            t -= Gudd[d][sym(x, y)] * Kdd[sym(d, c)];
+            t -= Gudd[d][sym(x, c)] * Kdd[sym(y, d)];
+          }
+          DKddd[x][sym(y, c)] = t;
+        }
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int bc = 0; bc < 6; ++bc) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) t += guu[sym(x, d)] * DKddd[d][bc];
+        DKudd[x][bc] = t;
+      }
+  }
+  const double H = R + (K * K) - KK - 16 * M_PI * rho;
+  double Mu[3] = {0, 0, 0};
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+      Mu[x] -= 8 * M_PI * guu[sym(x, y)] * Sd[y];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Mu[x] += guu[sym(x, y)] * DKudd[c][sym(y, c)];
+        Mu[x] -= guu[sym(y, c)] * DKudd[x][sym(y, c)];
+      }
+    }
+  double Md[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    double t = 0;
+#pragma unroll
+    for (int y = 0; y < 3; ++y) t += G[sym(x, y)] * Mu[y];
+    Md[x] = t;
+  }
+  double M = 0.;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y)
+      M += ((x == y) ? 1.0 : 2.0) * G[sym(x, y)] * Mu[x] * Mu[y];
+  // conformal contracted Christoffel from the prepared conformal derivatives
+  double cguu[6];
+  inv3(1. / det3(gt), gt, cguu);
+  double cGu[3];
+  {
+    double cGddd[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = x; y < 3; ++y)
+          cGddd[c][sym(x, y)] = 0.5 * (*DP(x, F_G + sym(y, c)) + *DP(y, F_G + sym(x, c)) -
+                                       *DP(c, F_G + sym(x, y)));
+    double cGudd[3][6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int ab = 0; ab < 6; ++ab) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) t += cguu[sym(c, d)] * cGddd[d][ab];
+        cGudd[c][ab] = t;
+      }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int c = y; c < 3; ++c)
+          t += ((y == c) ? 1.0 : 2.0) * cguu[sym(y, c)] * cGudd[x][sym(y, c)];
+      cGu[x] = t;
+    }
+  }
+  double Zc = 0.;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y)
+      Zc += ((x == y) ? 0.25 : 0.5) * G[sym(x, y)] * (u[(V_GAMX + x) * N + p] - cGu[x]) *
+            (u[(V_GAMX + y) * N + p] - cGu[y]);
+  const double x1 = B.x0[0] + i * B.dx[0];
+  const double x2 = B.x0[1] + j * B.dx[1];
+  const double x3 = B.x0[2] + k * B.dx[2];
+  const double R2 = x1 * x1 + x2 * x2 + x3 * x3;
+  const double th = u[V_THETA * N + p];
+  const bool cut = R2 > a.r_max_con * a.r_max_con;
+  con[C_H * N + p] = cut ? 0.0 : H;
+  con[C_M * N + p] = cut ? 0.0 : M;
+  con[C_Z * N + p] = cut ? 0.0 : Zc;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) con[(C_MX + x) * N + p] = cut ? 0.0 : Md[x];
+  con[C_C * N + p] = cut ? 0.0 : (H * H) + M + (th * th) + 4.0 * Zc;
+#undef DP
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels
+// ---------------------------------------------------------------------------
+constexpr int MAXK = 8;
+struct BtArgs {
+  double* out;
+  const double* u0;
+  const double* k[MAXK];
+  double w[MAXK];  // already multiplied by dt; zero weights removed
+  int nk;
+  long long n;
+};
+
+__global__ void __launch_bounds__(256) k_bt_sum(BtArgs a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < a.n; e += stride) {
+    double v = a.u0[e];
+    for (int s = 0; s < a.nk; ++s) v += a.w[s] * a.k[s][e];
+    a.out[e] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill(double* p, double v, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) p[e] = v;
+}
+
+__global__ void __launch_bounds__(256) k_copy(double* dst, const double* src, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) dst[e] = src[e];
+}
+
+struct WaveArgs {
+  const DevBlock* blocks;
+  Launch L;
+  int nvar;
+  double* out;
+  const double* in1;
+  const double* in2;
+  double w[3];
+  int mode;  // 0 WeightedAve, 1 AddRHS (w[0] = weight, in1 = rhs)
+};
+
+// MeshBlock::WeightedAveVC / WeightedAve_ (src/mesh/weighted_ave.cpp:21-132)
+// and Z4c::AddZ4cRHS (src/z4c/add_z4c_rhs.cpp:24-34), interior vertices.
+__global__ void __launch_bounds__(TILE) k_wave(WaveArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const long long base = B.off * a.nvar;
+  const int p = flat(B, i, j, k);
+  const double w0 = a.w[0], w1 = a.w[1], w2 = a.w[2];
+  for (int n = 0; n < a.nvar; ++n) {
+    const long long e = base + (long long)n * N + p;
+    double* o = a.out + e;
+    if (a.mode == 1) {
+      *o += w0 * a.in1[e];
+      continue;
+    }
+    if (w0 == 1.0) {
+      if (w2 != 0.0) *o += w1 * a.in1[e] + w2 * a.in2[e];
+      else if (w1 != 0.0) *o += w1 * a.in1[e];
+    } else if (w0 == 0.0) {
+      if (w2 != 0.0) *o = w1 * a.in1[e] + w2 * a.in2[e];
+      else if (w1 == 1.0) *o = a.in1[e];
+      else *o = w1 * a.in1[e];
+    } else {
+      if (w2 != 0.0) *o = w0 * *o + w1 * a.in1[e] + w2 * a.in2[e];
+      else if (w1 != 0.0) *o = w0 * *o + w1 * a.in1[e];
+      else *o *= w0;
+    }
+  }
+}
+
+// Periodic VC self-exchange: ghosts take the periodic image, shared
+// boundary vertices the mean of their images (read from a snapshot).
+struct PerArgs {
+  const DevBlock* blocks;
+  Launch L;
+  int nvar;
+  double* u;
+  const double* snap;
+};
+
+__device__ __forceinline__ int images(int i, int lo, int hi, int* s) {
+  const int n = hi - lo;
+  if (i < lo) { s[0] = i + n; return 1; }
+  if (i > hi) { s[0] = i - n; return 1; }
+  if (i == lo) { s[0] = lo; s[1] = hi; return 2; }
+  if (i == hi) { s[0] = hi; s[1] = lo; return 2; }
+  s[0] = i;
+  return 1;
+}
+
+__global__ void __launch_bounds__(TILE) k_periodic(PerArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  int si[2], sj[2], sk[2];
+  const int ni = images(i, B.il, B.iu, si);
+  const int nj = images(j, B.jl, B.ju, sj);
+  const int nk = images(k, B.kl, B.ku, sk);
+  if (ni * nj * nk == 1 && si[0] == i && sj[0] == j && sk[0] == k) return;
+  const int N = B.npts;
+  const long long base = B.off * a.nvar;
+  for (int n = 0; n < a.nvar; ++n) {
+    const double* s = a.snap + base + (long long)n * N;
+    double acc = 0.0;
+    for (int x = 0; x < nk; ++x)
+      for (int y = 0; y < nj; ++y)
+        for (int z = 0; z < ni; ++z) acc += s[si[z] + B.nn1 * (sj[y] + B.nn2 * sk[x])];
+    a.u[base + (long long)n * N + flat(B, i, j, k)] = acc / (double)(ni * nj * nk);
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Context
+// ===========================================================================
+struct z4c_ctx {
+  int device = 0;
+  int nblocks = 0;
+  long long total_pts = 0;
+  std::vector<z4c_block> hblocks;
+  DevBlock* d_blocks = nullptr;
+  Tile* d_tiles[NBOX] = {nullptr, nullptr, nullptr};
+  Box* d_boxes[NBOX] = {nullptr, nullptr, nullptr};
+  int ntiles[NBOX] = {0, 0, 0};
+  double* d1 = nullptr;
+  double* cv = nullptr;    // curvature-pass scratch (12 fields)
+  double* snap = nullptr;  // periodic-fill snapshot (lazy)
+  long long snap_elems = 0;
+  int n_cu = 256;
+
+  void release() {
+    hipFree(d_blocks);
+    for (int b = 0; b < NBOX; ++b) {
+      hipFree(d_tiles[b]);
+      hipFree(d_boxes[b]);
+      d_tiles[b] = nullptr;
+      d_boxes[b] = nullptr;
+      ntiles[b] = 0;
+    }
+    hipFree(d1);
+    hipFree(cv);
+    hipFree(snap);
+    d_blocks = nullptr;
+    d1 = nullptr;
+    cv = nullptr;
+    snap = nullptr;
+    snap_elems = 0;
+  }
+};
+
+namespace {
+
+int grid_for(int ntiles) { return std::max(8, (ntiles + 7) / 8 * 8); }
+
+Launch launch_of(const z4c_ctx* c, int kind) {
+  return Launch{c->d_tiles[kind], c->d_boxes[kind], c->ntiles[kind]};
+}
+
+int check_ctx(const z4c_ctx* c) {
+  if (!c) return fail(Z4C_EINVAL, "null context");
+  if (c->nblocks <= 0) return fail(Z4C_ESTATE, "no blocks registered (call z4c_set_blocks)");
+  return Z4C_OK;
+}
+
+RhsParams rhs_params(const z4c_params* p) {
+  RhsParams o;
+  o.chi_psi_power = p->chi_psi_power;
+  o.chi_div_floor = p->chi_div_floor;
+  o.diss = p->diss;
+  o.damp_kappa1 = p->damp_kappa1;
+  o.damp_kappa2 = p->damp_kappa2;
+  o.lapse_oplog = p->lapse_oplog;
+  o.lapse_harmonicf = p->lapse_harmonicf;
+  o.lapse_harmonic = p->lapse_harmonic;
+  o.lapse_advect = p->lapse_advect;
+  o.lapse_K = p->lapse_K;
+  o.shift_Gamma = p->shift_Gamma;
+  o.shift_alpha2Gamma = p->shift_alpha2Gamma;
+  o.shift_H = p->shift_H;
+  o.shift_advect = p->shift_advect;
+  o.shift_eta = p->shift_eta;
+  return o;
+}
+
+int elementwise_grid(long long n) {
+  long long g = (n + 255) / 256;
+  return (int)std::min<long long>(g, 256LL * 64);
+}
+
+int fill(z4c_ctx* c, double* p, double v, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(elementwise_grid(n)), dim3(256), 0, s, p, v, n);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int launch_prep(z4c_ctx* c, const z4c_params* p, const double* u, double* aux,
+                hipStream_t s) {
+  PrepArgs a{c->d_blocks, launch_of(c, BOX_PREP), u, c->d1, aux, p->chi_psi_power,
+             p->chi_div_floor};
+  hipLaunchKernelGGL(k_prep, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
+               const double* mat, double* rhs, bool sommerfeld, hipStream_t s) {
+  if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->d1, mat, rhs, c->cv, rhs_params(p)};
+  const int g = grid_for(a.L.ntiles);
+  hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  if (sommerfeld)
+    hipLaunchKernelGGL(k_rhs_asm<true>, dim3(g), dim3(TILE), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_rhs_asm<false>, dim3(g), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int launch_alg(z4c_ctx* c, const z4c_params* p, double* u, const int* range,
+               int skip, hipStream_t s) {
+  AlgArgs a{};
+  a.blocks = c->d_blocks;
+  a.L = launch_of(c, BOX_WHOLE);
+  a.u = u;
+  a.eps_floor = p->eps_floor;
+  a.tf_force = p->tf_force_regularization;
+  a.force_reg = p->force_regularization;
+  for (int q = 0; q < 6; ++q) a.range[q] = range ? range[q] : -1;
+  a.skip_physical = skip;
+  hipLaunchKernelGGL(k_algconstr, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int launch_bt(z4c_ctx* c, int nvar, double* out, const double* u0, int nk,
+              const double* const* k, const double* w, double dt, hipStream_t s) {
+  if (nk < 0 || nk > MAXK) return fail(Z4C_EINVAL, "bt_sum: nk must be in [0, 8]");
+  BtArgs a{};
+  a.out = out;
+  a.u0 = u0;
+  a.n = c->total_pts * nvar;
+  int m = 0;
+  for (int j = 0; j < nk; ++j) {
+    if (w[j] != 0) {
+      if (!k[j]) return fail(Z4C_EINVAL, "bt_sum: null k pointer with nonzero weight");
+      a.k[m] = k[j];
+      a.w[m] = w[j] * dt;
+      ++m;
+    }
+  }
+  a.nk = m;
+  hipLaunchKernelGGL(k_bt_sum, dim3(elementwise_grid(a.n)), dim3(256), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int launch_periodic(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
+  const long long n = c->total_pts * nvar;
+  if (c->snap_elems < n) {
+    hipFree(c->snap);
+    c->snap = nullptr;
+    c->snap_elems = 0;
+    if (hipMalloc(&c->snap, sizeof(double) * n) != hipSuccess)
+      return fail(Z4C_ENOMEM, "periodic fill: snapshot allocation failed");
+    c->snap_elems = n;
+  }
+  hipLaunchKernelGGL(k_copy, dim3(elementwise_grid(n)), dim3(256), 0, s, c->snap, (const double*)u, n);
+  HIPCALL(hipGetLastError());
+  PerArgs a{c->d_blocks, launch_of(c, BOX_WHOLE), nvar, u, c->snap};
+  hipLaunchKernelGGL(k_periodic, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* z4c_last_error(void) { return g_err.c_str(); }
+int z4c_abi_version(void) { return Z4C_ABI_VERSION; }
+
+int z4c_ctx_create(z4c_ctx** out, int device) {
+  if (!out) return fail(Z4C_EINVAL, "null output pointer");
+  *out = nullptr;
+  int ndev = 0;
+  HIPCALL(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(Z4C_EINVAL, "device index out of range");
+  HIPCALL(hipSetDevice(device));
+  z4c_ctx* c = new z4c_ctx();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  *out = c;
+  return Z4C_OK;
+}
+
+int z4c_ctx_destroy(z4c_ctx* c) {
+  if (!c) return Z4C_OK;
+  hipSetDevice(c->device);
+  c->release();
+  delete c;
+  return Z4C_OK;
+}
+
+int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
+                   long long* elems_per_var) {
+  if (!c || !blocks || nblocks <= 0) return fail(Z4C_EINVAL, "set_blocks: bad arguments");
+  HIPCALL(hipSetDevice(c->device));
+  std::vector<DevBlock> hb(nblocks);
+  std::vector<Box> boxes[NBOX];
+  std::vector<Tile> tiles[NBOX];
+  long long off = 0;
+  const double diss_scaling = std::pow(2.0, -2 * NG) * (NG % 2 == 0 ? -1 : 1);
+  for (int b = 0; b < nblocks; ++b) {
+    const z4c_block& in = blocks[b];
+    if (in.offset != off)
+      return fail(Z4C_EINVAL, "set_blocks: block " + std::to_string(b) +
+                                  " offset must equal the sum of npts of earlier blocks");
+    const int nn[3] = {in.nn1, in.nn2, in.nn3};
+    const int lo[3] = {in.il, in.jl, in.kl}, hi[3] = {in.iu, in.ju, in.ku};
+    for (int d = 0; d < 3; ++d) {
+      if (lo[d] != NG || hi[d] != nn[d] - 1 - NG || hi[d] - lo[d] < 2)
+        return fail(Z4C_EINVAL, "set_blocks: block " + std::to_string(b) +
+                                    " must be vertex-centred with 4 ghosts and >= 3 interior vertices per axis");
+      if (!(in.dx[d] > 0.0)) return fail(Z4C_EINVAL, "set_blocks: dx must be positive");
+    }
+    DevBlock& B = hb[b];
+    B.off = off;
+    B.d1off = off * ND1;
+    B.nn1 = in.nn1; B.nn2 = in.nn2; B.nn3 = in.nn3;
+    B.il = in.il; B.iu = in.iu; B.jl = in.jl; B.ju = in.ju; B.kl = in.kl; B.ku = in.ku;
+    const long long npts = (long long)in.nn1 * in.nn2 * in.nn3;
+    if (npts * ND1 > std::numeric_limits<int>::max())
+      return fail(Z4C_EINVAL, "set_blocks: block too large");
+    B.npts = (int)npts;
+    B.faces = in.sommerfeld_faces & 63;
+    for (int d = 0; d < 3; ++d) {
+      const double dx = in.dx[d];
+      const double t = 1.0 / dx;
+      B.dx[d] = dx;
+      B.x0[d] = in.x0[d];
+      B.cidx1[d] = 1.0 / (dx * 60.0);
+      B.cidx2[d] = (t * t) / 180.0;
+      B.cidx1lo[d] = 1.0 / (dx * 2.0);
+      B.cidxd[d] = diss_scaling / (dx * 1.0);
+      B.lidx[d] = 1.0 / (dx * 60.0);
+    }
+    Box bi, bp, bw;
+    for (int d = 0; d < 3; ++d) {
+      bi.lo[d] = lo[d]; bi.n[d] = hi[d] - lo[d] + 1;
+      bp.lo[d] = lo[d] - (NG - 1); bp.n[d] = hi[d] - lo[d] + 1 + 2 * (NG - 1);
+      bw.lo[d] = 0; bw.n[d] = nn[d];
+    }
+    const Box bx[NBOX] = {bi, bp, bw};
+    for (int kind = 0; kind < NBOX; ++kind) {
+      boxes[kind].push_back(bx[kind]);
+      const int np = bx[kind].n[0] * bx[kind].n[1] * bx[kind].n[2];
+      for (int s = 0; s < np; s += TILE) tiles[kind].push_back(Tile{b, s});
+    }
+    off += npts;
+  }
+  c->release();
+  c->nblocks = 0;
+  if (hipMalloc(&c->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_blocks: allocation failed");
+  HIPCALL(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice));
+  for (int kind = 0; kind < NBOX; ++kind) {
+    if (hipMalloc(&c->d_tiles[kind], sizeof(Tile) * tiles[kind].size()) != hipSuccess ||
+        hipMalloc(&c->d_boxes[kind], sizeof(Box) * nblocks) != hipSuccess)
+      return fail(Z4C_ENOMEM, "set_blocks: allocation failed");
+    HIPCALL(hipMemcpy(c->d_tiles[kind], tiles[kind].data(), sizeof(Tile) * tiles[kind].size(),
+                      hipMemcpyHostToDevice));
+    HIPCALL(hipMemcpy(c->d_boxes[kind], boxes[kind].data(), sizeof(Box) * nblocks,
+                      hipMemcpyHostToDevice));
+    c->ntiles[kind] = (int)tiles[kind].size();
+  }
+  if (hipMalloc(&c->d1, sizeof(double) * off * ND1) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_blocks: derivative storage allocation failed");
+  // prepared derivatives start at zero (the reference zero-initialises them)
+  HIPCALL(hipMemset(c->d1, 0, sizeof(double) * off * ND1));
+  if (hipMalloc(&c->cv, sizeof(double) * off * NCV) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_blocks: curvature scratch allocation failed");
+  c->hblocks.assign(blocks, blocks + nblocks);
+  c->nblocks = nblocks;
+  c->total_pts = off;
+  if (elems_per_var) *elems_per_var = off;
+  return Z4C_OK;
+}
+
+int z4c_prepare_derivatives(z4c_ctx* c, const z4c_params* p, const double* u,
+                            double* aux, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u) return fail(Z4C_EINVAL, "prepare_derivatives: null argument");
+  return launch_prep(c, p, u, aux, (hipStream_t)stream);
+}
+
+int z4c_rhs(z4c_ctx* c, const z4c_params* p, const double* u, const double* mat,
+            double* rhs, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !rhs) return fail(Z4C_EINVAL, "rhs: null argument");
+  return launch_rhs(c, p, u, mat, rhs, false, (hipStream_t)stream);
+}
+
+int z4c_boundary_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
+                     double* rhs, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !rhs) return fail(Z4C_EINVAL, "boundary_rhs: null argument");
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->d1, nullptr, rhs, nullptr, rhs_params(p)};
+  hipLaunchKernelGGL(k_sommerfeld, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0,
+                     (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_algconstr(z4c_ctx* c, const z4c_params* p, double* u, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u) return fail(Z4C_EINVAL, "algconstr: null argument");
+  return launch_alg(c, p, u, nullptr, 0, (hipStream_t)stream);
+}
+
+int z4c_algconstr_range(z4c_ctx* c, const z4c_params* p, double* u, int il,
+                        int iu, int jl, int ju, int kl, int ku,
+                        int skip_physical, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u) return fail(Z4C_EINVAL, "algconstr_range: null argument");
+  if (il < 0 || jl < 0 || kl < 0) return fail(Z4C_EINVAL, "algconstr_range: negative bound");
+  const int r[6] = {il, iu, jl, ju, kl, ku};
+  return launch_alg(c, p, u, r, skip_physical, (hipStream_t)stream);
+}
+
+int z4c_adm_to_z4c(z4c_ctx* c, const z4c_params* p, const double* adm, double* u,
+                   double* aux_ext, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !adm || !u) return fail(Z4C_EINVAL, "adm_to_z4c: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  AdmArgs a{c->d_blocks, launch_of(c, BOX_WHOLE), adm, u, nullptr, aux_ext,
+            p->chi_psi_power, p->chi_div_floor};
+  const int g = grid_for(a.L.ntiles);
+  hipLaunchKernelGGL(k_adm2z4c_point, dim3(g), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adm2z4c_gamma, dim3(g), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  rc = launch_alg(c, p, u, nullptr, 0, s);
+  if (rc) return rc;
+  if (aux_ext) {
+    hipLaunchKernelGGL(k_aux_ext, dim3(g), dim3(TILE), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
+int z4c_z4c_to_adm(z4c_ctx* c, const z4c_params* p, const double* u, double* adm,
+                   double* aux_ext, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !adm || !u) return fail(Z4C_EINVAL, "z4c_to_adm: null argument");
+  AdmArgs a{c->d_blocks, launch_of(c, BOX_WHOLE), nullptr, const_cast<double*>(u), adm,
+            aux_ext, p->chi_psi_power, p->chi_div_floor};
+  hipLaunchKernelGGL(k_z4c2adm, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0,
+                     (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_adm_constraints(z4c_ctx* c, const z4c_params* p, const double* u,
+                        const double* adm, const double* mat, double* con,
+                        void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !adm || !con) return fail(Z4C_EINVAL, "adm_constraints: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  rc = fill(c, con, 0.0, c->total_pts * NCON, s);
+  if (rc) return rc;
+  ConArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, adm, mat, c->d1, con,
+            p->chi_psi_power, p->chi_div_floor, p->r_max_con};
+  hipLaunchKernelGGL(k_constraints, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_bt_sum(z4c_ctx* c, int nvar, double* u_out, const double* u0, int nk,
+               const double* const* k, const double* w, double dt, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u_out || !u0 || (nk > 0 && (!k || !w)) || nvar <= 0)
+    return fail(Z4C_EINVAL, "bt_sum: bad argument");
+  return launch_bt(c, nvar, u_out, u0, nk, k, w, dt, (hipStream_t)stream);
+}
+
+int z4c_weighted_ave(z4c_ctx* c, int nvar, double* u_out, const double* u_in1,
+                     const double* u_in2, const double* wght3, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u_out || !wght3 || nvar <= 0) return fail(Z4C_EINVAL, "weighted_ave: bad argument");
+  if ((wght3[1] != 0.0 && !u_in1) || (wght3[2] != 0.0 && !u_in2))
+    return fail(Z4C_EINVAL, "weighted_ave: null input with nonzero weight");
+  WaveArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), nvar, u_out, u_in1, u_in2,
+             {wght3[0], wght3[1], wght3[2]}, 0};
+  hipLaunchKernelGGL(k_wave, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_add_rhs(z4c_ctx* c, int nvar, const double* rhs, double wght, double* u_out,
+                void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u_out || !rhs || nvar <= 0) return fail(Z4C_EINVAL, "add_rhs: bad argument");
+  WaveArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), nvar, u_out, rhs, nullptr,
+             {wght, 0.0, 0.0}, 1};
+  hipLaunchKernelGGL(k_wave, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_new_dt(z4c_ctx* c, double cfl, double* dt_out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!dt_out) return fail(Z4C_EINVAL, "new_dt: null output");
+  double m = std::numeric_limits<double>::infinity();
+  for (const z4c_block& b : c->hblocks) {
+    double md = (double)std::numeric_limits<float>::max();
+    for (int d = 0; d < 3; ++d) md = std::min(md, b.dx[d]);
+    m = std::min(m, md * cfl);
+  }
+  *dt_out = m;
+  return Z4C_OK;
+}
+
+int z4c_periodic_fill(z4c_ctx* c, int nvar, double* u, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u || nvar <= 0) return fail(Z4C_EINVAL, "periodic_fill: bad argument");
+  return launch_periodic(c, nvar, u, (hipStream_t)stream);
+}
+
+int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double* u,
+                  const double* un, double* const* k, const double* mat,
+                  int periodic, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !un || !k || stage < 1 || stage > 4)
+    return fail(Z4C_EINVAL, "rk4_stage: bad argument");
+  for (int s = 0; s < stage; ++s)
+    if (!k[s]) return fail(Z4C_EINVAL, "rk4_stage: null k register");
+  hipStream_t s = (hipStream_t)stream;
+  // Butcher tableau of bt_rk4 (task_list_integrators_butcher.cpp:37-49)
+  static const double A[4][4] = {{0, 0, 0, 0}, {0.5, 0, 0, 0}, {0, 0.5, 0, 0}, {0, 0, 1, 0}};
+  static const double Bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  rc = launch_rhs(c, p, u, mat, k[stage - 1], true, s);
+  if (rc) return rc;
+  const double* kk[4] = {k[0], k[1], k[2], k[3]};
+  if (stage < 4)
+    rc = launch_bt(c, NVAR, u, un, stage, kk, A[stage], dt, s);
+  else
+    rc = launch_bt(c, NVAR, u, un, 4, kk, Bw, dt, s);
+  if (rc) return rc;
+  if (periodic) {
+    rc = launch_periodic(c, NVAR, u, s);
+    if (rc) return rc;
+  }
+  rc = launch_alg(c, p, u, nullptr, 0, s);
+  if (rc) return rc;
+  return launch_prep(c, p, u, nullptr, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+"""Host-side mirror of the reference's Z4c class surface (src/z4c/z4c.hpp:448-610)
+for a batch of vertex-centred MeshBlocks, on one MI355X.
+
+``Z4c`` owns one device context (z4c_ctx) and the block list; the method
+names follow the reference (Z4cRHS, AlgConstr, ADMToZ4c, Z4cToADM,
+ADMConstraints, PrepareZ4cDerivatives, ...), each acting on every block in
+one launch.  Device memory comes from torch (plain flat float64 tensors); the
+arithmetic is all in the HIP library behind the C-ABI.
+
+Buffer layout: a buffer of ``nvar`` fields is a flat tensor of
+nvar*total_points doubles; block b occupies
+[offset_b*nvar, (offset_b+npts_b)*nvar) as (nvar, nn3, nn2, nn1).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _capi
+from ._capi import Params, Z4C_NADM, Z4C_NCON, Z4C_NMAT, Z4C_NVAR, check
+
+NGHOST = 4
+
+# Butcher tableau of bt_rk4 (src/task_list/task_list_integrators_butcher.cpp:37-49)
+BT_RK4_A = [[0, 0, 0, 0], [0.5, 0, 0, 0], [0, 0.5, 0, 0], [0, 0, 1, 0]]
+BT_RK4_B = [1. / 6., 1. / 3., 1. / 3., 1. / 6.]
+
+
+@dataclass
+class MeshBlockGeom:
+    """Geometry of one VC MeshBlock: ``nx`` cells per axis (nx+1 owned
+    vertices), spacing ``dx`` and coordinate ``xmin`` of the first interior
+    vertex (src/mesh/meshblock.cpp:852-915)."""
+    nx: tuple
+    dx: tuple
+    xmin: tuple
+    level: int = 0
+    sommerfeld_faces: int = 0
+
+    @property
+    def nn(self):
+        return tuple(int(n) + 1 + 2 * NGHOST for n in self.nx)
+
+    @property
+    def npts(self):
+        a, b, c = self.nn
+        return a * b * c
+
+    @property
+    def shape(self):
+        return (self.nn[2], self.nn[1], self.nn[0])
+
+    def coords(self):
+        return tuple(self.xmin[d] - NGHOST * self.dx[d] + np.arange(self.nn[d]) * self.dx[d]
+                     for d in range(3))
+
+    @staticmethod
+    def cube(nx, dx, xmin, level=0, faces=0):
+        t = lambda v: tuple(float(x) for x in np.broadcast_to(v, (3,)))
+        return MeshBlockGeom(tuple(int(x) for x in np.broadcast_to(nx, (3,))), t(dx), t(xmin),
+                             level, faces)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise _capi.Z4cError("device buffers must be contiguous float64 CUDA tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class Z4c:
+    """Batched Z4c on a list of MeshBlocks (one device context)."""
+
+    def __init__(self, blocks, params: Params | None = None, device: int = 0):
+        if not torch.cuda.is_available():
+            raise _capi.Z4cError("no GPU visible: the Z4c hot path runs only on the HIP device")
+        self.L = _capi.load()
+        self.device = torch.device("cuda", device)
+        self.params = params if params is not None else Params()
+        self.blocks = list(blocks)
+        self._ctx = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.L.z4c_ctx_create(C.byref(self._ctx), device), "ctx_create")
+        arr = (_capi.Block * len(self.blocks))()
+        off = 0
+        self.offsets = []
+        for b, g in enumerate(self.blocks):
+            nn = g.nn
+            x = arr[b]
+            x.offset = off
+            x.nn1, x.nn2, x.nn3 = nn
+            x.il, x.iu = NGHOST, NGHOST + g.nx[0]
+            x.jl, x.ju = NGHOST, NGHOST + g.nx[1]
+            x.kl, x.ku = NGHOST, NGHOST + g.nx[2]
+            for d in range(3):
+                x.dx[d] = g.dx[d]
+                x.x0[d] = g.xmin[d] - NGHOST * g.dx[d]
+            x.level = g.level
+            x.sommerfeld_faces = g.sommerfeld_faces
+            self.offsets.append(off)
+            off += g.npts
+        total = C.c_longlong()
+        check(self.L.z4c_set_blocks(self._ctx, arr, len(self.blocks), C.byref(total)),
+              "set_blocks")
+        self.total_points = int(total.value)
+        assert self.total_points == off
+
+    def close(self):
+        if self._ctx:
+            self.L.z4c_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers ---------------------------------------------------------------
+    def empty(self, nvar=Z4C_NVAR):
+        return torch.empty(nvar * self.total_points, dtype=torch.float64, device=self.device)
+
+    def zeros(self, nvar=Z4C_NVAR):
+        return torch.zeros(nvar * self.total_points, dtype=torch.float64, device=self.device)
+
+    def block_view(self, buf, b, nvar):
+        g = self.blocks[b]
+        o = self.offsets[b] * nvar
+        return buf[o:o + nvar * g.npts].view((nvar,) + g.shape)
+
+    def upload(self, arrays, nvar=None):
+        """List of per-block numpy arrays (nvar, nn3, nn2, nn1) -> device buffer."""
+        nvar = nvar or arrays[0].shape[0]
+        host = np.concatenate([np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in arrays])
+        if host.size != nvar * self.total_points:
+            raise ValueError("arrays do not match the block list")
+        return torch.from_numpy(host).to(self.device)
+
+    def download(self, buf, nvar):
+        host = buf.detach().cpu().numpy()
+        return [host[self.offsets[b] * nvar:(self.offsets[b] + g.npts) * nvar].reshape((nvar,) + g.shape)
+                for b, g in enumerate(self.blocks)]
+
+    # -- the reference surface (src/z4c/z4c.hpp:448-610) ---------------------------
+    def PrepareZ4cDerivatives(self, u, aux=None, stream=None):
+        check(self.L.z4c_prepare_derivatives(self._ctx, C.byref(self.params), _ptr(u), _ptr(aux),
+                                             _stream(stream)), "prepare_derivatives")
+
+    def Z4cRHS(self, u, mat, rhs, stream=None):
+        check(self.L.z4c_rhs(self._ctx, C.byref(self.params), _ptr(u), _ptr(mat), _ptr(rhs),
+                             _stream(stream)), "rhs")
+
+    def Z4cBoundaryRHS(self, u, mat, rhs, stream=None):
+        check(self.L.z4c_boundary_rhs(self._ctx, C.byref(self.params), _ptr(u), _ptr(rhs),
+                                      _stream(stream)), "boundary_rhs")
+
+    def AlgConstr(self, u, rng=None, skip_physical=False, stream=None):
+        if rng is None:
+            check(self.L.z4c_algconstr(self._ctx, C.byref(self.params), _ptr(u), _stream(stream)),
+                  "algconstr")
+        else:
+            check(self.L.z4c_algconstr_range(self._ctx, C.byref(self.params), _ptr(u), *rng,
+                                             int(skip_physical), _stream(stream)), "algconstr_range")
+
+    def ADMToZ4c(self, adm, u, aux_ext=None, stream=None):
+        check(self.L.z4c_adm_to_z4c(self._ctx, C.byref(self.params), _ptr(adm), _ptr(u),
+                                    _ptr(aux_ext), _stream(stream)), "adm_to_z4c")
+
+    def Z4cToADM(self, u, adm, aux_ext=None, stream=None):
+        check(self.L.z4c_z4c_to_adm(self._ctx, C.byref(self.params), _ptr(u), _ptr(adm),
+                                    _ptr(aux_ext), _stream(stream)), "z4c_to_adm")
+
+    def ADMConstraints(self, con, adm, mat, u, stream=None):
+        check(self.L.z4c_adm_constraints(self._ctx, C.byref(self.params), _ptr(u), _ptr(adm),
+                                         _ptr(mat), _ptr(con), _stream(stream)), "adm_constraints")
+
+    def AddZ4cRHS(self, rhs, wght, u_out, nvar=Z4C_NVAR, stream=None):
+        check(self.L.z4c_add_rhs(self._ctx, nvar, _ptr(rhs), float(wght), _ptr(u_out),
+                                 _stream(stream)), "add_rhs")
+
+    def WeightedAve(self, u_out, u_in1, u_in2, wght, nvar=Z4C_NVAR, stream=None):
+        w = (C.c_double * 3)(*wght)
+        check(self.L.z4c_weighted_ave(self._ctx, nvar, _ptr(u_out), _ptr(u_in1), _ptr(u_in2), w,
+                                      _stream(stream)), "weighted_ave")
+
+    def NewBlockTimeStep(self, cfl):
+        out = C.c_double()
+        check(self.L.z4c_new_dt(self._ctx, float(cfl), C.byref(out)), "new_dt")
+        return out.value
+
+    # -- integrator pieces ------------------------------------------------------------
+    def bt_sum(self, u_out, u0, ks, ws, dt, nvar=Z4C_NVAR, stream=None):
+        """Butcher SumBT_ak / SumBT_bk."""
+        kp = (C.c_void_p * max(1, len(ks)))(*[_ptr(k) for k in ks])
+        w = (C.c_double * max(1, len(ws)))(*ws)
+        check(self.L.z4c_bt_sum(self._ctx, nvar, _ptr(u_out), _ptr(u0), len(ks), kp, w, float(dt),
+                                _stream(stream)), "bt_sum")
+
+    def periodic_fill(self, u, nvar=Z4C_NVAR, stream=None):
+        check(self.L.z4c_periodic_fill(self._ctx, nvar, _ptr(u), _stream(stream)), "periodic_fill")
+
+    def rk4_stage(self, stage, dt, u, un, k, mat=None, periodic=False, stream=None):
+        kp = (C.c_void_p * 4)(*[_ptr(x) for x in k])
+        check(self.L.z4c_rk4_stage(self._ctx, C.byref(self.params), int(stage), float(dt), _ptr(u),
+                                   _ptr(un), kp, _ptr(mat), int(periodic), _stream(stream)),
+              "rk4_stage")
+
+    def rk4_step(self, dt, u, un, k, mat=None, periodic=False, stream=None):
+        """One bt_rk4 step in the GR_Z4c stage order; derivatives for ``u``
+        must be prepared (PrepareZ4cDerivatives) before the first step; each
+        stage re-prepares them for the next."""
+        un.copy_(u)  # Butcher::PutScratchBT_u at stage 1
+        for s in range(1, 5):
+            self.rk4_stage(s, dt, u, un, k, mat, periodic, stream)
+
+    def make_rk_registers(self):
+        """u_n and the four k registers, zero-initialised (k ghosts stay 0)."""
+        return self.zeros(), [self.zeros() for _ in range(4)]

@@ -1,0 +1,184 @@
+/*
+ * z4c_mi355x.h -- C-ABI of the MI355X (gfx950) Z4c evolution hot path.
+ *
+ * Drop-in for the per-MeshBlock Z4c class surface of GR-Athena++
+ * (src/z4c/z4c.hpp:448-610) on vertex-centred MeshBlocks, batched: every
+ * call acts on all blocks registered in the context in one launch.
+ *
+ * Conventions
+ *  - All field pointers are DEVICE pointers (hipMalloc'd, FP64).  A "state
+ *    buffer" holds every registered block back to back; block b starts at
+ *    element offset blocks[b].offset and stores nvar fields in the
+ *    reference's AthenaArray order u(n,k,j,i) = base[i + nn1*(j + nn2*(k +
+ *    nn3*n))] (src/athena_arrays.hpp:192-193).  The same offsets index every
+ *    state-shaped buffer (u, rhs, RK registers, adm, con, mat).
+ *  - Z4c variable order chi, g~(6), Khat, A~(6), Gam~(3), Theta, alpha,
+ *    beta(3) (src/z4c/z4c.hpp:54-88); ADM order src/z4c/z4c.hpp:99-135;
+ *    constraint order C,H,M,Z,Mx,My,Mz (src/z4c/z4c.hpp:140-160).
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *    Calls are stream ordered and never synchronise the host unless stated.
+ *  - Every function returns 0 on success or a negative Z4C_E* code; the
+ *    message is available from z4c_last_error().  Nothing throws across the
+ *    ABI.  A context is used by one host thread at a time (the reference's
+ *    Z4c objects are not re-entrant either, src/z4c/z4c.hpp:613-698).
+ */
+#ifndef Z4C_MI355X_H_
+#define Z4C_MI355X_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define Z4C_ABI_VERSION 1
+
+enum {
+  Z4C_OK = 0,
+  Z4C_EINVAL = -1,   /* bad argument (null pointer, size, range)   */
+  Z4C_EHIP = -2,     /* a HIP runtime call failed                   */
+  Z4C_ESTATE = -3,   /* call out of order (e.g. no blocks set)      */
+  Z4C_ENOMEM = -4    /* device allocation failed                    */
+};
+
+enum { Z4C_NVAR = 22, Z4C_NADM = 17, Z4C_NCON = 7, Z4C_NMAT = 10 };
+
+/* Z4c::opt (src/z4c/z4c.hpp:374-430); defaults src/z4c/z4c.cpp:318-472. */
+typedef struct {
+  double chi_psi_power;      /* -4 */
+  double chi_div_floor;      /* -1000 */
+  double diss;               /* 0 */
+  double eps_floor;          /* 1e-12 */
+  double damp_kappa1;        /* 0 */
+  double damp_kappa2;        /* 0 */
+  double lapse_oplog;        /* 2 */
+  double lapse_harmonicf;    /* 1 */
+  double lapse_harmonic;     /* 0 */
+  double lapse_advect;       /* 1 */
+  double lapse_K;            /* 0 */
+  double shift_Gamma;        /* 1 */
+  double shift_alpha2Gamma;  /* 0 */
+  double shift_H;            /* 0 */
+  double shift_advect;       /* 1 */
+  double shift_eta;          /* 2 */
+  double r_max_con;          /* +inf */
+  int cowling;               /* 0 */
+  int tf_force_regularization; /* 1 */
+  int force_regularization;  /* 0 */
+  int pad_;
+} z4c_params;
+
+/* One vertex-centred MeshBlock (MB_info, src/mesh/mesh.hpp:817-829, and the
+   VC ranges of src/mesh/meshblock.cpp:869-889): nn = nx + 1 + 2*4. */
+typedef struct {
+  long long offset;            /* element offset of the block in a state buffer */
+  int nn1, nn2, nn3;           /* padded vertex counts per axis                 */
+  int il, iu, jl, ju, kl, ku;  /* interior vertex ranges (inclusive)            */
+  double dx[3];                /* uniform spacing                               */
+  double x0[3];                /* coordinate of padded index 0 on each axis     */
+  int level;                   /* refinement level (bookkeeping only)           */
+  int sommerfeld_faces;        /* bit f set: face f (ix1,ox1,ix2,ox2,ix3,ox3)
+                                  carries gr_sommerfeld (Z4cBoundaryRHS)       */
+} z4c_block;
+
+typedef struct z4c_ctx z4c_ctx;
+
+/* Last error message of the calling thread (static storage). */
+const char* z4c_last_error(void);
+int z4c_abi_version(void);
+
+/* Context on HIP device `device`.  Replaces the per-block Z4c constructor's
+   allocation of derivative storage (src/z4c/z4c.cpp:475-550). */
+int z4c_ctx_create(z4c_ctx** ctx, int device);
+int z4c_ctx_destroy(z4c_ctx* ctx);
+
+/* Register the block list (host array).  Allocates the per-block prepared
+   derivative storage and the launch tables.  Replaces the MeshBlock loop of
+   TaskList::DoTaskListOneStage (src/task_list/task_list.cpp:122-261).
+   Returns the number of doubles one state buffer of `nvar` fields needs in
+   *elems_per_var (sum of nn1*nn2*nn3 over blocks; multiply by nvar). */
+int z4c_set_blocks(z4c_ctx* ctx, const z4c_block* blocks, int nblocks,
+                   long long* elems_per_var);
+
+/* Z4c::PrepareZ4cDerivatives (src/z4c/adm_z4c.cpp:972-1120): first
+   derivatives of alpha, chi, beta^i, g~_ij on the extended ranges, kept in
+   the context for the next z4c_rhs / z4c_adm_constraints.  `aux` (may be
+   NULL) receives the 30 physical ADM derivatives (storage.aux, interior). */
+int z4c_prepare_derivatives(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                            double* aux, void* stream);
+
+/* Z4c::Z4cRHS (src/z4c/calculate_z4c_rhs.cpp:29-916) incl. KO dissipation,
+   interior points only, from the derivatives prepared for `u`.  `mat` (10
+   fields, may be NULL for vacuum).  Ghosts of rhs are not written. */
+int z4c_rhs(z4c_ctx* ctx, const z4c_params* p, const double* u,
+            const double* mat, double* rhs, void* stream);
+
+/* Z4c::Z4cBoundaryRHS (src/z4c/boundary_conditions.cpp:16-174): Sommerfeld
+   conditions on the faces flagged in z4c_block.sommerfeld_faces. */
+int z4c_boundary_rhs(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                     double* rhs, void* stream);
+
+/* Z4c::AlgConstr (src/z4c/z4c.cpp:677-799), whole padded block, in place. */
+int z4c_algconstr(z4c_ctx* ctx, const z4c_params* p, double* u, void* stream);
+
+/* Z4c::AlgConstr ranged form (src/z4c/z4c.cpp:801-991): the same index
+   ranges on every block; skip_physical != 0 leaves interior points alone. */
+int z4c_algconstr_range(z4c_ctx* ctx, const z4c_params* p, double* u, int il,
+                        int iu, int jl, int ju, int kl, int ku,
+                        int skip_physical, void* stream);
+
+/* Z4c::ADMToZ4c (src/z4c/adm_z4c.cpp:46-192) incl. the closing AlgConstr.
+   `aux_ext` (7 fields: sqrt det gamma, gamma^ij; may be NULL) receives
+   PrepareAuxExtended (src/z4c/adm_z4c.cpp:1139-1231). */
+int z4c_adm_to_z4c(z4c_ctx* ctx, const z4c_params* p, const double* adm,
+                   double* u, double* aux_ext, void* stream);
+
+/* Z4c::Z4cToADM (src/z4c/adm_z4c.cpp:200-260), whole block. */
+int z4c_z4c_to_adm(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                   double* adm, double* aux_ext, void* stream);
+
+/* Z4c::ADMConstraints (src/z4c/adm_z4c.cpp:382-930): con (7 fields) zeroed
+   then set on the interior; needs derivatives prepared for `u`. */
+int z4c_adm_constraints(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                        const double* adm, const double* mat, double* con,
+                        void* stream);
+
+/* Butcher::SumBT_ak / SumBT_bk (src/task_list/task_list_integrators_butcher.cpp:573-680):
+   u_out = u0 + sum_{j<nk, w[j]!=0} (w[j]*dt) k[j], whole padded block, j in
+   order.  `k` is a HOST array of nk device pointers; nk <= 8.  u_out may
+   alias u0. */
+int z4c_bt_sum(z4c_ctx* ctx, int nvar, double* u_out, const double* u0, int nk,
+               const double* const* k, const double* w, double dt,
+               void* stream);
+
+/* MeshBlock::WeightedAveVC (src/mesh/weighted_ave.cpp:21-167, interior
+   vertices) and Z4c::AddZ4cRHS (src/z4c/add_z4c_rhs.cpp:24-34). */
+int z4c_weighted_ave(z4c_ctx* ctx, int nvar, double* u_out, const double* u_in1,
+                     const double* u_in2, const double* wght3, void* stream);
+int z4c_add_rhs(z4c_ctx* ctx, int nvar, const double* rhs, double wght,
+                double* u_out, void* stream);
+
+/* Z4c::NewBlockTimeStep (src/z4c/new_blockdt_z4c.cpp:24-70): minimum over
+   registered blocks of cfl*min(dx); host-side, no device work. */
+int z4c_new_dt(z4c_ctx* ctx, double cfl, double* dt_out);
+
+/* Single-block periodic vertex-centred ghost exchange (the periodic
+   self-neighbour case of CommRegistry::SendBoundaryBuffers/SetBoundaries with
+   additive VC unpack and node-multiplicity division,
+   src/comm/comm_channel.cpp:1277-1330, src/comm/index_utilities.hpp:116-139).
+   Applies to every registered block independently. */
+int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
+
+/* One bt_rk4 stage of the GR_Z4c task list
+   (src/task_list/gr/task_list_gr_z4c.cpp:45-89, stage s = 1..4) on all
+   blocks: k[s-1] = RHS(u_s) (+ Sommerfeld), u_{s+1} = u_n + sum a dt k
+   (SumBT_ak / SumBT_bk), optional periodic ghost fill, AlgConstr,
+   PrepareZ4cDerivatives.  `un` holds u at the start of the step; `u` is
+   overwritten with the next stage state; k[4] are device pointers (host
+   array). */
+int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
+                  double* u, const double* un, double* const* k,
+                  const double* mat, int periodic, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* Z4C_MI355X_H_ */
