@@ -219,11 +219,13 @@ __global__ void __launch_bounds__(TILE) k_rhs_asm(RhsArgs a) {
   const Buf CV = make_buf(a.cv + B.off * NCV, (long long)NCV * N * 8);
   const Pt P = make_pt(p, B.nn1, B.nn2, N);
   double* rhs = a.rhs + B.off * NVAR;
+  // The matter buffer (vacuum: absent) is passed by value: a buffer
+  // resource must stay in SGPRs, never behind a pointer.
   if (a.mat) {
     const Buf MT = make_buf(a.mat + B.off * NMAT, (long long)NMAT * N * 8);
-    rhs_assemble_point(a.o, B, U, D, CV, &MT, P, rhs, p, N);
+    rhs_assemble_point(a.o, B, U, D, CV, MT, true, P, rhs, p, N);
   } else {
-    rhs_assemble_point(a.o, B, U, D, CV, nullptr, P, rhs, p, N);
+    rhs_assemble_point(a.o, B, U, D, CV, U, false, P, rhs, p, N);
   }
   if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k))
     sommerfeld_point(B, U, P, u, rhs, i, j, k, N);
@@ -1132,6 +1134,7 @@ int z4c_rhs(z4c_ctx* c, const z4c_params* p, const double* u, const double* mat,
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !rhs) return fail(Z4C_EINVAL, "rhs: null argument");
+  if (mat) return fail(Z4C_EINVAL, "rhs: matter sources are not enabled in this build (vacuum only)");
   return launch_rhs(c, p, u, mat, rhs, false, (hipStream_t)stream);
 }
 
@@ -1279,6 +1282,7 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   if (rc) return rc;
   if (!p || !u || !un || !k || stage < 1 || stage > 4)
     return fail(Z4C_EINVAL, "rk4_stage: bad argument");
+  if (mat) return fail(Z4C_EINVAL, "rk4_stage: matter sources are not enabled in this build (vacuum only)");
   for (int s = 0; s < stage; ++s)
     if (!k[s]) return fail(Z4C_EINVAL, "rk4_stage: null k register");
   hipStream_t s = (hipStream_t)stream;

@@ -186,7 +186,7 @@ __device__ __forceinline__ void rhs_curv_point(const RhsParams& o, const DevBloc
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const DevBlock& B,
                                                    const Buf& U, const Buf& D,
-                                                   const Buf& CV, const Buf* MT,
+                                                   const Buf& CV, const Buf MT, bool has_mat,
                                                    const Pt& P, double* __restrict__ rhs,
                                                    int p, int N) {
   const double pw = o.chi_psi_power;
@@ -210,12 +210,12 @@ __device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const Dev
 
   // matter (zero in vacuum) and its trace (:466-476)
   double rho = 0.0, Sd[3] = {0.0, 0.0, 0.0}, Sdd[6] = {0, 0, 0, 0, 0, 0};
-  if (MT) {
-    rho = ldc(*MT, P, M_RHO);
+  if (has_mat) {
+    rho = ldc(MT, P, M_RHO);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) Sd[a] = ldc(*MT, P, M_SX + a);
+    for (int a = 0; a < 3; ++a) Sd[a] = ldc(MT, P, M_SX + a);
 #pragma unroll
-    for (int ab = 0; ab < 6; ++ab) Sdd[ab] = ldc(*MT, P, M_SXX + ab);
+    for (int ab = 0; ab < 6; ++ab) Sdd[ab] = ldc(MT, P, M_SXX + ab);
   }
   double Str = 0.0;
 #pragma unroll

@@ -68,10 +68,17 @@ def gauge_wave_adm(geom: MeshBlockGeom, A: float = 0.01, d: float = 1.0):
     return adm
 
 
-def brill_lindquist_adm(geom: MeshBlockGeom, m=(0.5, 0.5), sep: float = 10.0):
-    """Two punctures on the x axis at +-sep/2: psi = 1 + m1/(2 r1) + m2/(2 r2),
-    gamma = psi^4 delta, K = 0, alpha = psi^-2, beta = 0."""
+# Puncture offset: half the finest spacing of the octree (h0 = 1, level 7),
+# so that no vertex of any level coincides with a puncture.
+PUNCTURE_OFFSET = 0.5 * 2.0 ** -7
+
+
+def brill_lindquist_adm(geom: MeshBlockGeom, m=(0.5, 0.5), sep: float = 10.0,
+                        offset: float = PUNCTURE_OFFSET):
+    """Two punctures at (+-sep/2, 0, 0) + offset*(1,1,1): psi = 1 + m1/(2 r1)
+    + m2/(2 r2), gamma = psi^4 delta, K = 0, alpha = psi^-2, beta = 0."""
     X, Y, Z = _mesh(geom)
+    X, Y, Z = X - offset, Y - offset, Z - offset
     r1 = np.sqrt((X - sep / 2) ** 2 + Y ** 2 + Z ** 2)
     r2 = np.sqrt((X + sep / 2) ** 2 + Y ** 2 + Z ** 2)
     psi = 1.0 + m[0] / (2.0 * r1) + m[1] / (2.0 * r2)
