@@ -19,7 +19,6 @@ namespace z4c {
 constexpr int NG = 4;
 constexpr int NVAR = 22;
 constexpr int ND1 = 33;  // d(alpha, chi, beta^c, g~_cd)/dx^b, index b*11 + f
-constexpr int NCV = 12;  // curvature pass: R~_ab + R^phi_ab, D_a D_b alpha
 
 // Z4c variable indices (src/z4c/z4c.hpp:54-88)
 enum : int {
@@ -33,6 +32,8 @@ enum : int { M_RHO = 0, M_SX = 1, M_SXX = 4, NMAT = 10 };
 // prepared first-derivative fields, per direction b: f = 0 alpha, 1 chi,
 // 2..4 beta^x..z, 5..10 g~_xx..g~_zz
 enum : int { F_ALPHA = 0, F_CHI = 1, F_BETA = 2, F_G = 5 };
+// prepared-derivative field index of direction b, field f
+__host__ __device__ constexpr int d1f(int b, int f) { return b * 11 + f; }
 
 // SYM2 component order xx,xy,xz,yy,yz,zz (src/athena_tensor.hpp:57-179)
 __host__ __device__ constexpr int sym(int a, int b) {
@@ -42,11 +43,13 @@ __host__ __device__ constexpr int sym(int a, int b) {
 
 // Device-side description of one MeshBlock.
 struct DevBlock {
-  long long off;     // element offset in state buffers (per-block [nvar][npts])
-  long long d1off;   // element offset in the prepared-derivative buffer
+  long long off;     // point offset in state buffers (block data at off*nvar)
+  long long ioff;    // interior-point offset in interior-indexed buffers
   int nn1, nn2, nn3;
   int il, iu, jl, ju, kl, ku;
   int npts;          // nn1*nn2*nn3
+  int ni, nj, nk;    // interior extents
+  int nint;          // ni*nj*nk
   int faces;         // Sommerfeld face bits
   double dx[3];
   double x0[3];
@@ -55,6 +58,44 @@ struct DevBlock {
   double cidx1lo[3]; // 1/(dx*2)         Ds
   double cidxd[3];   // -2^-8/(dx*1)     Diss (times diss)
   double lidx[3];    // 1/(dx*60)        Lx
+};
+
+// ---------------------------------------------------------------------------
+// Derivative slots per interior point, written by the stencil pass and read
+// by the algebra pass (field stride = interior points of the block).
+//   FULL fields (chi, g~_ij, alpha, beta^i): d_a (3), d_a d_b (6, SYM2
+//        order; off-diagonal = nested Dx(a, Dx(b, .)) with a < b), L, D
+//   GRAD fields (Khat, Gam~^i, Theta):       d_a (3), L, D
+//   ADV  fields (A~_ij):                      L, D
+// L = sum_a Lx(a, beta^a, .) in a order; D = (Diss_x + Diss_y) + Diss_z.
+// ---------------------------------------------------------------------------
+enum : int { DK_FULL = 0, DK_GRAD = 1, DK_ADV = 2 };
+__host__ __device__ constexpr int dkind(int n) {
+  return (n == V_CHI || (n >= V_GXX && n < V_GXX + 6) || n == V_ALPHA || n >= V_BETAX)
+             ? DK_FULL
+             : ((n == V_KHAT || n == V_THETA || (n >= V_GAMX && n < V_GAMX + 3)) ? DK_GRAD
+                                                                                 : DK_ADV);
+}
+__host__ __device__ constexpr int dnslots(int kind) {
+  return kind == DK_FULL ? 11 : (kind == DK_GRAD ? 5 : 2);
+}
+__host__ __device__ constexpr int dbase(int n) {
+  int s = 0;
+  for (int m = 0; m < n; ++m) s += dnslots(dkind(m));
+  return s;
+}
+constexpr int NDV = 158;
+static_assert(dbase(NVAR) == NDV, "derivative slot layout");
+__host__ __device__ constexpr int s_d(int n, int a) { return dbase(n) + a; }
+__host__ __device__ constexpr int s_dd(int n, int ab) { return dbase(n) + 3 + ab; }
+__host__ __device__ constexpr int s_L(int n) { return dbase(n) + dnslots(dkind(n)) - 2; }
+__host__ __device__ constexpr int s_D(int n) { return dbase(n) + dnslots(dkind(n)) - 1; }
+
+// Tile of the interior x-y plane of a block for the z-marching stencil pass
+// (interior-relative origin i0, j0 and extents ni, nj).
+struct PlaneTile {
+  int block;
+  int i0, j0, ni, nj;
 };
 
 // A tile = up to TILE consecutive points of one box of one block.
@@ -199,6 +240,54 @@ __device__ __forceinline__ double fd_lx(const Buf& b, const Pt& P, int n, int d,
   dr += -35.0 * u0;
   dr += -24.0 * um1;
   dr += 2.0 * um2;
+  return ((vx < 0) ? (vx * dl) : (vx * dr)) * lidx;
+}
+
+// ---------------------------------------------------------------------------
+// The same stencils on 9 gathered values v[0..8] (offsets -4..+4), used by
+// the LDS stencil pass.  Identical operations in identical order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double st_dx(const double* v, double cidx) {
+  double out = 0.0;
+  out += -1.0 * (v[1] - v[7]);
+  out += 9.0 * (v[2] - v[6]);
+  out += -45.0 * (v[3] - v[5]);
+  return out * cidx;
+}
+__device__ __forceinline__ double st_dxx(const double* v, double cidx2) {
+  double out = 0.0;
+  out += 2.0 * (v[1] + v[7]);
+  out += -27.0 * (v[2] + v[6]);
+  out += 270.0 * (v[3] + v[5]);
+  out += -490.0 * v[4];
+  return out * cidx2;
+}
+__device__ __forceinline__ double st_diss(const double* v, double cidxd, double diss) {
+  double out = 0.0;
+  out += 1.0 * (v[0] + v[8]);
+  out += -8.0 * (v[1] + v[7]);
+  out += 28.0 * (v[2] + v[6]);
+  out += -56.0 * (v[3] + v[5]);
+  out += 70.0 * v[4];
+  return out * cidxd * diss;
+}
+__device__ __forceinline__ double st_lx(const double* v, double lidx, double vx) {
+  double dl = 0.0;
+  dl += 1.0 * v[0];
+  dl += -8.0 * v[1];
+  dl += 30.0 * v[2];
+  dl += -80.0 * v[3];
+  dl += 35.0 * v[4];
+  dl += 24.0 * v[5];
+  dl += -2.0 * v[6];
+  double dr = 0.0;
+  dr += -1.0 * v[8];
+  dr += 8.0 * v[7];
+  dr += -30.0 * v[6];
+  dr += 80.0 * v[5];
+  dr += -35.0 * v[4];
+  dr += -24.0 * v[3];
+  dr += 2.0 * v[2];
   return ((vx < 0) ? (vx * dl) : (vx * dr)) * lidx;
 }
 

@@ -22,6 +22,7 @@
 #include "z4c_device.hpp"
 #include "z4c_mi355x.h"
 #include "z4c_rhs_point.hpp"
+#include "z4c_stencil.hpp"
 
 using namespace z4c;
 
@@ -144,10 +145,10 @@ struct RhsArgs {
   const DevBlock* blocks;
   Launch L;
   const double* u;
-  const double* d1;
+  const double* dv;   // derivative slots from the stencil pass
   const double* mat;  // may be null
   double* rhs;
-  double* cv;         // curvature-pass output, 12 fields
+  double* cv;         // curvature part, 12 fields per interior point
   RhsParams o;
 };
 
@@ -194,41 +195,62 @@ __device__ __forceinline__ bool on_sommerfeld_face(const DevBlock& B, int i,
          ((f & 16) && k == B.kl) || ((f & 32) && k == B.ku);
 }
 
+// Stencil pass: grid (tiles, 22 fields), z-marching over the tile.
+__global__ void __launch_bounds__(ST_THREADS) k_stencil(StencilArgs a) {
+  extern __shared__ double sm[];
+  const int G = gridDim.x;  // multiple of 8: XCD-contiguous tile ranges
+  const int w = blockIdx.x;
+  const int t = (w & 7) * (G >> 3) + (w >> 3);
+  if (t >= a.ntiles) return;
+  stencil_tile(a, a.tiles[t], (int)blockIdx.y, sm);
+}
+
+__device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B, int p,
+                                            int qi) {
+  PointIO X;
+  const int N = B.npts;
+  X.U = make_buf(a.u + B.off * NVAR, (long long)NVAR * N * 8);
+  X.DV = make_buf(a.dv + B.ioff * NDV, (long long)NDV * B.nint * 8);
+  X.has_mat = a.mat != nullptr;
+  X.MT = X.has_mat ? make_buf(a.mat + B.off * NMAT, (long long)NMAT * N * 8) : X.U;
+  X.vu = (unsigned)p * 8u;
+  X.vd = (unsigned)qi * 8u;
+  X.fu = N * 8;
+  X.fd = B.nint * 8;
+  return X;
+}
+
+// Algebra pass, curvature part: one thread per interior point.
 __global__ void __launch_bounds__(TILE) k_rhs_curv(RhsArgs a) {
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
-  const int N = B.npts;
-  const int p = flat(B, i, j, k);
-  const Buf U = make_buf(a.u + B.off * NVAR, (long long)NVAR * N * 8);
-  const Buf D = make_buf(a.d1 + B.off * ND1, (long long)ND1 * N * 8);
-  const Pt P = make_pt(p, B.nn1, B.nn2, N);
-  rhs_curv_point(a.o, B, U, D, P, a.cv + B.off * NCV, p, N);
+  const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
+  const PointIO X = point_io(a, B, flat(B, i, j, k), qi);
+  double* cv = a.cv + B.ioff * 12;
+  const int NI = B.nint;
+  rhs_curvature_point(a.o, X, [&](int n, double v) { cv[n * NI + qi] = v; });
 }
 
+// Algebra pass, assembly part.
 template <bool WITH_SOMMERFELD>
-__global__ void __launch_bounds__(TILE) k_rhs_asm(RhsArgs a) {
+__global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
   const int N = B.npts;
   const int p = flat(B, i, j, k);
-  const double* u = a.u + B.off * NVAR;
-  const Buf U = make_buf(u, (long long)NVAR * N * 8);
-  const Buf D = make_buf(a.d1 + B.off * ND1, (long long)ND1 * N * 8);
-  const Buf CV = make_buf(a.cv + B.off * NCV, (long long)NCV * N * 8);
-  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
+  const PointIO X = point_io(a, B, p, qi);
+  const Buf CV = make_buf(a.cv + B.ioff * 12, (long long)12 * B.nint * 8);
+  const int fc = B.nint * 8;
   double* rhs = a.rhs + B.off * NVAR;
-  // The matter buffer (vacuum: absent) is passed by value: a buffer
-  // resource must stay in SGPRs, never behind a pointer.
-  if (a.mat) {
-    const Buf MT = make_buf(a.mat + B.off * NMAT, (long long)NMAT * N * 8);
-    rhs_assemble_point(a.o, B, U, D, CV, MT, true, P, rhs, p, N);
-  } else {
-    rhs_assemble_point(a.o, B, U, D, CV, U, false, P, rhs, p, N);
+  rhs_assembly_point(a.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
+                     [&](int n, double v) { rhs[n * N + p] = v; });
+  if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
+    const Pt P = make_pt(p, B.nn1, B.nn2, N);
+    sommerfeld_point(B, X.U, P, a.u + B.off * NVAR, rhs, i, j, k, N);
   }
-  if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k))
-    sommerfeld_point(B, U, P, u, rhs, i, j, k, N);
 }
 
 __global__ void __launch_bounds__(TILE) k_sommerfeld(RhsArgs a) {
@@ -852,8 +874,13 @@ struct z4c_ctx {
   Tile* d_tiles[NBOX] = {nullptr, nullptr, nullptr};
   Box* d_boxes[NBOX] = {nullptr, nullptr, nullptr};
   int ntiles[NBOX] = {0, 0, 0};
-  double* d1 = nullptr;
-  double* cv = nullptr;    // curvature-pass scratch (12 fields)
+  double* d1 = nullptr;     // prepared first derivatives (PrepareZ4cDerivatives)
+  double* dv = nullptr;     // derivative slots of the RHS stencil pass
+  double* cv = nullptr;     // curvature part of the RHS algebra
+  PlaneTile* d_ptiles = nullptr;
+  int nptiles = 0;
+  int stencil_lds = 0;      // doubles of LDS per stencil workgroup
+  long long total_int = 0;  // interior points
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
   int n_cu = 256;
@@ -868,11 +895,16 @@ struct z4c_ctx {
       ntiles[b] = 0;
     }
     hipFree(d1);
+    hipFree(dv);
     hipFree(cv);
+    hipFree(d_ptiles);
     hipFree(snap);
     d_blocks = nullptr;
     d1 = nullptr;
+    dv = nullptr;
     cv = nullptr;
+    d_ptiles = nullptr;
+    nptiles = 0;
     snap = nullptr;
     snap_elems = 0;
   }
@@ -935,14 +967,18 @@ int launch_prep(z4c_ctx* c, const z4c_params* p, const double* u, double* aux,
 int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
                const double* mat, double* rhs, bool sommerfeld, hipStream_t s) {
   if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
-  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->d1, mat, rhs, c->cv, rhs_params(p)};
+  StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, u, c->dv, p->diss};
+  hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
+                     sizeof(double) * c->stencil_lds, s, sa);
+  HIPCALL(hipGetLastError());
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
   const int g = grid_for(a.L.ntiles);
   hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   if (sommerfeld)
-    hipLaunchKernelGGL(k_rhs_asm<true>, dim3(g), dim3(TILE), 0, s, a);
+    hipLaunchKernelGGL(k_rhs_alg<true>, dim3(g), dim3(TILE), 0, s, a);
   else
-    hipLaunchKernelGGL(k_rhs_asm<false>, dim3(g), dim3(TILE), 0, s, a);
+    hipLaunchKernelGGL(k_rhs_alg<false>, dim3(g), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
@@ -1043,7 +1079,9 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   std::vector<DevBlock> hb(nblocks);
   std::vector<Box> boxes[NBOX];
   std::vector<Tile> tiles[NBOX];
-  long long off = 0;
+  long long off = 0, ioff = 0;
+  std::vector<PlaneTile> ptiles;
+  int max_lds = 0;
   const double diss_scaling = std::pow(2.0, -2 * NG) * (NG % 2 == 0 ? -1 : 1);
   for (int b = 0; b < nblocks; ++b) {
     const z4c_block& in = blocks[b];
@@ -1060,14 +1098,45 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     }
     DevBlock& B = hb[b];
     B.off = off;
-    B.d1off = off * ND1;
+    B.ioff = ioff;
     B.nn1 = in.nn1; B.nn2 = in.nn2; B.nn3 = in.nn3;
     B.il = in.il; B.iu = in.iu; B.jl = in.jl; B.ju = in.ju; B.kl = in.kl; B.ku = in.ku;
     const long long npts = (long long)in.nn1 * in.nn2 * in.nn3;
     if (npts * ND1 > std::numeric_limits<int>::max())
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.npts = (int)npts;
+    B.ni = in.iu - in.il + 1;
+    B.nj = in.ju - in.jl + 1;
+    B.nk = in.ku - in.kl + 1;
+    B.nint = B.ni * B.nj * B.nk;
     B.faces = in.sommerfeld_faces & 63;
+    // x-y plane tiles of the stencil pass: the fewest tiles that fit the
+    // 256-thread mapping, preferring the widest x extent (coalescing)
+    {
+      int best_tx = 0, best_ty = 0, best_n = 0;
+      for (int ntx = 1; ntx <= B.ni; ++ntx) {
+        const int tx = (B.ni + ntx - 1) / ntx;
+        int nty = 1, ty = B.nj;
+        while (nty <= B.nj && !stencil_tile_fits(tx, ty)) {
+          ++nty;
+          ty = (B.nj + nty - 1) / nty;
+        }
+        if (!stencil_tile_fits(tx, ty)) continue;
+        if (best_n == 0 || ntx * nty < best_n) {
+          best_n = ntx * nty;
+          best_tx = tx;
+          best_ty = ty;
+        }
+      }
+      if (best_n == 0)
+        return fail(Z4C_EINVAL, "set_blocks: no stencil tile fits block " + std::to_string(b));
+      for (int j0 = 0; j0 < B.nj; j0 += best_ty)
+        for (int i0 = 0; i0 < B.ni; i0 += best_tx) {
+          PlaneTile pt{b, i0, j0, std::min(best_tx, B.ni - i0), std::min(best_ty, B.nj - j0)};
+          ptiles.push_back(pt);
+          max_lds = std::max(max_lds, stencil_lds_doubles(pt.ni, pt.nj));
+        }
+    }
     for (int d = 0; d < 3; ++d) {
       const double dx = in.dx[d];
       const double t = 1.0 / dx;
@@ -1092,7 +1161,10 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
       for (int s = 0; s < np; s += TILE) tiles[kind].push_back(Tile{b, s});
     }
     off += npts;
+    ioff += B.nint;
   }
+  if (max_lds * (long long)sizeof(double) > 160 * 1024)
+    return fail(Z4C_EINVAL, "set_blocks: block x extent too large for the stencil pass LDS tile");
   c->release();
   c->nblocks = 0;
   if (hipMalloc(&c->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess)
@@ -1112,11 +1184,23 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     return fail(Z4C_ENOMEM, "set_blocks: derivative storage allocation failed");
   // prepared derivatives start at zero (the reference zero-initialises them)
   HIPCALL(hipMemset(c->d1, 0, sizeof(double) * off * ND1));
-  if (hipMalloc(&c->cv, sizeof(double) * off * NCV) != hipSuccess)
-    return fail(Z4C_ENOMEM, "set_blocks: curvature scratch allocation failed");
+  if (hipMalloc(&c->dv, sizeof(double) * ioff * NDV) != hipSuccess ||
+      hipMalloc(&c->cv, sizeof(double) * ioff * 12) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_blocks: derivative-slot allocation failed");
+  if (hipMalloc(&c->d_ptiles, sizeof(PlaneTile) * ptiles.size()) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_blocks: allocation failed");
+  HIPCALL(hipMemcpy(c->d_ptiles, ptiles.data(), sizeof(PlaneTile) * ptiles.size(),
+                    hipMemcpyHostToDevice));
+  c->nptiles = (int)ptiles.size();
+  c->stencil_lds = max_lds;
+  if (max_lds * sizeof(double) > 64 * 1024)
+    HIPCALL(hipFuncSetAttribute((const void*)k_stencil,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(max_lds * sizeof(double))));
   c->hblocks.assign(blocks, blocks + nblocks);
   c->nblocks = nblocks;
   c->total_pts = off;
+  c->total_int = ioff;
   if (elems_per_var) *elems_per_var = off;
   return Z4C_OK;
 }
@@ -1143,7 +1227,7 @@ int z4c_boundary_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !rhs) return fail(Z4C_EINVAL, "boundary_rhs: null argument");
-  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->d1, nullptr, rhs, nullptr, rhs_params(p)};
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, nullptr, nullptr, rhs, nullptr, rhs_params(p)};
   hipLaunchKernelGGL(k_sommerfeld, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0,
                      (hipStream_t)stream, a);
   HIPCALL(hipGetLastError());

@@ -1,16 +1,13 @@
-// z4c_rhs_point.hpp -- the Z4c right-hand side at one interior vertex, in
-// two passes with bounded register footprints:
-//   curvature pass : R~_ab + R^phi_ab (conformal + conformal-factor Ricci)
-//                    and D_a D_b alpha -> 12 values per point, kept in HBM;
-//   assembly pass  : A~ contractions, Lie/advective terms, KO dissipation
-//                    and the 22 right-hand sides.
-// One thread evaluates every stencil its point needs directly from the
-// state and the prepared first derivatives (buffer loads, L1/L2 resident
-// for the tile).  The arithmetic follows Z4c::Z4cRHS
-// (src/z4c/calculate_z4c_rhs.cpp:29-916) term by term and in the same
-// order; the reference's pencil loops only batch this per-point sequence.
+// z4c_rhs_point.hpp -- algebra pass of the Z4c right-hand side.
+//
+// Pointwise: reads the state at the point and the derivative slots written
+// by the stencil pass (coalesced, one buffer offset per thread) and
+// evaluates Z4c::Z4cRHS (src/z4c/calculate_z4c_rhs.cpp:29-916) term by term
+// in the reference's operation order.  The only reordering: the three KO
+// terms, which the reference adds to the finished RHS one direction at a
+// time (:908-915), arrive pre-summed as (D_x + D_y) + D_z.
 // Vacuum build and constant shift damping (no Z4C_ETA_CONF /
-// Z4C_ETA_TRACK_TP); matter terms are kept for a non-null matter buffer.
+// Z4C_ETA_TRACK_TP); matter sources enter through `mat` when present.
 #pragma once
 
 #include "z4c_device.hpp"
@@ -23,40 +20,55 @@ struct RhsParams {
   double shift_Gamma, shift_alpha2Gamma, shift_H, shift_advect, shift_eta;
 };
 
-// d1 field index of direction b, prepared field f
-__device__ __forceinline__ constexpr int d1f(int b, int f) { return b * 11 + f; }
-
-// Conformal metric inverse and Christoffels from the state and the prepared
-// first derivatives (calculate_z4c_rhs.cpp:292-352).
-struct Metric {
-  double gdd[6], guu[6], Gudd[3][6], Gu[3];
+// Accessors of one point: state (padded index) and derivative slots
+// (interior index).
+struct PointIO {
+  Buf U;        // state, field stride N
+  Buf DV;       // derivative slots, field stride NI
+  Buf MT;       // matter (valid if has_mat)
+  unsigned vu;  // p*8
+  unsigned vd;  // qi*8
+  int fu;       // N*8
+  int fd;       // NI*8
+  bool has_mat;
+  __device__ __forceinline__ double u(int n) const { return bload(U, vu, n * fu); }
+  __device__ __forceinline__ double d(int slot) const { return bload(DV, vd, slot * fd); }
+  __device__ __forceinline__ double m(int n) const { return bload(MT, vu, n * fu); }
 };
 
-__device__ __forceinline__ void metric_at(const Buf& U, const Buf& D, const Pt& P,
-                                          Metric& M, double (*Gddd)[6]) {
+// Curvature part: cv(0..5) = R~_ab + R^phi_ab, cv(6..11) = D_a D_b alpha.
+template <class Out>
+__device__ __forceinline__ void rhs_curvature_point(const RhsParams& o, const PointIO& X,
+                                                    Out&& cv) {
+  const double pw = o.chi_psi_power;
+  const double oochi_psi_power = 1.0 / pw;
+
+  // ---------------------------------------------------------------- metric
+  double gdd[6], guu[6];
 #pragma unroll
-  for (int ab = 0; ab < 6; ++ab) M.gdd[ab] = ldc(U, P, V_GXX + ab);
+  for (int ab = 0; ab < 6; ++ab) gdd[ab] = X.u(V_GXX + ab);
   {
-    const double detg = det3(M.gdd);
-    inv3(1.0 / detg, M.gdd, M.guu);
+    const double detg = det3(gdd);
+    inv3(1.0 / detg, gdd, guu);
   }
+  double Gddd[3][6], Gudd[3][6], Gu[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = a; b < 3; ++b)
-        Gddd[c][sym(a, b)] = 0.5 * (ldc(D, P, d1f(a, F_G + sym(b, c))) +
-                                    ldc(D, P, d1f(b, F_G + sym(a, c))) -
-                                    ldc(D, P, d1f(c, F_G + sym(a, b))));
+        Gddd[c][sym(a, b)] = 0.5 * (X.d(s_d(V_GXX + sym(b, c), a)) +
+                                    X.d(s_d(V_GXX + sym(a, c), b)) -
+                                    X.d(s_d(V_GXX + sym(a, b), c)));
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int ab = 0; ab < 6; ++ab) {
       double t = 0.0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) t += M.guu[sym(c, d)] * Gddd[d][ab];
-      M.Gudd[c][ab] = t;
+      for (int d = 0; d < 3; ++d) t += guu[sym(c, d)] * Gddd[d][ab];
+      Gudd[c][ab] = t;
     }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -65,31 +77,12 @@ __device__ __forceinline__ void metric_at(const Buf& U, const Buf& D, const Pt& 
     for (int b = 0; b < 3; ++b)
 #pragma unroll
       for (int c = b; c < 3; ++c)
-        t += ((b == c) ? 1.0 : 2.0) * M.guu[sym(b, c)] * M.Gudd[a][sym(b, c)];
-    M.Gu[a] = t;
+        t += ((b == c) ? 1.0 : 2.0) * guu[sym(b, c)] * Gudd[a][sym(b, c)];
+    Gu[a] = t;
   }
-}
 
-// ---------------------------------------------------------------------------
-// Curvature pass: cv[0..5] = R~_ab + R^phi_ab, cv[6..11] = D_a D_b alpha.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void rhs_curv_point(const RhsParams& o, const DevBlock& B,
-                                               const Buf& U, const Buf& D,
-                                               const Pt& P, double* __restrict__ cv,
-                                               int p, int N) {
-  const double pw = o.chi_psi_power;
-  const double oochi_psi_power = 1.0 / pw;
-  Metric M;
-  double Gddd[3][6];
-  metric_at(U, D, P, M, Gddd);
-
-  // conformal Ricci R~_ab (:357-396)
-  double dGam[3][3];  // [direction][component]
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) dGam[b][a] = fd_dx(U, P, V_GAMX + a, b, B.cidx1[b]);
-  double Rdd[6];
+  // ------------------------------------------------ conformal Ricci (:357-396)
+  double RR[6];  // R~_ab, then + R^phi_ab
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -97,48 +90,47 @@ __device__ __forceinline__ void rhs_curv_point(const RhsParams& o, const DevBloc
       double r = 0.0;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        r += 0.5 * (M.gdd[sym(c, a)] * dGam[b][c] + M.gdd[sym(c, b)] * dGam[a][c] +
-                    M.Gu[c] * (Gddd[a][sym(b, c)] + Gddd[b][sym(a, c)]));
+        r += 0.5 * (gdd[sym(c, a)] * X.d(s_d(V_GAMX + c, b)) +
+                    gdd[sym(c, b)] * X.d(s_d(V_GAMX + c, a)) +
+                    Gu[c] * (Gddd[a][sym(b, c)] + Gddd[b][sym(a, c)]));
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int d = c; d < 3; ++d) {
-          // d_c d_d g~_ab: Dxx on the diagonal, nested Dx(c, d_d g~) else
-          const double ddg = (c == d)
-                                 ? fd_dxx(U, P, V_GXX + sym(a, b), c, B.cidx2[c])
-                                 : fd_dx(D, P, d1f(d, F_G + sym(a, b)), c, B.cidx1[c]);
-          r -= ((c == d) ? 0.5 : 1.0) * M.guu[sym(c, d)] * ddg;
-        }
+        for (int d = c; d < 3; ++d)
+          r -= ((c == d) ? 0.5 : 1.0) * guu[sym(c, d)] * X.d(s_dd(V_GXX + sym(a, b), sym(c, d)));
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
           for (int e = 0; e < 3; ++e)
-            r += M.guu[sym(c, d)] * (M.Gudd[e][sym(c, a)] * Gddd[b][sym(e, d)] +
-                                     M.Gudd[e][sym(c, b)] * Gddd[a][sym(e, d)] +
-                                     M.Gudd[e][sym(a, d)] * Gddd[e][sym(c, b)]);
-      Rdd[sym(a, b)] = r;
+            r += guu[sym(c, d)] * (Gudd[e][sym(c, a)] * Gddd[b][sym(e, d)] +
+                                   Gudd[e][sym(c, b)] * Gddd[a][sym(e, d)] +
+                                   Gudd[e][sym(a, d)] * Gddd[e][sym(c, b)]);
+      RR[sym(a, b)] = r;
     }
 
-  // conformal factor terms R^phi_ab (:402-458)
-  const double chi = ldc(U, P, V_CHI);
+  // --------------------------------------- conformal factor terms (:402-458)
+  const double chi = X.u(V_CHI);
   const double chi_guarded = dmax(chi, o.chi_div_floor);
   const double oochi = 1.0 / chi_guarded;
-  double dphi[3];
+  const double oopsi4 = (pw == -4.) ? chi_guarded : pow(chi_guarded, -4. / pw);
+  double dchi[3], dphi[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) dphi[a] = ldc(D, P, d1f(a, F_CHI)) * oochi * oochi_psi_power;
+  for (int a = 0; a < 3; ++a) {
+    dchi[a] = X.d(s_d(V_CHI, a));
+    dphi[a] = dchi[a] * oochi * oochi_psi_power;
+  }
   {
     double Ddphi[6];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = a; b < 3; ++b) {
-        const double ddchi = (a == b) ? fd_dxx(U, P, V_CHI, a, B.cidx2[a])
-                                      : fd_dx(D, P, d1f(b, F_CHI), a, B.cidx1[a]);
-        double t = ddchi * oochi * oochi_psi_power - pw * dphi[a] * dphi[b];
+        double t = X.d(s_dd(V_CHI, sym(a, b))) * oochi * oochi_psi_power -
+                   pw * dphi[a] * dphi[b];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) t -= M.Gudd[c][sym(a, b)] * dphi[c];
+        for (int c = 0; c < 3; ++c) t -= Gudd[c][sym(a, b)] * dphi[c];
         Ddphi[sym(a, b)] = t;
       }
     double trphi = 0.0;
@@ -146,102 +138,139 @@ __device__ __forceinline__ void rhs_curv_point(const RhsParams& o, const DevBloc
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int d = c; d < 3; ++d)
-        trphi += ((c == d) ? 1.0 : 2.0) * M.guu[sym(c, d)] *
+        trphi += ((c == d) ? 1.0 : 2.0) * guu[sym(c, d)] *
                  (Ddphi[sym(c, d)] + 2. * dphi[c] * dphi[d]);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
+      for (int b = a; b < 3; ++b)
+        RR[sym(a, b)] = RR[sym(a, b)] + (4. * dphi[a] * dphi[b] - 2. * Ddphi[sym(a, b)] -
+                                         2. * gdd[sym(a, b)] * trphi);
+  }
+
+  // ------------------------------------------------ D_a D_b alpha (:484-532)
+  double dalpha[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) dalpha[a] = X.d(s_d(V_ALPHA, a));
+  double Ddalpha_dd[6];
+  {
+    double Dda = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) Dda += guu[sym(c, d)] * dphi[c] * dalpha[d];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
       for (int b = a; b < 3; ++b) {
-        const double Rphi = 4. * dphi[a] * dphi[b] - 2. * Ddphi[sym(a, b)] -
-                            2. * M.gdd[sym(a, b)] * trphi;
-        cv[sym(a, b) * N + p] = Rdd[sym(a, b)] + Rphi;
+        double t = X.d(s_dd(V_ALPHA, sym(a, b))) -
+                   2. * (dphi[a] * dalpha[b] + dphi[b] * dalpha[a]) +
+                   2. * gdd[sym(a, b)] * Dda;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t -= Gudd[c][sym(a, b)] * dalpha[c];
+        Ddalpha_dd[sym(a, b)] = t;
       }
   }
 
-  // D_a D_b alpha (:484-521)
-  double dalpha[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) dalpha[a] = ldc(D, P, d1f(a, F_ALPHA));
-  double Dda = 0.0;
+  for (int ab = 0; ab < 6; ++ab) {
+    cv(ab, RR[ab]);
+    cv(6 + ab, Ddalpha_dd[ab]);
+  }
+}
+
+// Assembly part: the 22 right-hand sides at the point from the state, the
+// derivative slots and the curvature part cv(0..11); out(n, value) stores.
+template <class In, class Out>
+__device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const PointIO& X,
+                                                   In&& cv, Out&& out) {
+  const double pw = o.chi_psi_power;
+  const double chi_pow_DA_u = -6.0 / pw;
+
+  // ---------------------------------------------------------------- metric
+  double gdd[6], guu[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) gdd[ab] = X.u(V_GXX + ab);
+  {
+    const double detg = det3(gdd);
+    inv3(1.0 / detg, gdd, guu);
+  }
+  double Gudd[3][6], Gu[3];
+  {
+  double Gddd[3][6];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) Dda += M.guu[sym(c, d)] * dphi[c] * dalpha[d];
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+      for (int b = a; b < 3; ++b)
+        Gddd[c][sym(a, b)] = 0.5 * (X.d(s_d(V_GXX + sym(b, c), a)) +
+                                    X.d(s_d(V_GXX + sym(a, c), b)) -
+                                    X.d(s_d(V_GXX + sym(a, b), c)));
 #pragma unroll
-    for (int b = a; b < 3; ++b) {
-      const double dda = (a == b) ? fd_dxx(U, P, V_ALPHA, a, B.cidx2[a])
-                                  : fd_dx(D, P, d1f(b, F_ALPHA), a, B.cidx1[a]);
-      double t = dda - 2. * (dphi[a] * dalpha[b] + dphi[b] * dalpha[a]) +
-                 2. * M.gdd[sym(a, b)] * Dda;
+  for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) t -= M.Gudd[c][sym(a, b)] * dalpha[c];
-      cv[(6 + sym(a, b)) * N + p] = t;
+    for (int ab = 0; ab < 6; ++ab) {
+      double t = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) t += guu[sym(c, d)] * Gddd[d][ab];
+      Gudd[c][ab] = t;
     }
-}
-
-// ---------------------------------------------------------------------------
-// Assembly pass: the 22 right-hand sides (incl. KO dissipation) at point p.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const DevBlock& B,
-                                                   const Buf& U, const Buf& D,
-                                                   const Buf& CV, const Buf MT, bool has_mat,
-                                                   const Pt& P, double* __restrict__ rhs,
-                                                   int p, int N) {
-  const double pw = o.chi_psi_power;
-  const double chi_pow_DA_u = -6.0 / pw;
-  Metric M;
-  {
-    double Gddd[3][6];
-    metric_at(U, D, P, M, Gddd);
   }
-  const double alpha = ldc(U, P, V_ALPHA), chi = ldc(U, P, V_CHI),
-               Khat = ldc(U, P, V_KHAT), Theta = ldc(U, P, V_THETA);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double t = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = b; c < 3; ++c)
+        t += ((b == c) ? 1.0 : 2.0) * guu[sym(b, c)] * Gudd[a][sym(b, c)];
+    Gu[a] = t;
+  }
+
+  const double chi = X.u(V_CHI);
   const double chi_guarded = dmax(chi, o.chi_div_floor);
   const double oochi = 1.0 / chi_guarded;
   const double oopsi4 = (pw == -4.) ? chi_guarded : pow(chi_guarded, -4. / pw);
   double dchi[3], dalpha[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    dchi[a] = ldc(D, P, d1f(a, F_CHI));
-    dalpha[a] = ldc(D, P, d1f(a, F_ALPHA));
+    dchi[a] = X.d(s_d(V_CHI, a));
+    dalpha[a] = X.d(s_d(V_ALPHA, a));
   }
+  double RR[6], Ddalpha_dd[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    RR[ab] = cv(ab);
+    Ddalpha_dd[ab] = cv(6 + ab);
+  }
+  double Ddalpha = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a; b < 3; ++b)
+      Ddalpha += ((a == b) ? 1.0 : 2.0) * oopsi4 * guu[sym(a, b)] * Ddalpha_dd[sym(a, b)];
 
-  // matter (zero in vacuum) and its trace (:466-476)
+  // ------------------------------------------------------- matter (:466-476)
   double rho = 0.0, Sd[3] = {0.0, 0.0, 0.0}, Sdd[6] = {0, 0, 0, 0, 0, 0};
-  if (has_mat) {
-    rho = ldc(MT, P, M_RHO);
+  if (X.has_mat) {
+    rho = X.m(M_RHO);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) Sd[a] = ldc(MT, P, M_SX + a);
+    for (int a = 0; a < 3; ++a) Sd[a] = X.m(M_SX + a);
 #pragma unroll
-    for (int ab = 0; ab < 6; ++ab) Sdd[ab] = ldc(MT, P, M_SXX + ab);
+    for (int ab = 0; ab < 6; ++ab) Sdd[ab] = X.m(M_SXX + ab);
   }
   double Str = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = a; b < 3; ++b)
-      Str += ((a == b) ? 1.0 : 2.0) * oopsi4 * M.guu[sym(a, b)] * Sdd[sym(a, b)];
+      Str += ((a == b) ? 1.0 : 2.0) * oopsi4 * guu[sym(a, b)] * Sdd[sym(a, b)];
 
-  // traces of the curvature pass output (:522-532, :606-613)
-  double Ddalpha = 0.0, R = 0.0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = a; b < 3; ++b)
-      Ddalpha += ((a == b) ? 1.0 : 2.0) * oopsi4 * M.guu[sym(a, b)] *
-                 ldc(CV, P, 6 + sym(a, b));
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = a; b < 3; ++b)
-      R += ((a == b) ? 1.0 : 2.0) * oopsi4 * M.guu[sym(a, b)] * ldc(CV, P, sym(a, b));
-
-  // A~ contractions (:536-601)
+  // ------------------------------------------------ A~ contractions (:536-601)
   double Add[6];
 #pragma unroll
-  for (int ab = 0; ab < 6; ++ab) Add[ab] = ldc(U, P, V_AXX + ab);
+  for (int ab = 0; ab < 6; ++ab) Add[ab] = X.u(V_AXX + ab);
   double Aud[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -249,7 +278,7 @@ __device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const Dev
     for (int b = 0; b < 3; ++b) {
       double t = 0.0;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) t += M.guu[sym(c, d)] * Add[sym(d, b)];
+      for (int d = 0; d < 3; ++d) t += guu[sym(c, d)] * Add[sym(d, b)];
       Aud[c][b] = t;
     }
   double AA[6];
@@ -267,7 +296,7 @@ __device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const Dev
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = a; b < 3; ++b)
-      trAA += ((a == b) ? 1.0 : 2.0) * M.guu[sym(a, b)] * AA[sym(a, b)];
+      trAA += ((a == b) ? 1.0 : 2.0) * guu[sym(a, b)] * AA[sym(a, b)];
   double Auu[6];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -275,171 +304,146 @@ __device__ __forceinline__ void rhs_assemble_point(const RhsParams& o, const Dev
     for (int b = a; b < 3; ++b) {
       double t = 0.0;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) t += M.guu[sym(b, c)] * Aud[a][c];
+      for (int c = 0; c < 3; ++c) t += guu[sym(b, c)] * Aud[a][c];
       Auu[sym(a, b)] = t;
     }
+  double DA[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double t = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      t -= chi_pow_DA_u * Auu[sym(a, b)] * dchi[b] * oochi;
+      t -= (1. / 3.) * guu[sym(a, b)] * (2. * X.d(s_d(V_KHAT, b)) + X.d(s_d(V_THETA, b)));
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int c = b; c < 3; ++c)
+        t += ((b == c) ? 1.0 : 2.0) * Gudd[a][sym(b, c)] * Auu[sym(b, c)];
+    DA[a] = t;
+  }
+
+  // ------------------------------------------------- R, H~, K (:606-624)
+  double R = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a; b < 3; ++b)
+      R += ((a == b) ? 1.0 : 2.0) * oopsi4 * guu[sym(a, b)] * RR[sym(a, b)];
+  const double alpha = X.u(V_ALPHA), Khat = X.u(V_KHAT), Theta = X.u(V_THETA);
   const double K = Khat + 2. * Theta;
   const double Ht = R + (2. / 3.) * (K * K) - trAA;
 
-  // shift derivatives
-  double beta[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) beta[a] = ldc(U, P, V_BETAX + a);
+  // ---------------------------------------- shift derivatives (:630-640)
   double dbeta[3][3];  // [direction][component]
 #pragma unroll
   for (int b = 0; b < 3; ++b)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) dbeta[b][a] = ldc(D, P, d1f(b, F_BETA + a));
+    for (int a = 0; a < 3; ++a) dbeta[b][a] = X.d(s_d(V_BETAX + a, b));
   double dbeta_div = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) dbeta_div += dbeta[a][a];
-
   const double k1 = o.damp_kappa1, k2 = o.damp_kappa2;
-  const double diss = o.diss;
-  // advective derivative sum_a Lx(a, beta^a, v), in a order (:184-226)
-  auto advect = [&](int v) {
-    double L = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) L += fd_lx(U, P, v, a, B.lidx[a], beta[a]);
-    return L;
-  };
-  // KO dissipation added in direction order after the RHS (:908-915)
-  auto dissip = [&](int v, double r) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) r += fd_diss(U, P, v, a, B.cidxd[a], diss);
-    return r;
-  };
 
-  // chi, Khat, Theta (:694-711)
+  // ----------------------------------------- chi, Khat, Theta (:694-711)
   {
-    double Lchi = advect(V_CHI);
+    double Lchi = X.d(s_L(V_CHI));
     Lchi += (1. / 6.) * pw * chi_guarded * dbeta_div;
-    const double r_chi = Lchi - (1. / 6.) * pw * chi_guarded * alpha * K;
-    rhs[V_CHI * N + p] = dissip(V_CHI, r_chi);
+    const double r = Lchi - (1. / 6.) * pw * chi_guarded * alpha * K;
+    out(V_CHI, r + X.d(s_D(V_CHI)));
   }
   {
-    const double LKhat = advect(V_KHAT);
-    double r = -Ddalpha + alpha * (trAA + (1. / 3.) * (K * K)) + LKhat +
+    double r = -Ddalpha + alpha * (trAA + (1. / 3.) * (K * K)) + X.d(s_L(V_KHAT)) +
                k1 * (1 - k2) * alpha * Theta;
     r += 4 * M_PI * alpha * (Str + rho);
-    rhs[V_KHAT * N + p] = dissip(V_KHAT, r);
+    out(V_KHAT, r + X.d(s_D(V_KHAT)));
   }
   {
-    const double LTheta = advect(V_THETA);
-    double r = LTheta + alpha * (0.5 * Ht - (2. + k2) * k1 * Theta);
+    double r = X.d(s_L(V_THETA)) + alpha * (0.5 * Ht - (2. + k2) * k1 * Theta);
     r -= 8. * M_PI * alpha * rho;
-    rhs[V_THETA * N + p] = dissip(V_THETA, r);
+    out(V_THETA, r + X.d(s_D(V_THETA)));
   }
 
-  // Gamma~ (:568-589 DA, :640-668 Lie completion, :713-729 assembly)
+  // -------------------------------------------------- Gamma~ (:640-729)
   {
-    double DA[3];
+    double ddbeta_d[3];  // (1/3) sum_b d_a d_b beta^b
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       double t = 0.0;
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const double dKhat = fd_dx(U, P, V_KHAT, b, B.cidx1[b]);
-        const double dTheta = fd_dx(U, P, V_THETA, b, B.cidx1[b]);
-        t -= chi_pow_DA_u * Auu[sym(a, b)] * dchi[b] * oochi;
-        t -= (1. / 3.) * M.guu[sym(a, b)] * (2. * dKhat + dTheta);
-      }
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int c = b; c < 3; ++c)
-          t += ((b == c) ? 1.0 : 2.0) * M.Gudd[a][sym(b, c)] * Auu[sym(b, c)];
-      DA[a] = t;
-    }
-    // ddbeta_d^a = (1/3) sum_b d_a d_b beta^b; mixed second derivatives are
-    // the reference's nested Dx(lower index, d_(higher index) beta)
-    double ddbeta_d[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      double t = 0.0;
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        const int lo = (a < b) ? a : b, hi = (a < b) ? b : a;
-        const double v = (a == b) ? fd_dxx(U, P, V_BETAX + b, a, B.cidx2[a])
-                                  : fd_dx(D, P, d1f(hi, F_BETA + b), lo, B.cidx1[lo]);
-        t += (1. / 3.) * v;
-      }
+      for (int b = 0; b < 3; ++b) t += (1. / 3.) * X.d(s_dd(V_BETAX + b, sym(a, b)));
       ddbeta_d[a] = t;
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      double LGam = advect(V_GAMX + a);
-      LGam += (2. / 3.) * M.Gu[a] * dbeta_div;
+      double LGam = X.d(s_L(V_GAMX + a));
+      LGam += (2. / 3.) * Gu[a] * dbeta_div;
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        LGam += M.guu[sym(a, b)] * ddbeta_d[b] - M.Gu[b] * dbeta[b][a];
+        LGam += guu[sym(a, b)] * ddbeta_d[b] - Gu[b] * dbeta[b][a];
 #pragma unroll
-        for (int c = b; c < 3; ++c) {
-          const double ddb = (b == c) ? fd_dxx(U, P, V_BETAX + a, b, B.cidx2[b])
-                                      : fd_dx(D, P, d1f(c, F_BETA + a), b, B.cidx1[b]);
-          LGam += ((b == c) ? 1.0 : 2.0) * M.guu[sym(b, c)] * ddb;
-        }
+        for (int c = b; c < 3; ++c)
+          LGam += ((b == c) ? 1.0 : 2.0) * guu[sym(b, c)] * X.d(s_dd(V_BETAX + a, sym(b, c)));
       }
-      const double Gam_a = ldc(U, P, V_GAMX + a);
       double t = 2. * alpha * DA[a] + LGam;
-      t -= 2. * alpha * k1 * (Gam_a - M.Gu[a]);
+      t -= 2. * alpha * k1 * (X.u(V_GAMX + a) - Gu[a]);
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         t -= 2. * Auu[sym(a, b)] * dalpha[b];
-        t -= 16. * M_PI * alpha * M.guu[sym(a, b)] * Sd[b];
+        t -= 16. * M_PI * alpha * guu[sym(a, b)] * Sd[b];
       }
-      rhs[(V_GAMX + a) * N + p] = dissip(V_GAMX + a, t);
+      out(V_GAMX + a, t + X.d(s_D(V_GAMX + a)));
     }
   }
 
-  // g~ and A~ (:671-687 Lie completion, :732-754 assembly)
+  // ------------------------------------------- g~ and A~ (:671-754)
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = a; b < 3; ++b) {
       const int ab = sym(a, b);
-      double Lg = advect(V_GXX + ab);
-      double LA = advect(V_AXX + ab);
-      Lg -= (2. / 3.) * M.gdd[ab] * dbeta_div;
+      double Lg = X.d(s_L(V_GXX + ab));
+      double LA = X.d(s_L(V_AXX + ab));
+      Lg -= (2. / 3.) * gdd[ab] * dbeta_div;
       LA -= (2. / 3.) * Add[ab] * dbeta_div;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        Lg += dbeta[a][c] * M.gdd[sym(b, c)];
+        Lg += dbeta[a][c] * gdd[sym(b, c)];
         LA += dbeta[b][c] * Add[sym(a, c)];
-        Lg += dbeta[b][c] * M.gdd[sym(a, c)];
+        Lg += dbeta[b][c] * gdd[sym(a, c)];
         LA += dbeta[a][c] * Add[sym(b, c)];
       }
       const double r_g = -2. * alpha * Add[ab] + Lg;
-      double t = oopsi4 * (-ldc(CV, P, 6 + ab) + alpha * ldc(CV, P, ab));
-      t -= (1. / 3.) * M.gdd[ab] * (-Ddalpha + alpha * R);
+      double t = oopsi4 * (-Ddalpha_dd[ab] + alpha * RR[ab]);
+      t -= (1. / 3.) * gdd[ab] * (-Ddalpha + alpha * R);
       t += alpha * (K * Add[ab] - 2. * AA[ab]);
       t += LA;
-      t -= 8. * M_PI * alpha * (oopsi4 * Sdd[ab] - (1. / 3.) * Str * M.gdd[ab]);
-      rhs[(V_GXX + ab) * N + p] = dissip(V_GXX + ab, r_g);
-      rhs[(V_AXX + ab) * N + p] = dissip(V_AXX + ab, t);
+      t -= 8. * M_PI * alpha * (oopsi4 * Sdd[ab] - (1. / 3.) * Str * gdd[ab]);
+      out(V_GXX + ab, r_g + X.d(s_D(V_GXX + ab)));
+      out(V_AXX + ab, t + X.d(s_D(V_AXX + ab)));
     }
 
-  // lapse (:757-766) and shift (:768-902)
+  // ------------------------------------------- lapse and shift (:757-902)
   {
-    const double Lalpha = advect(V_ALPHA);
     const double f = o.lapse_oplog * o.lapse_harmonicf + o.lapse_harmonic * alpha;
-    const double r = (o.lapse_advect * Lalpha - f * alpha * (Khat + 2 * o.lapse_K * Theta));
-    rhs[V_ALPHA * N + p] = dissip(V_ALPHA, r);
+    const double r = (o.lapse_advect * X.d(s_L(V_ALPHA)) -
+                      f * alpha * (Khat + 2 * o.lapse_K * Theta));
+    out(V_ALPHA, r + X.d(s_D(V_ALPHA)));
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const double Lbeta = advect(V_BETAX + a);
-    const double Gam_a = ldc(U, P, V_GAMX + a);
-    double r = o.shift_Gamma * Gam_a + o.shift_advect * Lbeta;
+    const double Gam_a = X.u(V_GAMX + a);
+    double r = o.shift_Gamma * Gam_a + o.shift_advect * X.d(s_L(V_BETAX + a));
     if (fabs(o.shift_alpha2Gamma) > 0.0) r += o.shift_alpha2Gamma * (alpha * alpha) * Gam_a;
     if (fabs(o.shift_H) > 0.0) {
 #pragma unroll
       for (int b = 0; b < 3; ++b)
         r += o.shift_H * alpha * chi_guarded * (0.5 * alpha * dchi[b] - dalpha[b]) *
-             M.guu[sym(a, b)];
+             guu[sym(a, b)];
     }
-    r -= o.shift_eta * beta[a];
-    rhs[(V_BETAX + a) * N + p] = dissip(V_BETAX + a, r);
+    r -= o.shift_eta * X.u(V_BETAX + a);
+    out(V_BETAX + a, r + X.d(s_D(V_BETAX + a)));
   }
 }
 
