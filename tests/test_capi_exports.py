@@ -4,6 +4,7 @@ import ctypes
 import glob
 import os
 import re
+import shutil
 import subprocess
 
 import pytest
@@ -45,9 +46,12 @@ def test_no_torch_types_in_header():
         assert "hip_runtime" not in src  # plain pointers and sizes only
 
 
-def test_binary_is_gfx950():
-    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-objdump", "--offloading", _capi.LIB_PATH],
-                         capture_output=True, text=True)
+def test_binary_is_gfx950(tmp_path):
+    # --offloading extracts the device images next to its input: work on a copy
+    lib = tmp_path / "lib.so"
+    shutil.copy(_capi.LIB_PATH, lib)
+    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-objdump", "--offloading", str(lib)],
+                         capture_output=True, text=True, cwd=tmp_path)
     if out.returncode != 0:
         pytest.skip("llvm-objdump --offloading unavailable")
     assert "gfx950" in out.stdout + out.stderr
