@@ -6,10 +6,10 @@ MeshBlocks from a seeded octree refined toward two punctures (levels 0-7,
 spacing 2^-level), interior sizes 70% 16^3 / 20% 24^3 / 10% 32^3 cells,
 Brill-Lindquist data (m1 = m2 = 0.5, separation 10, pre-collapsed lapse,
 zero shift, K = 0), moving-puncture gauge.  A step is one classical RK4 step
-(bt_rk4) in the GR_Z4c stage order on every block: Z4cRHS (+KO), SumBT_ak /
-SumBT_bk, AlgConstr, PrepareZ4cDerivatives -- one launch per kernel per
-stage for all blocks.  Ghost zones stay at u_n through the step (no inter-
-block exchange between blocks of different sizes).
+(bt_rk4) on every block: per stage Z4cRHS (+KO) with SumBT and AlgConstr
+fused into its assembly kernel, one launch per kernel per stage for all
+blocks, and PrepareZ4cDerivatives of the new state at the end.  Ghost
+vertices stay at u_n through the step (independent blocks, no exchange).
 
 value = owned point-updates per second: sum over blocks of nx^3 (the
 reference's zone count, src/main.hpp:729-731) per RK4 step, divided by the
@@ -28,12 +28,23 @@ sys.path.insert(0, os.path.join(ROOT, "gr-athena_amd"))
 
 import numpy as np  # noqa: E402
 
-# Algorithmic cost per interior point of one Z4cRHS evaluation (FP64 flops)
-# and of one RK4 point-step (bytes), the figures the roofline is priced with.
-F_RHS = 8.5e3
-B_STEP = 2992.0
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+
+# Algorithmic HBM bytes of each pass per launch, from what the pass must read
+# and write once (doubles): per interior vertex (I) and per padded vertex (P).
+#  stencil:   reads the 22 fields over the padded block, writes 158 slots
+#  curvature: reads 7 fields + 81 slots, writes 12 curvature values
+#  assembly:  reads 98 slots + 12 curvature + 22 fields, and per stage on
+#             average 16.5 (u_n) + 16.5 (running sum) doubles, writes 16.5
+#             (running sum) + 22 (stage state)
+PASS_BYTES = {
+    "stencil": lambda I, P: 8.0 * (158 * I + 22 * P),
+    "curvature": lambda I, P: 8.0 * (7 + 81 + 12) * I,
+    "assembly": lambda I, P: 8.0 * (98 + 12 + 22 + 16.5 + 16.5 + 16.5 + 22) * I,
+    "ghosts": lambda I, P: 8.0 * (22 + 14 + 44) * (P - I),
+    "prepare": lambda I, P: 8.0 * (11 + 33) * P,
+}
 
 
 def parse():
@@ -121,48 +132,43 @@ def main():
     u = z.zeros()
     z.ADMToZ4c(adm, u)
     del adm
-    un, k = z.make_rk_registers()
-    z.PrepareZ4cDerivatives(u)
+    work = z.make_step_work()
     dt = min(z.NewBlockTimeStep(0.25), 1.0)
     torch.cuda.synchronize()
     log("initial data ready")
 
-    stream = torch.cuda.current_stream()
-    ev = []
-
-    def step(record):
-        un.copy_(u)
-        for s in range(1, 5):
-            if record:
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                z.Z4cRHS(u, None, k[s - 1])
-                e1.record(stream)
-                ev.append((e0, e1))
-                # the rest of the stage (update, AlgConstr, derivative prep)
-                from z4c_amd.z4c import BT_RK4_A, BT_RK4_B
-                if s < 4:
-                    z.bt_sum(u, un, k[:s], BT_RK4_A[s][:s], dt)
-                else:
-                    z.bt_sum(u, un, k, BT_RK4_B, dt)
-                z.AlgConstr(u)
-                z.PrepareZ4cDerivatives(u)
-            else:
-                z.rk4_stage(s, dt, u, un, k)
+    def step():
+        z.rk4_step_fused(dt, u, work)
 
     for _ in range(args.warmup):
-        step(False)
+        step()
     torch.cuda.synchronize()
+    z.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(True)
+        step()
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
+    z.set_timing(False)
+    passes = z.get_timing()
     rate = owned * args.steps / T
-    t_rhs = float(np.mean([a.elapsed_time(b) for a, b in ev])) * 1e-3
     if not torch.isfinite(u).all():
         raise RuntimeError("non-finite state after the timed steps")
-    log(f"{args.steps} steps in {T:.3f} s -> {rate:.4e} point-updates/s; Z4cRHS avg {t_rhs*1e3:.3f} ms")
+    log(f"{args.steps} steps in {T:.3f} s -> {rate:.4e} point-updates/s")
+    pinfo = {}
+    for name, (ms, n) in passes.items():
+        if n == 0:
+            continue
+        avg = ms / n * 1e-3
+        e = {"launches": n, "avg_ms": avg * 1e3}
+        if name in PASS_BYTES:
+            bts = PASS_BYTES[name](interior_pts, z.total_points)
+            e["alg_bytes"] = bts
+            e["gbs"] = bts / avg / 1e9
+            e["frac_hbm"] = bts / avg / PEAK_HBM
+        pinfo[name] = e
+        log(f"  {name:10s} {n:4d} launches, avg {avg*1e3:8.3f} ms"
+            + (f", {e['gbs']:7.1f} GB/s ({e['frac_hbm']*100:5.1f}% of HBM)" if "gbs" in e else ""))
 
     out = {
         "metric": "FP64 Z4c point-updates/s, batched vs per-block launch; adjoint/forward cost",
@@ -176,44 +182,35 @@ def main():
                    "interior_points": interior_pts, "padded_points": z.total_points,
                    "parallelism": "single GPU"},
     }
-    ach = F_RHS * interior_pts / t_rhs
-    out["roofline"] = {"bound": "mfma", "achieved": ach / 1e12, "peak": PEAK_FP64 / 1e12,
-                       "unit": "TFLOP/s", "frac": ach / PEAK_FP64, "traffic": None,
-                       "kernel": "Z4cRHS (curvature + assembly passes)",
-                       "note": "FP64 vector peak; no MFMA is used (stencil + pointwise algebra)",
-                       "avg_ms": t_rhs * 1e3,
-                       "frac_hbm_step": rate * B_STEP / PEAK_HBM}
+    dom = max((k for k in pinfo if "gbs" in pinfo[k]), key=lambda k: pinfo[k]["avg_ms"] * pinfo[k]["launches"])
+    d = pinfo[dom]
+    out["roofline"] = {"bound": "hbm", "achieved": d["gbs"], "peak": PEAK_HBM / 1e9,
+                       "unit": "GB/s", "frac": d["frac_hbm"], "traffic": None,
+                       "kernel": dom, "avg_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes"]}
+    out["passes"] = pinfo
 
     if not args.no_perblock:
-        out["per_block_launch"] = per_block(z, blocks, params, u, un, k, dt, owned)
+        out["per_block_launch"] = per_block(z, blocks, params, u, dt, owned)
         out["batched_speedup_vs_per_block"] = rate / out["per_block_launch"]["value"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
-def per_block(z, blocks, params, u, un, k, dt, owned):
-    """The same RK4 step issued block by block (one launch per kernel per
-    stage per block, a single-block context each, sharing the batched
-    registers): the per-MeshBlock loop of the reference's task list on the
-    same GPU."""
+def per_block(z, blocks, params, u, dt, owned):
+    """The same fused RK4 step issued block by block (one launch per kernel
+    per stage per block, a single-block context each, over views of the
+    batched state, one shared work buffer): the per-MeshBlock loop of the
+    reference's task list on the same GPU."""
     import torch
     from z4c_amd.z4c import Z4c
     ctxs = [Z4c([g], params) for g in blocks]
-
-    def view(buf, b, g):
-        o = z.offsets[b] * 22
-        return buf[o:o + g.npts * 22]
-    regs = [(view(u, b, g), view(un, b, g), [view(x, b, g) for x in k])
-            for b, g in enumerate(blocks)]
-    for c, (v, _, _) in zip(ctxs, regs):
-        c.PrepareZ4cDerivatives(v)
+    views = [u[z.offsets[b] * 22:z.offsets[b] * 22 + g.npts * 22] for b, g in enumerate(blocks)]
+    work = torch.empty(3 * 22 * max(g.npts for g in blocks), dtype=torch.float64, device=u.device)
 
     def step():
-        un.copy_(u)
-        for s in range(1, 5):
-            for c, (v, vn, vk) in zip(ctxs, regs):
-                c.rk4_stage(s, dt, v, vn, vk)
+        for c, v in zip(ctxs, views):
+            c.rk4_step_fused(dt, v, work[:3 * 22 * c.total_points])
     step()
     torch.cuda.synchronize()
     n = 2
@@ -226,8 +223,8 @@ def per_block(z, blocks, params, u, un, k, dt, owned):
         c.close()
     return {"value": owned * n / T, "unit": "point-updates/s", "steps": n,
             "ms_per_step": T / n * 1e3,
-            "form": "per-block launches: 2048 single-block contexts, one rk4_stage call "
-                    "(5 kernel launches) per block per stage"}
+            "form": "per-block launches: 2048 single-block contexts, one fused rk4 step "
+                    "(14 kernel launches) per block per step"}
 
 
 if __name__ == "__main__":

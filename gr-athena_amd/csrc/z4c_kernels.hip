@@ -152,9 +152,10 @@ struct RhsArgs {
   RhsParams o;
 };
 
-__device__ void sommerfeld_point(const DevBlock& B, const Buf& U, const Pt& P,
-                                 const double* __restrict__ u, double* __restrict__ rhs,
-                                 int i, int j, int k, int N) {
+template <class Out>
+__device__ __forceinline__ void sommerfeld_point(const DevBlock& B, const Buf& U, const Pt& P,
+                                                 const double* __restrict__ u, int i, int j,
+                                                 int k, int N, Out&& out) {
   const int p = flat(B, i, j, k);
   const double sq2 = sqrt(2.);
   const double x1 = B.x0[0] + i * B.dx[0];
@@ -169,21 +170,21 @@ __device__ void sommerfeld_point(const DevBlock& B, const Buf& U, const Pt& P,
     th -= su[a] * fd_ds(U, P, V_THETA, a, B.cidx1lo[a]);
     kh -= sq2 * su[a] * fd_ds(U, P, V_KHAT, a, B.cidx1lo[a]);
   }
-  rhs[V_THETA * N + p] = th;
-  rhs[V_KHAT * N + p] = kh;
+  out(V_THETA, th);
+  out(V_KHAT, kh);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     double t = -u[(V_GAMX + a) * N + p] / r;
 #pragma unroll
     for (int b = 0; b < 3; ++b) t -= su[b] * fd_ds(U, P, V_GAMX + a, b, B.cidx1lo[b]);
-    rhs[(V_GAMX + a) * N + p] = t;
+    out(V_GAMX + a, t);
   }
 #pragma unroll
   for (int ab = 0; ab < 6; ++ab) {
     double t = -u[(V_AXX + ab) * N + p] / r;
 #pragma unroll
     for (int c = 0; c < 3; ++c) t -= su[c] * fd_ds(U, P, V_AXX + ab, c, B.cidx1lo[c]);
-    rhs[(V_AXX + ab) * N + p] = t;
+    out(V_AXX + ab, t);
   }
 }
 
@@ -249,7 +250,8 @@ __global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
                      [&](int n, double v) { rhs[n * N + p] = v; });
   if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
     const Pt P = make_pt(p, B.nn1, B.nn2, N);
-    sommerfeld_point(B, X.U, P, a.u + B.off * NVAR, rhs, i, j, k, N);
+    sommerfeld_point(B, X.U, P, a.u + B.off * NVAR, i, j, k, N,
+                     [&](int n, double v) { rhs[n * N + p] = v; });
   }
 }
 
@@ -262,7 +264,9 @@ __global__ void __launch_bounds__(TILE) k_sommerfeld(RhsArgs a) {
   const double* u = a.u + B.off * NVAR;
   const Buf U = make_buf(u, (long long)NVAR * N * 8);
   const Pt P = make_pt(flat(B, i, j, k), B.nn1, B.nn2, N);
-  sommerfeld_point(B, U, P, u, a.rhs + B.off * NVAR, i, j, k, N);
+  double* rhs = a.rhs + B.off * NVAR;
+  const int p = flat(B, i, j, k);
+  sommerfeld_point(B, U, P, u, i, j, k, N, [&](int n, double v) { rhs[n * N + p] = v; });
 }
 
 // ---------------------------------------------------------------------------
@@ -326,6 +330,113 @@ __global__ void __launch_bounds__(TILE) k_algconstr(AlgArgs a) {
   }
   algconstr_point(a.u + B.off * NVAR, B.npts, flat(B, i, j, k), a.eps_floor,
                   a.tf_force, a.force_reg);
+}
+
+// ---------------------------------------------------------------------------
+// Fused bt_rk4 stage: RHS assembly + stage update + AlgConstr at interior
+// vertices.  The reference runs CalculateZ4cRHS, IntegrateZ4c (SumBT over the
+// registers) and EnforceAlgConstr as separate tasks
+// (src/task_list/gr/task_list_gr_z4c.cpp:46-64); the update of each variable
+// is pointwise, so it is formed as soon as the variable's right-hand side is,
+// with the same sums in the same order:
+//   u_{s+1} = u_n + (A[s+1][s] dt) k_s          (the one nonzero of bt_rk4's row)
+//   acc_1 = u_n + (b_1 dt) k_1,  acc_s = acc_{s-1} + (b_s dt) k_s
+//   u_{n+1} = acc_3 + (b_4 dt) k_4 = ((u_n + b_1 dt k_1) + b_2 dt k_2) + ...
+// followed by AlgConstr of the point (src/z4c/z4c.cpp:677-991).
+// ---------------------------------------------------------------------------
+struct StageArgs {
+  RhsArgs r;         // r.u: stage input state (read only); r.rhs unused
+  const double* un;  // state at the start of the step (unused at the last stage)
+  double* acc;       // running sum of the final combination
+  double* out;       // stage output state (the caller's state at the last stage)
+  double wa, wb;     // A[s+1][s] * dt and b_s * dt
+  int stage;         // 1..4
+  int cowling;
+  double eps_floor;
+  int tf_force, force_reg;
+};
+
+__device__ __forceinline__ bool sommerfeld_var(int n) {
+  return n == V_THETA || n == V_KHAT || (n >= V_GAMX && n < V_GAMX + 3) ||
+         (n >= V_AXX && n < V_AXX + 6);
+}
+
+template <bool WITH_SOMMERFELD>
+__global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.r.L, b, i, j, k)) return;
+  const DevBlock B = a.r.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const long long base = B.off * NVAR;
+  const double* un = a.un + base;
+  double* acc = a.acc + base;
+  double* out = a.out + base;
+  const int stage = a.stage;
+  const double wa = a.wa, wb = a.wb;
+  auto update = [&](int n, double kv) {
+    const int e = n * N + p;
+    if (stage == 1) {
+      acc[e] = un[e] + wb * kv;
+      out[e] = un[e] + wa * kv;
+    } else if (stage < 4) {
+      acc[e] = acc[e] + wb * kv;
+      out[e] = un[e] + wa * kv;
+    } else {
+      out[e] = acc[e] + wb * kv;
+    }
+  };
+  if (a.cowling) {
+    for (int n = 0; n < NVAR; ++n) update(n, 0.0);
+  } else {
+    const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
+    const PointIO X = point_io(a.r, B, p, qi);
+    const Buf CV = make_buf(a.r.cv + B.ioff * 12, (long long)12 * B.nint * 8);
+    const int fc = B.nint * 8;
+    const bool face = WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k);
+    rhs_assembly_point(a.r.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
+                       [&](int n, double v) {
+                         if (!(face && sommerfeld_var(n))) update(n, v);
+                       });
+    if (WITH_SOMMERFELD && face) {
+      const Pt P = make_pt(p, B.nn1, B.nn2, N);
+      sommerfeld_point(B, X.U, P, a.r.u + base, i, j, k, N, update);
+    }
+  }
+  algconstr_point(out, N, p, a.eps_floor, a.tf_force, a.force_reg);
+}
+
+// Ghost vertices of a fused step with ghosts held at u_n: what the unfused
+// sequence leaves there after every stage is AlgConstr(u_n + 0), so the step
+// applies AlgConstr to the state's ghosts once (after the first stage has
+// read them) and copies the result into the two stage registers.
+struct GhostArgs {
+  const DevBlock* blocks;
+  Launch L;
+  double* u;
+  double* w0;
+  double* w1;
+  double eps_floor;
+  int tf_force, force_reg;
+};
+
+__global__ void __launch_bounds__(TILE) k_rk4_ghosts(GhostArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  if (i >= B.il && i <= B.iu && j >= B.jl && j <= B.ju && k >= B.kl && k <= B.ku) return;
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const long long base = B.off * NVAR;
+  double* u = a.u + base;
+  algconstr_point(u, N, p, a.eps_floor, a.tf_force, a.force_reg);
+  double* w0 = a.w0 + base;
+  double* w1 = a.w1 + base;
+  for (int n = 0; n < NVAR; ++n) {
+    const double v = u[n * N + p];
+    w0[n * N + p] = v;
+    w1[n * N + p] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -884,6 +995,21 @@ struct z4c_ctx {
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
   int n_cu = 256;
+  // per-pass kernel timing (events recorded on the launch stream)
+  struct TimedLaunch {
+    int pass;
+    hipEvent_t a, b;
+  };
+  bool timing = false;
+  std::vector<TimedLaunch> tev;
+
+  void drop_timing() {
+    for (auto& t : tev) {
+      hipEventDestroy(t.a);
+      hipEventDestroy(t.b);
+    }
+    tev.clear();
+  }
 
   void release() {
     hipFree(d_blocks);
@@ -911,6 +1037,27 @@ struct z4c_ctx {
 };
 
 namespace {
+
+// Brackets the launches of one pass with events when timing is on.
+struct PassTimer {
+  z4c_ctx* c;
+  int pass;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  PassTimer(z4c_ctx* c_, int pass_, hipStream_t s_) : c(c_), pass(pass_), s(s_) {
+    if (c->timing && hipEventCreate(&a) == hipSuccess) hipEventRecord(a, s);
+  }
+  ~PassTimer() {
+    if (!a) return;
+    hipEvent_t b = nullptr;
+    if (hipEventCreate(&b) != hipSuccess) {
+      hipEventDestroy(a);
+      return;
+    }
+    hipEventRecord(b, s);
+    c->tev.push_back({pass, a, b});
+  }
+};
 
 int grid_for(int ntiles) { return std::max(8, (ntiles + 7) / 8 * 8); }
 
@@ -959,6 +1106,7 @@ int launch_prep(z4c_ctx* c, const z4c_params* p, const double* u, double* aux,
                 hipStream_t s) {
   PrepArgs a{c->d_blocks, launch_of(c, BOX_PREP), u, c->d1, aux, p->chi_psi_power,
              p->chi_div_floor};
+  PassTimer t(c, Z4C_PASS_PREPARE, s);
   hipLaunchKernelGGL(k_prep, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
@@ -968,17 +1116,26 @@ int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
                const double* mat, double* rhs, bool sommerfeld, hipStream_t s) {
   if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
   StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, u, c->dv, p->diss};
-  hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
-                     sizeof(double) * c->stencil_lds, s, sa);
+  {
+    PassTimer t(c, Z4C_PASS_STENCIL, s);
+    hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
+                       sizeof(double) * c->stencil_lds, s, sa);
+  }
   HIPCALL(hipGetLastError());
   RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
   const int g = grid_for(a.L.ntiles);
-  hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a);
+  {
+    PassTimer t(c, Z4C_PASS_CURVATURE, s);
+    hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a);
+  }
   HIPCALL(hipGetLastError());
-  if (sommerfeld)
-    hipLaunchKernelGGL(k_rhs_alg<true>, dim3(g), dim3(TILE), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_rhs_alg<false>, dim3(g), dim3(TILE), 0, s, a);
+  {
+    PassTimer t(c, Z4C_PASS_ASSEMBLY, s);
+    if (sommerfeld)
+      hipLaunchKernelGGL(k_rhs_alg<true>, dim3(g), dim3(TILE), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_rhs_alg<false>, dim3(g), dim3(TILE), 0, s, a);
+  }
   HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
@@ -994,6 +1151,7 @@ int launch_alg(z4c_ctx* c, const z4c_params* p, double* u, const int* range,
   a.force_reg = p->force_regularization;
   for (int q = 0; q < 6; ++q) a.range[q] = range ? range[q] : -1;
   a.skip_physical = skip;
+  PassTimer t(c, Z4C_PASS_ALGCONSTR, s);
   hipLaunchKernelGGL(k_algconstr, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
@@ -1016,6 +1174,7 @@ int launch_bt(z4c_ctx* c, int nvar, double* out, const double* u0, int nk,
     }
   }
   a.nk = m;
+  PassTimer t(c, Z4C_PASS_BT_SUM, s);
   hipLaunchKernelGGL(k_bt_sum, dim3(elementwise_grid(a.n)), dim3(256), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
@@ -1068,6 +1227,7 @@ int z4c_ctx_destroy(z4c_ctx* c) {
   if (!c) return Z4C_OK;
   hipSetDevice(c->device);
   c->release();
+  c->drop_timing();
   delete c;
   return Z4C_OK;
 }
@@ -1357,6 +1517,99 @@ int z4c_periodic_fill(z4c_ctx* c, int nvar, double* u, void* stream) {
   if (rc) return rc;
   if (!u || nvar <= 0) return fail(Z4C_EINVAL, "periodic_fill: bad argument");
   return launch_periodic(c, nvar, u, (hipStream_t)stream);
+}
+
+int z4c_set_timing(z4c_ctx* c, int enable) {
+  if (!c) return fail(Z4C_EINVAL, "set_timing: null context");
+  c->timing = enable != 0;
+  return Z4C_OK;
+}
+
+int z4c_get_timing(z4c_ctx* c, double* ms_total, long long* launches, int npass) {
+  if (!c || !ms_total || !launches || npass < 0) return fail(Z4C_EINVAL, "get_timing: bad argument");
+  HIPCALL(hipSetDevice(c->device));
+  for (int q = 0; q < npass; ++q) {
+    ms_total[q] = 0.0;
+    launches[q] = 0;
+  }
+  int rc = Z4C_OK;
+  for (auto& t : c->tev) {
+    float ms = 0.f;
+    if (hipEventSynchronize(t.b) != hipSuccess || hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) {
+      rc = fail(Z4C_EHIP, "get_timing: event query failed");
+      break;
+    }
+    if (t.pass >= 0 && t.pass < npass) {
+      ms_total[t.pass] += ms;
+      launches[t.pass] += 1;
+    }
+  }
+  c->drop_timing();
+  return rc;
+}
+
+int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                 const double* mat, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step: bad argument");
+  if (mat) return fail(Z4C_EINVAL, "rk4_step: matter sources are not enabled in this build (vacuum only)");
+  hipStream_t s = (hipStream_t)stream;
+  const long long n = c->total_pts * NVAR;
+  double* W0 = work;
+  double* W1 = work + n;
+  double* ACC = work + 2 * n;
+  // bt_rk4 (task_list_integrators_butcher.cpp:37-49): subdiagonal of A and b
+  static const double Asub[4] = {0.5, 0.5, 1.0, 0.0};
+  static const double Bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  double* const ins[4] = {u, W0, W1, W0};
+  double* const outs[4] = {W0, W1, W0, u};
+  for (int st = 1; st <= 4; ++st) {
+    const double* in = ins[st - 1];
+    StageArgs a{};
+    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, nullptr, nullptr, c->cv,
+                  rhs_params(p)};
+    a.un = u;
+    a.acc = ACC;
+    a.out = outs[st - 1];
+    a.wa = Asub[st - 1] * dt;
+    a.wb = Bw[st - 1] * dt;
+    a.stage = st;
+    a.cowling = p->cowling;
+    a.eps_floor = p->eps_floor;
+    a.tf_force = p->tf_force_regularization;
+    a.force_reg = p->force_regularization;
+    const int g = grid_for(a.r.L.ntiles);
+    if (!p->cowling) {
+      StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, in, c->dv, p->diss};
+      {
+        PassTimer t(c, Z4C_PASS_STENCIL, s);
+        hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
+                           sizeof(double) * c->stencil_lds, s, sa);
+      }
+      HIPCALL(hipGetLastError());
+      {
+        PassTimer t(c, Z4C_PASS_CURVATURE, s);
+        hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a.r);
+      }
+      HIPCALL(hipGetLastError());
+    }
+    {
+      PassTimer t(c, Z4C_PASS_ASSEMBLY, s);
+      hipLaunchKernelGGL(k_rk4_stage<true>, dim3(g), dim3(TILE), 0, s, a);
+    }
+    HIPCALL(hipGetLastError());
+    if (st == 1) {
+      GhostArgs ga{c->d_blocks, launch_of(c, BOX_WHOLE), u, W0, W1, p->eps_floor,
+                   p->tf_force_regularization, p->force_regularization};
+      {
+        PassTimer t(c, Z4C_PASS_GHOSTS, s);
+        hipLaunchKernelGGL(k_rk4_ghosts, dim3(grid_for(ga.L.ntiles)), dim3(TILE), 0, s, ga);
+      }
+      HIPCALL(hipGetLastError());
+    }
+  }
+  return launch_prep(c, p, u, nullptr, s);
 }
 
 int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double* u,

@@ -222,6 +222,33 @@ class Z4c:
         for s in range(1, 5):
             self.rk4_stage(s, dt, u, un, k, mat, periodic, stream)
 
+    def rk4_step_fused(self, dt, u, work, mat=None, stream=None):
+        """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr
+        in one kernel), ghosts held at u_n, ending with PrepareZ4cDerivatives
+        of the new state.  Equal to ``rk4_step`` with ``periodic=False``;
+        ``work`` comes from ``make_step_work``."""
+        check(self.L.z4c_rk4_step(self._ctx, C.byref(self.params), float(dt), _ptr(u),
+                                  _ptr(work), _ptr(mat), _stream(stream)), "rk4_step")
+
+    def set_timing(self, enable=True):
+        """Bracket every pass launch with HIP events on its stream."""
+        check(self.L.z4c_set_timing(self._ctx, int(enable)), "set_timing")
+
+    def get_timing(self):
+        """{pass: (total ms, launches)} since the last call; waits for the
+        recorded events."""
+        from ._capi import PASSES
+        n = len(PASSES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_longlong * n)()
+        check(self.L.z4c_get_timing(self._ctx, ms, cnt, n), "get_timing")
+        return {name: (ms[q], cnt[q]) for q, name in enumerate(PASSES)}
+
+    def make_step_work(self):
+        """Work registers of ``rk4_step_fused``: two stage states and the
+        running sum of the final combination."""
+        return self.empty(3 * Z4C_NVAR)
+
     def make_rk_registers(self):
         """u_n and the four k registers, zero-initialised (k ghosts stay 0)."""
         return self.zeros(), [self.zeros() for _ in range(4)]

@@ -178,6 +178,41 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
                   double* u, const double* un, double* const* k,
                   const double* mat, int periodic, void* stream);
 
+/* One whole bt_rk4 step of the GR_Z4c task list
+   (src/task_list/gr/task_list_gr_z4c.cpp:45-89 over stages 1..4, tableau
+   src/task_list/task_list_integrators_butcher.cpp:37-49) on all blocks, with
+   ghost vertices held at u_n (no exchange) and Sommerfeld faces honoured.
+   Per stage: the stencil and curvature passes of Z4cRHS, then one kernel
+   that assembles the RHS and applies SumBT and AlgConstr at each interior
+   vertex; the ghosts get AlgConstr(u_n) once.  Ends with
+   PrepareZ4cDerivatives of the new state, as the final stage of the task
+   list does.  The results equal z4c_rk4_stage over stages 1..4 (same sums in
+   the same order) without k registers: `work` is caller-owned device memory
+   of 3 * 22 * elems_per_var doubles (two stage states and the running sum).
+   `u` holds u_n on entry and u_{n+1} on return. */
+int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
+                 double* work, const double* mat, void* stream);
+
+/* Passes of the kernel timer. */
+enum {
+  Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */
+  Z4C_PASS_CURVATURE = 1, /* Ricci and D_i D_j alpha */
+  Z4C_PASS_ASSEMBLY = 2,  /* RHS assembly (fused with SumBT + AlgConstr in z4c_rk4_step) */
+  Z4C_PASS_GHOSTS = 3,    /* ghost pass of z4c_rk4_step */
+  Z4C_PASS_PREPARE = 4,   /* PrepareZ4cDerivatives */
+  Z4C_PASS_BT_SUM = 5,    /* SumBT */
+  Z4C_PASS_ALGCONSTR = 6, /* AlgConstr */
+  Z4C_NPASS = 7
+};
+
+/* Kernel timer (a profiling aid; no counterpart in the reference, whose
+   timing is wall-clock per cycle, src/main.cpp).  While enabled, every pass
+   launch is bracketed by HIP events on its own stream.  z4c_get_timing
+   waits for the recorded events, returns the summed milliseconds and the
+   launch count per pass (arrays of npass entries) and clears the record. */
+int z4c_set_timing(z4c_ctx* ctx, int enable);
+int z4c_get_timing(z4c_ctx* ctx, double* ms_total, long long* launches, int npass);
+
 #ifdef __cplusplus
 }
 #endif

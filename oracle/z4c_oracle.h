@@ -127,7 +127,8 @@ void oracle_adm_constraints(const oracle_params* o, const oracle_geom* g,
 void oracle_bt_sum(const oracle_geom* g, int nvar, double* u_out,
                    const double* u0, int nk, const double* const* k,
                    const double* w, double dt);
-/* WeightedAve (VC, whole block) and AddZ4cRHS (interior). */
+/* WeightedAve (WeightedAveVC: owned vertices is..ive) and AddZ4cRHS
+   (owned vertices). */
 void oracle_weighted_ave(const oracle_geom* g, int nvar, double* u_out,
                          const double* u_in1, const double* u_in2,
                          const double* wght);
