@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-perblock", action="store_true", help="skip the per-block launch comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
 
 
@@ -189,6 +191,12 @@ def main():
                        "kernel": dom, "avg_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes"]}
     out["passes"] = pinfo
 
+    if args.calib_copy:
+        tmp = torch.empty_like(u)
+        tmp.copy_(u)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 2 * u.numel() * 8
+        del tmp
     if not args.no_perblock:
         out["per_block_launch"] = per_block(z, blocks, params, u, dt, owned)
         out["batched_speedup_vs_per_block"] = rate / out["per_block_launch"]["value"]

@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(StencilArgs a) {
   const int w = blockIdx.x;
   const int t = (w & 7) * (G >> 3) + (w >> 3);
   if (t >= a.ntiles) return;
-  stencil_tile(a, a.tiles[t], (int)blockIdx.y, sm);
+  const PlaneTile T = a.tiles[t];
+  stencil_tile(a, T, (int)blockIdx.y, sm);
 }
 
 __device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B, int p,
@@ -436,6 +437,73 @@ __global__ void __launch_bounds__(TILE) k_rk4_ghosts(GhostArgs a) {
     const double v = u[n * N + p];
     w0[n * N + p] = v;
     w1[n * N + p] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// History sums of the Z4c constraints (src/outputs/history.cpp:178-205): over
+// the reference's cell-index loop is..ie (vertices il..iu-1 per axis), with
+// cell volume dx*dy*dz and the r_max_con cut on the cell-centre radius.
+// One partial sum per tile; the host adds the partials in tile order.
+// ---------------------------------------------------------------------------
+constexpr int NHST = 8;
+struct HstArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* u;
+  const double* con;
+  double r2max;
+  double* partial;  // NHST per tile
+};
+
+__global__ void __launch_bounds__(TILE) k_history(HstArgs a) {
+  __shared__ double red[NHST][TILE / 64];
+  double v[NHST];
+#pragma unroll
+  for (int q = 0; q < NHST; ++q) v[q] = 0.0;
+  int b, i, j, k;
+  const int G = gridDim.x;
+  const int t = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  if (tile_point(a.L, b, i, j, k)) {
+    const DevBlock B = a.blocks[b];
+    if (i < B.iu && j < B.ju && k < B.ku) {
+      const double x1 = B.x0[0] + (i + 0.5) * B.dx[0];
+      const double x2 = B.x0[1] + (j + 0.5) * B.dx[1];
+      const double x3 = B.x0[2] + (k + 0.5) * B.dx[2];
+      const double R2 = x1 * x1 + x2 * x2 + x3 * x3;
+      if (R2 <= a.r2max) {
+        const int N = B.npts;
+        const int p = flat(B, i, j, k);
+        const double* c = a.con + B.off * NCON;
+        const double vol = B.dx[0] * B.dx[1] * B.dx[2];
+        const double H = fabs(c[C_H * N + p]);
+        const double Mx = fabs(c[(C_MX + 0) * N + p]);
+        const double My = fabs(c[(C_MX + 1) * N + p]);
+        const double Mz = fabs(c[(C_MX + 2) * N + p]);
+        const double th = fabs(a.u[B.off * NVAR + V_THETA * N + p]);
+        v[0] = vol * (H * H);
+        v[1] = vol * fabs(c[C_M * N + p]);
+        v[2] = vol * (Mx * Mx);
+        v[3] = vol * (My * My);
+        v[4] = vol * (Mz * Mz);
+        v[5] = vol * fabs(c[C_Z * N + p]);
+        v[6] = vol * (th * th);
+        v[7] = vol * fabs(c[C_C * N + p]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NHST; ++q) {
+    double x = v[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NHST && t < a.L.ntiles) {
+    double x = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) x += red[threadIdx.x][w];
+    a.partial[(long long)t * NHST + threadIdx.x] = x;
   }
 }
 
@@ -994,6 +1062,8 @@ struct z4c_ctx {
   long long total_int = 0;  // interior points
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
+  double* hst = nullptr;   // history partial sums (lazy)
+  std::vector<double> hst_host;
   int n_cu = 256;
   // per-pass kernel timing (events recorded on the launch stream)
   struct TimedLaunch {
@@ -1025,6 +1095,9 @@ struct z4c_ctx {
     hipFree(cv);
     hipFree(d_ptiles);
     hipFree(snap);
+    hipFree(hst);
+    hst = nullptr;
+    hst_host.clear();
     d_blocks = nullptr;
     d1 = nullptr;
     dv = nullptr;
@@ -1517,6 +1590,31 @@ int z4c_periodic_fill(z4c_ctx* c, int nvar, double* u, void* stream) {
   if (rc) return rc;
   if (!u || nvar <= 0) return fail(Z4C_EINVAL, "periodic_fill: bad argument");
   return launch_periodic(c, nvar, u, (hipStream_t)stream);
+}
+
+int z4c_history(z4c_ctx* c, const z4c_params* p, const double* u, const double* con,
+                double* out, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !con || !out) return fail(Z4C_EINVAL, "history: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const Launch L = launch_of(c, BOX_INTERIOR);
+  if (!c->hst) {
+    if (hipMalloc(&c->hst, sizeof(double) * NHST * (size_t)L.ntiles) != hipSuccess)
+      return fail(Z4C_ENOMEM, "history: allocation failed");
+    c->hst_host.assign((size_t)NHST * L.ntiles, 0.0);
+  }
+  const double r = p->r_max_con;
+  HstArgs a{c->d_blocks, L, u, con, r * r, c->hst};
+  hipLaunchKernelGGL(k_history, dim3(grid_for(L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  HIPCALL(hipMemcpyAsync(c->hst_host.data(), c->hst, sizeof(double) * NHST * (size_t)L.ntiles,
+                         hipMemcpyDeviceToHost, s));
+  HIPCALL(hipStreamSynchronize(s));
+  for (int q = 0; q < NHST; ++q) out[q] = 0.0;
+  for (int t = 0; t < L.ntiles; ++t)
+    for (int q = 0; q < NHST; ++q) out[q] += c->hst_host[(size_t)t * NHST + q];
+  return Z4C_OK;
 }
 
 int z4c_set_timing(z4c_ctx* c, int enable) {

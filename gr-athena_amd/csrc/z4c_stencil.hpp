@@ -57,14 +57,17 @@ __host__ __device__ constexpr int stencil_lds_doubles(int ni, int nj) {
   return 2 * stencil_lds_buffer(ni, nj);
 }
 
-__device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTile& T,
+// The tile and block descriptors are taken by value: held in registers, they
+// are not reloaded (and waited for) after every store of the z loop.
+__device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTile T,
                                              int f, double* sm) {
   const DevBlock B = a.blocks[T.block];
   const int N = B.npts;
   const int sz = B.nn1 * B.nn2;  // z stride
-  const double* uf = a.u + B.off * NVAR + (long long)f * N;
-  const double* ubeta = a.u + B.off * NVAR + (long long)V_BETAX * N;
-  double* dv = a.dv + B.ioff * NDV;
+  const double* __restrict__ uf = a.u + B.off * NVAR + (long long)f * N;
+  const double* __restrict__ ubeta = a.u + B.off * NVAR + (long long)V_BETAX * N;
+  double* __restrict__ dv = a.dv + B.ioff * NDV;
+  const int db = dbase(f), sl = s_L(f), sd = s_D(f);
   const int NI = B.nint;
   const int PX = T.ni + 8, PP = PX * (T.nj + 8);
   const int EX = T.ni + 6, EE = EX * (T.nj + 6);
@@ -97,7 +100,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   }
 
   // registers: next plane, z rings (zr: z-4..z+4 at the point; er: z-3..z+3)
-  double pc[ST_CUR], zr[9], er[ST_EXT][7], pz = 0.0, pe[ST_EXT];
+  double pc[ST_CUR], zr[9], er[ST_EXT][7], pz = 0.0, pe[ST_EXT], pb[3] = {0.0, 0.0, 0.0};
   {
     const int k0 = B.kl;
 #pragma unroll
@@ -108,6 +111,8 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
 #pragma unroll
       for (int m = 0; m < 8; ++m) zr[m] = uf[pxy + sz * (k0 + m - 4)];
       pz = uf[pxy + sz * (k0 + 4)];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pb[c] = ubeta[c * N + pxy + sz * k0];
     }
 #pragma unroll
     for (int r = 0; r < ST_EXT; ++r) {
@@ -127,6 +132,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
     double* dyp = cur + PP;                     // Dx_y(u) on the tile +-3
     double* dzp = dyp + EE;                     // Dx_z(u) on the tile +-3
     zr[8] = pz;
+    const double bx = pb[0], by = pb[1], bz = pb[2];  // shift at the point
 #pragma unroll
     for (int r = 0; r < ST_CUR; ++r)
       if (tid + r * ST_THREADS < PP) cur[tid + r * ST_THREADS] = pc[r];
@@ -162,14 +168,17 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       const int k1 = kz + 1;
 #pragma unroll
       for (int r = 0; r < ST_CUR; ++r) pc[r] = cofs[r] >= 0 ? uf[cofs[r] + sz * k1] : 0.0;
-      if (active) pz = uf[pxy + sz * (k1 + 4)];
+      if (active) {
+        pz = uf[pxy + sz * (k1 + 4)];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pb[c] = ubeta[c * N + pxy + sz * k1];
+      }
 #pragma unroll
       for (int r = 0; r < ST_EXT; ++r)
         if (eofs[r] >= 0) pe[r] = uf[eofs[r] + sz * (k1 + 3)];
     }
     if (active) {
       const int qi = (T.i0 + ti) + B.ni * ((T.j0 + tj) + B.nj * (kz - B.kl));
-      const int p = pxy + sz * kz;
       double xv[9], yv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -177,36 +186,36 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
         yv[m] = cur[o + (m - 4) * PX];
       }
       if (grad) {
-        dv[(long long)(dbase(f) + 0) * NI + qi] = st_dx(xv, B.cidx1[0]);
-        dv[(long long)(dbase(f) + 1) * NI + qi] = st_dx(yv, B.cidx1[1]);
-        dv[(long long)(dbase(f) + 2) * NI + qi] = st_dx(zr, B.cidx1[2]);
+        dv[(long long)(db + 0) * NI + qi] = st_dx(xv, B.cidx1[0]);
+        dv[(long long)(db + 1) * NI + qi] = st_dx(yv, B.cidx1[1]);
+        dv[(long long)(db + 2) * NI + qi] = st_dx(zr, B.cidx1[2]);
       }
       if (full) {
         double v[9];
-        dv[(long long)(dbase(f) + 3 + 0) * NI + qi] = st_dxx(xv, B.cidx2[0]);
-        dv[(long long)(dbase(f) + 3 + 3) * NI + qi] = st_dxx(yv, B.cidx2[1]);
-        dv[(long long)(dbase(f) + 3 + 5) * NI + qi] = st_dxx(zr, B.cidx2[2]);
+        dv[(long long)(db + 3 + 0) * NI + qi] = st_dxx(xv, B.cidx2[0]);
+        dv[(long long)(db + 3 + 3) * NI + qi] = st_dxx(yv, B.cidx2[1]);
+        dv[(long long)(db + 3 + 5) * NI + qi] = st_dxx(zr, B.cidx2[2]);
         v[0] = v[8] = 0.0;
 #pragma unroll
         for (int m = 1; m < 8; ++m) v[m] = dyp[w + (m - 4)];
-        dv[(long long)(dbase(f) + 3 + 1) * NI + qi] = st_dx(v, B.cidx1[0]);  // d_x(d_y u)
+        dv[(long long)(db + 3 + 1) * NI + qi] = st_dx(v, B.cidx1[0]);  // d_x(d_y u)
 #pragma unroll
         for (int m = 1; m < 8; ++m) v[m] = dzp[w + (m - 4)];
-        dv[(long long)(dbase(f) + 3 + 2) * NI + qi] = st_dx(v, B.cidx1[0]);  // d_x(d_z u)
+        dv[(long long)(db + 3 + 2) * NI + qi] = st_dx(v, B.cidx1[0]);  // d_x(d_z u)
 #pragma unroll
         for (int m = 1; m < 8; ++m) v[m] = dzp[w + (m - 4) * EX];
-        dv[(long long)(dbase(f) + 3 + 4) * NI + qi] = st_dx(v, B.cidx1[1]);  // d_y(d_z u)
+        dv[(long long)(db + 3 + 4) * NI + qi] = st_dx(v, B.cidx1[1]);  // d_y(d_z u)
       }
       // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
       double L = 0.0;
-      L += st_lx(xv, B.lidx[0], ubeta[p]);
-      L += st_lx(yv, B.lidx[1], ubeta[N + p]);
-      L += st_lx(zr, B.lidx[2], ubeta[2 * N + p]);
-      dv[(long long)s_L(f) * NI + qi] = L;
+      L += st_lx(xv, B.lidx[0], bx);
+      L += st_lx(yv, B.lidx[1], by);
+      L += st_lx(zr, B.lidx[2], bz);
+      dv[(long long)sl * NI + qi] = L;
       double D = st_diss(xv, B.cidxd[0], a.diss);
       D += st_diss(yv, B.cidxd[1], a.diss);
       D += st_diss(zr, B.cidxd[2], a.diss);
-      dv[(long long)s_D(f) * NI + qi] = D;
+      dv[(long long)sd * NI + qi] = D;
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) zr[m] = zr[m + 1];

@@ -230,6 +230,17 @@ class Z4c:
         check(self.L.z4c_rk4_step(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                   _ptr(work), _ptr(mat), _stream(stream)), "rk4_step")
 
+    HISTORY_COLUMNS = ["H-norm2", "M-norm2", "Mx-norm2", "My-norm2", "Mz-norm2",
+                       "Z-norm2", "Theta-norm2", "C-norm2"]
+
+    def history(self, u, con, stream=None):
+        """The Z4c columns of the reference's .hst output (history.cpp:178-205)
+        for state ``u`` and constraints ``con``; waits for the stream."""
+        out = (C.c_double * 8)()
+        check(self.L.z4c_history(self._ctx, C.byref(self.params), _ptr(u), _ptr(con), out,
+                                 _stream(stream)), "history")
+        return list(out)
+
     def set_timing(self, enable=True):
         """Bracket every pass launch with HIP events on its stream."""
         check(self.L.z4c_set_timing(self._ctx, int(enable)), "set_timing")

@@ -193,6 +193,15 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
 int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                  double* work, const double* mat, void* stream);
 
+/* Z4c constraint sums of the history output (src/outputs/history.cpp:178-205):
+   out[8] = sum over blocks and over the reference's cell-index loop (vertices
+   il..iu-1, jl..ju-1, kl..ku-1) of vol*{H^2, |M|, Mx^2, My^2, Mz^2, |Z|,
+   Theta^2, |C|} with vol = dx*dy*dz, for cell centres inside r_max_con
+   (the columns H-norm2 .. C-norm2 of the .hst file).  `con` comes from
+   z4c_adm_constraints.  Waits for the stream; out is host memory. */
+int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                const double* con, double* out, void* stream);
+
 /* Passes of the kernel timer. */
 enum {
   Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */

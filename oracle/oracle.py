@@ -136,6 +136,7 @@ def lib():
         L.oracle_new_dt.restype = d
         L.oracle_new_dt.argtypes = [pg, d]
         L.oracle_periodic_fill.argtypes = [pg, C.c_int, pd]
+        L.oracle_history.argtypes = [pp, pg, pd, pd, pd]
     return _lib
 
 
@@ -229,6 +230,13 @@ def add_rhs(g, rhs_, wght, u_out):
 
 def new_dt(g, cfl):
     return lib().oracle_new_dt(C.byref(g), cfl)
+
+
+def history(o, g, u, con, out=None):
+    """Z4c .hst columns of one block, accumulated into ``out`` (8 values)."""
+    acc = (C.c_double * 8)(*(out if out is not None else [0.0] * 8))
+    lib().oracle_history(C.byref(o), C.byref(g), _p(u), _p(con), acc)
+    return list(acc)
 
 
 def periodic_fill(g, u):

@@ -1174,3 +1174,37 @@ void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u) {
   }
   free(snap);
 }
+
+/* ------------------------------------------------------------------------ */
+/* History sums of the Z4c constraints (src/outputs/history.cpp:178-205).    */
+/* The reference loops over cell indices is..ie of the vertex arrays and     */
+/* tests the cell-centre radius x1v = x1f + dx/2.                            */
+/* ------------------------------------------------------------------------ */
+void oracle_history(const oracle_params* o, const oracle_geom* g,
+                    const double* u, const double* con, double* out) {
+  const long N = (long)g->nn1 * g->nn2 * g->nn3;
+  const double r2max = o->r_max_con * o->r_max_con;
+  const double vol = g->dx[0] * g->dx[1] * g->dx[2];
+  for (int k = g->kl; k < g->ku; ++k)
+    for (int j = g->jl; j < g->ju; ++j)
+      for (int i = g->il; i < g->iu; ++i) {
+        const double x1 = g->x0[0] + (i + 0.5) * g->dx[0];
+        const double x2 = g->x0[1] + (j + 0.5) * g->dx[1];
+        const double x3 = g->x0[2] + (k + 0.5) * g->dx[2];
+        if (x1 * x1 + x2 * x2 + x3 * x3 > r2max) continue;
+        const long p = i + (long)g->nn1 * (j + (long)g->nn2 * k);
+        const double H = fabs(con[OR_CON_H * N + p]);
+        const double Mx = fabs(con[(OR_CON_MX + 0) * N + p]);
+        const double My = fabs(con[(OR_CON_MX + 1) * N + p]);
+        const double Mz = fabs(con[(OR_CON_MX + 2) * N + p]);
+        const double th = fabs(u[OR_THETA * N + p]);
+        out[0] += vol * (H * H);
+        out[1] += vol * fabs(con[OR_CON_M * N + p]);
+        out[2] += vol * (Mx * Mx);
+        out[3] += vol * (My * My);
+        out[4] += vol * (Mz * Mz);
+        out[5] += vol * fabs(con[OR_CON_Z * N + p]);
+        out[6] += vol * (th * th);
+        out[7] += vol * fabs(con[OR_CON_C * N + p]);
+      }
+}

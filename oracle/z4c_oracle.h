@@ -136,6 +136,12 @@ void oracle_add_rhs(const oracle_geom* g, int nvar, const double* rhs,
                     double wght, double* u_out);
 double oracle_new_dt(const oracle_geom* g, double cfl);
 
+/* Z4c columns of the history output (src/outputs/history.cpp:178-205) of
+   one block: out[8] += vol*{H^2,|M|,Mx^2,My^2,Mz^2,|Z|,Theta^2,|C|} over the
+   cell-index loop (vertices il..iu-1 etc.), cell centres within r_max_con. */
+void oracle_history(const oracle_params* o, const oracle_geom* g,
+                    const double* u, const double* con, double* out);
+
 /* Single-block periodic vertex-centred ghost fill (all three axes). */
 void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u);
 

@@ -1,0 +1,79 @@
+"""The periodic gauge-wave configuration on the device against the oracle:
+initial data (ADMToZ4c + periodic exchange + ghost AlgConstr), bt_rk4 steps
+with the periodic exchange after every stage, and the Z4c history columns
+(src/outputs/history.cpp:178-205)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from z4c_amd import awa  # noqa: E402
+from z4c_amd.initial_data import gauge_wave_adm  # noqa: E402
+
+
+def _oracle_run(geom, params, nsteps, dt):
+    o = orc.Params(**params.as_dict())
+    g = orc.Geom.block(geom.nx, geom.dx, geom.xmin)
+    u = orc.adm_to_z4c(o, g, np.ascontiguousarray(gauge_wave_adm(geom, A=0.01, d=1.0)))
+    orc.periodic_fill(g, u)
+    orc.algconstr_range(o, g, u, 0, g.nn1 - 1, 0, g.nn2 - 1, 0, g.nn3 - 1, 1)
+
+    def hist(u):
+        adm = orc.z4c_to_adm(o, g, u)
+        con = orc.adm_constraints(o, g, u, adm)
+        return orc.history(o, g, u, con)
+    rows = [hist(u)]
+    for _ in range(nsteps):
+        u, _ = orc.rk4_step(o, g, u, dt, between_stages=lambda c: orc.periodic_fill(g, c))
+        rows.append(hist(u))
+    return u, np.array(rows)
+
+
+@pytest.mark.parametrize("gauge", ["harmonic", "puncture"])
+def test_gauge_wave_steps_and_history(gauge):
+    n, nsteps = 16, 3
+    geom, params, z, u = awa.gauge_wave_setup(n=n, gauge=gauge)
+    dt = z.NewBlockTimeStep(0.25)
+    un, k = z.make_rk_registers()
+    rows = [awa.constraint_history(z, u)]
+    for _ in range(nsteps):
+        z.rk4_step(dt, u, un, k, periodic=True)
+        rows.append(awa.constraint_history(z, u))
+    torch.cuda.synchronize()
+    exp_u, exp_rows = _oracle_run(geom, params, nsteps, dt)
+    got = z.download(u, 22)[0]
+    num = np.abs(got - exp_u).reshape(22, -1).max(axis=1)
+    den = np.abs(exp_u).reshape(22, -1).max(axis=1)
+    r = np.where(den > 0, num / np.where(den > 0, den, 1), num)
+    assert np.all(r <= 1e-12), r
+    rows = np.array(rows)
+    # H-norm2 and C-norm2 are sums of squares of cancellation-dominated
+    # constraint values: agree to a relative 1e-8 or an absolute 1e-24
+    ok = np.isclose(rows, exp_rows, rtol=1e-8, atol=1e-24)
+    assert ok.all(), (rows, exp_rows)
+
+
+def test_history_of_known_fields():
+    """H = 2, M = 3, Mx..Mz = 1, Z = 5, C = 7, Theta = 0.5 on one block:
+    each column is value * (nx^3 cells) * (cell volume); a cut r_max_con
+    removes the cells whose centre lies outside it."""
+    from z4c_amd import Params
+    from z4c_amd.z4c import MeshBlockGeom, Z4c
+    g = MeshBlockGeom.cube(6, 0.5, -1.5)
+    z = Z4c([g], Params())
+    con = np.zeros((7,) + g.shape)
+    con[0], con[1], con[2], con[3], con[4:7] = 7.0, 2.0, 3.0, 5.0, 1.0
+    u = np.zeros((22,) + g.shape)
+    u[17] = 0.5
+    out = z.history(z.upload([u]), z.upload([con]))
+    V = 6 ** 3 * 0.5 ** 3
+    np.testing.assert_allclose(out, np.array([4, 3, 1, 1, 1, 5, 0.25, 7]) * V, rtol=1e-14)
+    z.params.r_max_con = 1.0
+    out = z.history(z.upload([u]), z.upload([con]))
+    c = -1.5 + 0.25 + 0.5 * np.arange(6)
+    inside = sum(1 for a in c for b in c for e in c if a * a + b * b + e * e <= 1.0)
+    np.testing.assert_allclose(out[0], 4 * inside * 0.125, rtol=1e-14)

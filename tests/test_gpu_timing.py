@@ -1,0 +1,29 @@
+"""The kernel timer of the C-ABI (z4c_set_timing / z4c_get_timing): every
+pass launch of a fused RK4 step is recorded on its stream."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from z4c_amd import Params  # noqa: E402
+from z4c_amd.initial_data import perturbed_flat  # noqa: E402
+from z4c_amd.z4c import MeshBlockGeom, Z4c  # noqa: E402
+
+
+def test_pass_timer_counts_fused_step():
+    gs = [MeshBlockGeom.cube(8, 0.125, 0.0), MeshBlockGeom.cube(6, 0.25, 1.0)]
+    z = Z4c(gs, Params(diss=0.02))
+    u = z.upload([perturbed_flat(g, seed=b) for b, g in enumerate(gs)])
+    work = z.make_step_work()
+    z.set_timing(True)
+    z.rk4_step_fused(0.01, u, work)
+    z.set_timing(False)
+    t = z.get_timing()
+    counts = {k: v[1] for k, v in t.items()}
+    assert counts == {"stencil": 4, "curvature": 4, "assembly": 4, "ghosts": 1,
+                      "prepare": 1, "bt_sum": 0, "algconstr": 0}
+    assert all(v[0] > 0 for v in t.values() if v[1] > 0)
+    # the record is cleared by reading it; nothing is recorded while off
+    z.rk4_step_fused(0.01, u, work)
+    assert all(v[1] == 0 for v in z.get_timing().values())

@@ -250,3 +250,20 @@ def test_adm_to_z4c_against_closed_form():
                                        rtol=1e-10, atol=1e-13)
     ratio = errs[0] / errs[1]
     assert np.all((ratio > 40) & (ratio < 100)), ratio
+
+
+def test_history_sums_known_fields():
+    """The oracle's .hst sums: constant fields give value * cells * volume
+    over the reference's cell loop (nx^3 cells, not (nx+1)^3 vertices)."""
+    g = orc.Geom.block(6, 0.5, -1.5)
+    con = orc.zeros(7, g)
+    con[0], con[1], con[2], con[3], con[4:7] = 7.0, 2.0, 3.0, 5.0, 1.0
+    u = orc.zeros(22, g)
+    u[17] = 0.5
+    out = orc.history(orc.Params(), g, u, con)
+    V = 6 ** 3 * 0.5 ** 3
+    np.testing.assert_allclose(out, np.array([4, 3, 1, 1, 1, 5, 0.25, 7]) * V, rtol=1e-14)
+    out = orc.history(orc.Params(r_max_con=1.0), g, u, con)
+    c = -1.5 + 0.25 + 0.5 * np.arange(6)
+    inside = sum(1 for a in c for b in c for e in c if a * a + b * b + e * e <= 1.0)
+    np.testing.assert_allclose(out[0], 4 * inside * 0.125, rtol=1e-14)
