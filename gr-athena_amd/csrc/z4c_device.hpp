@@ -42,9 +42,16 @@ __host__ __device__ constexpr int sym(int a, int b) {
 }
 
 // Device-side description of one MeshBlock.
+// Derivative slots of a block: NDV rows of nis = nint + DV_PAD doubles; the
+// DV_PAD entries past the interior take the stores of stencil threads that
+// lie past their tile (never read).
+constexpr int DV_PAD = 256;
+
 struct DevBlock {
   long long off;     // point offset in state buffers (block data at off*nvar)
   long long ioff;    // interior-point offset in interior-indexed buffers
+  long long doff;    // offset of the block's derivative slots (NDV * nis each)
+  int nis;           // derivative-slot row length, nint + DV_PAD
   int nn1, nn2, nn3;
   int il, iu, jl, ju, kl, ku;
   int npts;          // nn1*nn2*nn3
@@ -138,6 +145,11 @@ __device__ __forceinline__ Buf make_buf(const double* base, long long nbytes) {
 __device__ __forceinline__ double bload(const Buf& b, unsigned vbyte, int sbyte) {
   return __builtin_bit_cast(double,
                             __builtin_amdgcn_raw_buffer_load_b64(b.r, vbyte, sbyte, 0));
+}
+
+__device__ __forceinline__ void bstore(const Buf& b, double v, unsigned vbyte, int sbyte) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vbyte, sbyte, 0);
 }
 
 // Per-thread addressing of one point.

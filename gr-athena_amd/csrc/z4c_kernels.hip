@@ -196,15 +196,25 @@ __device__ __forceinline__ bool on_sommerfeld_face(const DevBlock& B, int i,
          ((f & 16) && k == B.kl) || ((f & 32) && k == B.ku);
 }
 
-// Stencil pass: grid (tiles, 22 fields), z-marching over the tile.
+// Stencil pass: one workgroup per (tile, field), z-marching over the tile.
+// The field index varies fastest and each XCD takes a contiguous range of
+// (tile, field) pairs, so the 22 fields of a tile stream through the same
+// L2 together: the shift planes every field reads, and the halo rows of
+// neighbouring tiles, come from HBM once.
 __global__ void __launch_bounds__(ST_THREADS) k_stencil(StencilArgs a) {
-  extern __shared__ double sm[];
-  const int G = gridDim.x;  // multiple of 8: XCD-contiguous tile ranges
+  __shared__ double sm[ST_LDS];
+  const int G = gridDim.x;  // multiple of 8: XCD-contiguous ranges
   const int w = blockIdx.x;
   const int t = (w & 7) * (G >> 3) + (w >> 3);
-  if (t >= a.ntiles) return;
-  const PlaneTile T = a.tiles[t];
-  stencil_tile(a, T, (int)blockIdx.y, sm);
+  if (t >= a.ntiles * NVAR) return;
+  const int tile = t / NVAR;
+  const int f = t - tile * NVAR;
+  const PlaneTile T = a.tiles[tile];
+  switch (dkind(f)) {
+    case DK_FULL: stencil_tile<DK_FULL>(a, T, f, sm); break;
+    case DK_GRAD: stencil_tile<DK_GRAD>(a, T, f, sm); break;
+    default: stencil_tile<DK_ADV>(a, T, f, sm); break;
+  }
 }
 
 __device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B, int p,
@@ -212,33 +222,43 @@ __device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B,
   PointIO X;
   const int N = B.npts;
   X.U = make_buf(a.u + B.off * NVAR, (long long)NVAR * N * 8);
-  X.DV = make_buf(a.dv + B.ioff * NDV, (long long)NDV * B.nint * 8);
+  X.DV = make_buf(a.dv + B.doff, (long long)NDV * B.nis * 8);
   X.has_mat = a.mat != nullptr;
   X.MT = X.has_mat ? make_buf(a.mat + B.off * NMAT, (long long)NMAT * N * 8) : X.U;
   X.vu = (unsigned)p * 8u;
   X.vd = (unsigned)qi * 8u;
   X.fu = N * 8;
-  X.fd = B.nint * 8;
+  X.fd = B.nis * 8;
   return X;
 }
 
 // Algebra pass, curvature part: one thread per interior point.
+// Outputs staged in LDS and written at the end (see k_rhs_alg).
 __global__ void __launch_bounds__(TILE) k_rhs_curv(RhsArgs a) {
+  __shared__ double ko[12 * TILE];
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
+  const int t = threadIdx.x;
   const DevBlock B = a.blocks[b];
   const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
   const PointIO X = point_io(a, B, flat(B, i, j, k), qi);
+  rhs_curvature_point(a.o, X, [&](int n, double v) { ko[n * TILE + t] = v; });
   double* cv = a.cv + B.ioff * 12;
   const int NI = B.nint;
-  rhs_curvature_point(a.o, X, [&](int n, double v) { cv[n * NI + qi] = v; });
+#pragma unroll
+  for (int n = 0; n < 12; ++n) cv[n * NI + qi] = ko[n * TILE + t];
 }
 
 // Algebra pass, assembly part.
+// The outputs are staged in LDS (each thread its own column) and written at
+// the end: with no global store among the loads, the compiler is free to
+// issue the derivative loads early.
 template <bool WITH_SOMMERFELD>
 __global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
+  __shared__ double ko[NVAR * TILE];
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
+  const int t = threadIdx.x;
   const DevBlock B = a.blocks[b];
   const int N = B.npts;
   const int p = flat(B, i, j, k);
@@ -246,14 +266,16 @@ __global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
   const PointIO X = point_io(a, B, p, qi);
   const Buf CV = make_buf(a.cv + B.ioff * 12, (long long)12 * B.nint * 8);
   const int fc = B.nint * 8;
-  double* rhs = a.rhs + B.off * NVAR;
   rhs_assembly_point(a.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
-                     [&](int n, double v) { rhs[n * N + p] = v; });
+                     [&](int n, double v) { ko[n * TILE + t] = v; });
   if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
     const Pt P = make_pt(p, B.nn1, B.nn2, N);
     sommerfeld_point(B, X.U, P, a.u + B.off * NVAR, i, j, k, N,
-                     [&](int n, double v) { rhs[n * N + p] = v; });
+                     [&](int n, double v) { ko[n * TILE + t] = v; });
   }
+  double* rhs = a.rhs + B.off * NVAR;
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) rhs[n * N + p] = ko[n * TILE + t];
 }
 
 __global__ void __launch_bounds__(TILE) k_sommerfeld(RhsArgs a) {
@@ -283,15 +305,10 @@ struct AlgArgs {
   int skip_physical;
 };
 
-__device__ __forceinline__ void algconstr_point(double* __restrict__ u, int N,
-                                                int p, double eps_floor,
-                                                int tf_force, int force_reg) {
-  double g[6], A[6];
-#pragma unroll
-  for (int ab = 0; ab < 6; ++ab) {
-    g[ab] = u[(V_GXX + ab) * N + p];
-    A[ab] = u[(V_AXX + ab) * N + p];
-  }
+// AlgConstr of one point's values (z4c.cpp:677-799); `chi` and `alpha` are
+// changed only with force_reg.
+__device__ __forceinline__ void algconstr_vals(double* g, double* A, double& chi, double& alpha,
+                                               double eps_floor, int tf_force, int force_reg) {
   double detg = det3(g);
   bool project = true;
   if (tf_force)
@@ -305,15 +322,32 @@ __device__ __forceinline__ void algconstr_point(double* __restrict__ u, int N,
     const double tr = trace_rank2(1.0, g, A);
 #pragma unroll
     for (int ab = 0; ab < 6; ++ab) A[ab] -= (1.0 / 3.0) * tr * g[ab];
-#pragma unroll
-    for (int ab = 0; ab < 6; ++ab) {
-      u[(V_GXX + ab) * N + p] = g[ab];
-      u[(V_AXX + ab) * N + p] = A[ab];
-    }
   }
   if (force_reg) {
-    u[V_CHI * N + p] = dmax(u[V_CHI * N + p], eps_floor);
-    u[V_ALPHA * N + p] = regularize_near_zero(u[V_ALPHA * N + p], eps_floor);
+    chi = dmax(chi, eps_floor);
+    alpha = regularize_near_zero(alpha, eps_floor);
+  }
+}
+
+__device__ __forceinline__ void algconstr_point(double* __restrict__ u, int N,
+                                                int p, double eps_floor,
+                                                int tf_force, int force_reg) {
+  double g[6], A[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    g[ab] = u[(V_GXX + ab) * N + p];
+    A[ab] = u[(V_AXX + ab) * N + p];
+  }
+  double chi = u[V_CHI * N + p], alpha = u[V_ALPHA * N + p];
+  algconstr_vals(g, A, chi, alpha, eps_floor, tf_force, force_reg);
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    u[(V_GXX + ab) * N + p] = g[ab];
+    u[(V_AXX + ab) * N + p] = A[ab];
+  }
+  if (force_reg) {
+    u[V_CHI * N + p] = chi;
+    u[V_ALPHA * N + p] = alpha;
   }
 }
 
@@ -357,54 +391,56 @@ struct StageArgs {
   int tf_force, force_reg;
 };
 
-__device__ __forceinline__ bool sommerfeld_var(int n) {
-  return n == V_THETA || n == V_KHAT || (n >= V_GAMX && n < V_GAMX + 3) ||
-         (n >= V_AXX && n < V_AXX + 6);
-}
-
 template <bool WITH_SOMMERFELD>
 __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
+  __shared__ double ko[NVAR * TILE];  // this point's k_s, staged (see k_rhs_alg)
   int b, i, j, k;
   if (!tile_point(a.r.L, b, i, j, k)) return;
+  const int t = threadIdx.x;
   const DevBlock B = a.r.blocks[b];
   const int N = B.npts;
   const int p = flat(B, i, j, k);
   const long long base = B.off * NVAR;
-  const double* un = a.un + base;
-  double* acc = a.acc + base;
-  double* out = a.out + base;
-  const int stage = a.stage;
-  const double wa = a.wa, wb = a.wb;
-  auto update = [&](int n, double kv) {
-    const int e = n * N + p;
-    if (stage == 1) {
-      acc[e] = un[e] + wb * kv;
-      out[e] = un[e] + wa * kv;
-    } else if (stage < 4) {
-      acc[e] = acc[e] + wb * kv;
-      out[e] = un[e] + wa * kv;
-    } else {
-      out[e] = acc[e] + wb * kv;
-    }
-  };
   if (a.cowling) {
-    for (int n = 0; n < NVAR; ++n) update(n, 0.0);
+    for (int n = 0; n < NVAR; ++n) ko[n * TILE + t] = 0.0;
   } else {
     const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
     const PointIO X = point_io(a.r, B, p, qi);
     const Buf CV = make_buf(a.r.cv + B.ioff * 12, (long long)12 * B.nint * 8);
     const int fc = B.nint * 8;
-    const bool face = WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k);
     rhs_assembly_point(a.r.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
-                       [&](int n, double v) {
-                         if (!(face && sommerfeld_var(n))) update(n, v);
-                       });
-    if (WITH_SOMMERFELD && face) {
+                       [&](int n, double v) { ko[n * TILE + t] = v; });
+    if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
       const Pt P = make_pt(p, B.nn1, B.nn2, N);
-      sommerfeld_point(B, X.U, P, a.r.u + base, i, j, k, N, update);
+      sommerfeld_point(B, X.U, P, a.r.u + base, i, j, k, N,
+                       [&](int n, double v) { ko[n * TILE + t] = v; });
     }
   }
-  algconstr_point(out, N, p, a.eps_floor, a.tf_force, a.force_reg);
+  // stage update of every variable, then AlgConstr of the point
+  const double* un = a.un + base;
+  double* acc = a.acc + base;
+  double* out = a.out + base;
+  const int stage = a.stage;
+  const double wa = a.wa, wb = a.wb;
+  double s[NVAR];
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) {
+    const int e = n * N + p;
+    const double kv = ko[n * TILE + t];
+    if (stage == 1) {
+      acc[e] = un[e] + wb * kv;
+      s[n] = un[e] + wa * kv;
+    } else if (stage < 4) {
+      acc[e] = acc[e] + wb * kv;
+      s[n] = un[e] + wa * kv;
+    } else {
+      s[n] = acc[e] + wb * kv;
+    }
+  }
+  algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
+                 a.force_reg);
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) out[n * N + p] = s[n];
 }
 
 // Ghost vertices of a fused step with ghosts held at u_n: what the unfused
@@ -1058,7 +1094,6 @@ struct z4c_ctx {
   double* cv = nullptr;     // curvature part of the RHS algebra
   PlaneTile* d_ptiles = nullptr;
   int nptiles = 0;
-  int stencil_lds = 0;      // doubles of LDS per stencil workgroup
   long long total_int = 0;  // interior points
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
@@ -1191,8 +1226,8 @@ int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
   StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, u, c->dv, p->diss};
   {
     PassTimer t(c, Z4C_PASS_STENCIL, s);
-    hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
-                       sizeof(double) * c->stencil_lds, s, sa);
+    hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles * NVAR)), dim3(ST_THREADS), 0, s,
+                       sa);
   }
   HIPCALL(hipGetLastError());
   RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
@@ -1312,9 +1347,8 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   std::vector<DevBlock> hb(nblocks);
   std::vector<Box> boxes[NBOX];
   std::vector<Tile> tiles[NBOX];
-  long long off = 0, ioff = 0;
+  long long off = 0, ioff = 0, doff = 0;
   std::vector<PlaneTile> ptiles;
-  int max_lds = 0;
   const double diss_scaling = std::pow(2.0, -2 * NG) * (NG % 2 == 0 ? -1 : 1);
   for (int b = 0; b < nblocks; ++b) {
     const z4c_block& in = blocks[b];
@@ -1332,6 +1366,7 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     DevBlock& B = hb[b];
     B.off = off;
     B.ioff = ioff;
+    B.doff = doff;
     B.nn1 = in.nn1; B.nn2 = in.nn2; B.nn3 = in.nn3;
     B.il = in.il; B.iu = in.iu; B.jl = in.jl; B.ju = in.ju; B.kl = in.kl; B.ku = in.ku;
     const long long npts = (long long)in.nn1 * in.nn2 * in.nn3;
@@ -1342,6 +1377,9 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     B.nj = in.ju - in.jl + 1;
     B.nk = in.ku - in.kl + 1;
     B.nint = B.ni * B.nj * B.nk;
+    B.nis = B.nint + DV_PAD;
+    if ((long long)B.nis * NDV > std::numeric_limits<int>::max())
+      return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
     // x-y plane tiles of the stencil pass: the fewest tiles that fit the
     // 256-thread mapping, preferring the widest x extent (coalescing)
@@ -1367,7 +1405,6 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
         for (int i0 = 0; i0 < B.ni; i0 += best_tx) {
           PlaneTile pt{b, i0, j0, std::min(best_tx, B.ni - i0), std::min(best_ty, B.nj - j0)};
           ptiles.push_back(pt);
-          max_lds = std::max(max_lds, stencil_lds_doubles(pt.ni, pt.nj));
         }
     }
     for (int d = 0; d < 3; ++d) {
@@ -1395,9 +1432,8 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     }
     off += npts;
     ioff += B.nint;
+    doff += (long long)NDV * B.nis;
   }
-  if (max_lds * (long long)sizeof(double) > 160 * 1024)
-    return fail(Z4C_EINVAL, "set_blocks: block x extent too large for the stencil pass LDS tile");
   c->release();
   c->nblocks = 0;
   if (hipMalloc(&c->d_blocks, sizeof(DevBlock) * nblocks) != hipSuccess)
@@ -1417,7 +1453,7 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     return fail(Z4C_ENOMEM, "set_blocks: derivative storage allocation failed");
   // prepared derivatives start at zero (the reference zero-initialises them)
   HIPCALL(hipMemset(c->d1, 0, sizeof(double) * off * ND1));
-  if (hipMalloc(&c->dv, sizeof(double) * ioff * NDV) != hipSuccess ||
+  if (hipMalloc(&c->dv, sizeof(double) * doff) != hipSuccess ||
       hipMalloc(&c->cv, sizeof(double) * ioff * 12) != hipSuccess)
     return fail(Z4C_ENOMEM, "set_blocks: derivative-slot allocation failed");
   if (hipMalloc(&c->d_ptiles, sizeof(PlaneTile) * ptiles.size()) != hipSuccess)
@@ -1425,11 +1461,6 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   HIPCALL(hipMemcpy(c->d_ptiles, ptiles.data(), sizeof(PlaneTile) * ptiles.size(),
                     hipMemcpyHostToDevice));
   c->nptiles = (int)ptiles.size();
-  c->stencil_lds = max_lds;
-  if (max_lds * sizeof(double) > 64 * 1024)
-    HIPCALL(hipFuncSetAttribute((const void*)k_stencil,
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(max_lds * sizeof(double))));
   c->hblocks.assign(blocks, blocks + nblocks);
   c->nblocks = nblocks;
   c->total_pts = off;
@@ -1682,8 +1713,8 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
       StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, in, c->dv, p->diss};
       {
         PassTimer t(c, Z4C_PASS_STENCIL, s);
-        hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles), NVAR), dim3(ST_THREADS),
-                           sizeof(double) * c->stencil_lds, s, sa);
+        hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles * NVAR)), dim3(ST_THREADS), 0,
+                           s, sa);
       }
       HIPCALL(hipGetLastError());
       {

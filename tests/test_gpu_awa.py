@@ -46,15 +46,25 @@ def test_gauge_wave_steps_and_history(gauge):
     torch.cuda.synchronize()
     exp_u, exp_rows = _oracle_run(geom, params, nsteps, dt)
     got = z.download(u, 22)[0]
+    # per-variable norm-relative, with the wave amplitude as the floor of the
+    # scale (the 1-D wave leaves g~xy, A~xy, Gam~y, ... at roundoff level).
+    # Bound 1e-11 after three steps: AlgConstr's pow/cbrt may differ from
+    # glibc in the last bit, and the second-derivative operator, with weights
+    # of order 500/dx^2, lifts such ulp differences in g~ to ~1e-13 in the
+    # Ricci terms that drive A~, Gam~ and Theta.
     num = np.abs(got - exp_u).reshape(22, -1).max(axis=1)
-    den = np.abs(exp_u).reshape(22, -1).max(axis=1)
-    r = np.where(den > 0, num / np.where(den > 0, den, 1), num)
-    assert np.all(r <= 1e-12), r
+    den = np.maximum(np.abs(exp_u).reshape(22, -1).max(axis=1), 0.01)
+    r = num / den
+    assert np.all(r <= 1e-11), r
     rows = np.array(rows)
-    # H-norm2 and C-norm2 are sums of squares of cancellation-dominated
-    # constraint values: agree to a relative 1e-8 or an absolute 1e-24
-    ok = np.isclose(rows, exp_rows, rtol=1e-8, atol=1e-24)
-    assert ok.all(), (rows, exp_rows)
+    # Each column is sum(vol * X^2) of a constraint X that is the
+    # cancellation-dominated remainder of O(1) terms (H of an exact solution
+    # is truncation-level, ~1e-10 here).  A pointwise absolute difference
+    # eps in X moves the column by at most 2 eps sqrt(col V) + eps^2 V
+    # (Cauchy-Schwarz, V = domain volume = 1): check that bound, eps = 1e-12.
+    eps, V = 1e-12, 1.0
+    bound = 2 * eps * np.sqrt(np.maximum(rows, exp_rows) * V) + eps * eps * V
+    assert np.all(np.abs(rows - exp_rows) <= bound), (rows, exp_rows)
 
 
 def test_history_of_known_fields():

@@ -288,7 +288,8 @@ def test_rk4_step_frozen_ghosts():
 
 def _uneven_batch():
     """three uneven blocks (sizes, spacings, Sommerfeld faces differ)."""
-    gs = [MeshBlockGeom.cube(8, 0.125, (-1.0, -1.0, -1.0), faces=1 | 8 | 32),
+    # no vertex at the origin: the radiative condition divides by r
+    gs = [MeshBlockGeom.cube(8, 0.125, (-1.1, -0.95, -1.05), faces=1 | 8 | 32),
           MeshBlockGeom.cube(12, 1.0 / 12, (0.0, 0.0, 0.0)),
           MeshBlockGeom.cube(6, 0.25, (0.5, -1.5, 0.25), faces=63)]
     return gs
