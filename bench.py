@@ -30,6 +30,12 @@ import numpy as np  # noqa: E402
 
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
+# kernel of each timed pass, and the committed counter summary holding the
+# measured HBM bytes per launch of each kernel (FETCH_SIZE / WRITE_SIZE
+# passes, calibrated against a copy kernel of known size)
+PASS_KERNEL = {"stencil": "k_stencil", "curvature": "k_rhs_curv", "assembly": "k_rk4_stage",
+               "ghosts": "k_rk4_ghosts", "prepare": "k_prep"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 
 # Algorithmic HBM bytes of each pass per launch, from what the pass must read
 # and write once (doubles): per interior vertex (I) and per padded vertex (P).
@@ -45,6 +51,24 @@ PASS_BYTES = {
     "ghosts": lambda I, P: 8.0 * (22 + 14 + 44) * (P - I),
     "prepare": lambda I, P: 8.0 * (11 + 33) * P,
 }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (the block between 'traffic-bytes-per-launch' markers: kernel, read
+    bytes, write bytes), or None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    if "traffic-bytes-per-launch" not in text:
+        return None
+    body = text.split("traffic-bytes-per-launch", 1)[1].split("-->", 1)[0]
+    for line in body.splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == kernel:
+            return float(f[1]) + float(f[2])
+    return None
 
 
 def parse():
@@ -193,8 +217,10 @@ def main():
     dom = max((k for k in pinfo if "gbs" in pinfo[k]), key=lambda k: pinfo[k]["avg_ms"] * pinfo[k]["launches"])
     d = pinfo[dom]
     out["roofline"] = {"bound": "hbm", "achieved": d["gbs"], "peak": PEAK_HBM / 1e9,
-                       "unit": "GB/s", "frac": d["frac_hbm"], "traffic": None,
-                       "kernel": dom, "avg_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes"]}
+                       "unit": "GB/s", "frac": d["frac_hbm"],
+                       "traffic": measured_traffic(PASS_KERNEL[dom]),
+                       "kernel": dom, "avg_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes"],
+                       "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)}
     out["passes"] = pinfo
 
     if args.calib_copy:

@@ -108,13 +108,16 @@ struct PlaneTile {
 // A tile = up to TILE consecutive points of one box of one block.
 struct Tile {
   int block;
+  int box;  // index into the launch's box table
   int start;
 };
 
 constexpr int TILE = 256;
 
 // Box kinds used by the launch tables.
-enum BoxKind : int { BOX_INTERIOR = 0, BOX_PREP = 1, BOX_WHOLE = 2, NBOX = 3 };
+// BOX_GHOST holds six boxes per block that together cover exactly the
+// ghost vertices (the padded block minus the interior).
+enum BoxKind : int { BOX_INTERIOR = 0, BOX_PREP = 1, BOX_WHOLE = 2, BOX_GHOST = 3, NBOX = 4 };
 
 struct Box {
   int lo[3];   // i, j, k

@@ -59,7 +59,7 @@ __device__ __forceinline__ bool tile_point(const Launch& L, int& blk, int& i,
   if (t >= L.ntiles) return false;
   const Tile T = L.tiles[t];
   blk = T.block;
-  const Box bx = L.boxes[blk];
+  const Box bx = L.boxes[T.box];
   const int q = T.start + (int)threadIdx.x;
   const int n01 = bx.n[0] * bx.n[1];
   if (q >= n01 * bx.n[2]) return false;
@@ -461,7 +461,6 @@ __global__ void __launch_bounds__(TILE) k_rk4_ghosts(GhostArgs a) {
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
-  if (i >= B.il && i <= B.iu && j >= B.jl && j <= B.ju && k >= B.kl && k <= B.ku) return;
   const int N = B.npts;
   const int p = flat(B, i, j, k);
   const long long base = B.off * NVAR;
@@ -1086,9 +1085,9 @@ struct z4c_ctx {
   long long total_pts = 0;
   std::vector<z4c_block> hblocks;
   DevBlock* d_blocks = nullptr;
-  Tile* d_tiles[NBOX] = {nullptr, nullptr, nullptr};
-  Box* d_boxes[NBOX] = {nullptr, nullptr, nullptr};
-  int ntiles[NBOX] = {0, 0, 0};
+  Tile* d_tiles[NBOX] = {};
+  Box* d_boxes[NBOX] = {};
+  int ntiles[NBOX] = {};
   double* d1 = nullptr;     // prepared first derivatives (PrepareZ4cDerivatives)
   double* dv = nullptr;     // derivative slots of the RHS stencil pass
   double* cv = nullptr;     // curvature part of the RHS algebra
@@ -1424,11 +1423,29 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
       bp.lo[d] = lo[d] - (NG - 1); bp.n[d] = hi[d] - lo[d] + 1 + 2 * (NG - 1);
       bw.lo[d] = 0; bw.n[d] = nn[d];
     }
-    const Box bx[NBOX] = {bi, bp, bw};
-    for (int kind = 0; kind < NBOX; ++kind) {
+    const Box bx[3] = {bi, bp, bw};
+    for (int kind = 0; kind < 3; ++kind) {
+      const int id = (int)boxes[kind].size();
       boxes[kind].push_back(bx[kind]);
       const int np = bx[kind].n[0] * bx[kind].n[1] * bx[kind].n[2];
-      for (int s = 0; s < np; s += TILE) tiles[kind].push_back(Tile{b, s});
+      for (int s = 0; s < np; s += TILE) tiles[kind].push_back(Tile{b, id, s});
+    }
+    {
+      // ghost slabs: below/above in z, then in y, then in x
+      const int nj = bi.n[1], nk = bi.n[2];
+      const Box gb[6] = {
+          {{0, 0, 0}, {nn[0], nn[1], lo[2]}},
+          {{0, 0, hi[2] + 1}, {nn[0], nn[1], nn[2] - hi[2] - 1}},
+          {{0, 0, lo[2]}, {nn[0], lo[1], nk}},
+          {{0, hi[1] + 1, lo[2]}, {nn[0], nn[1] - hi[1] - 1, nk}},
+          {{0, lo[1], lo[2]}, {lo[0], nj, nk}},
+          {{hi[0] + 1, lo[1], lo[2]}, {nn[0] - hi[0] - 1, nj, nk}}};
+      for (int q = 0; q < 6; ++q) {
+        const int id = (int)boxes[BOX_GHOST].size();
+        boxes[BOX_GHOST].push_back(gb[q]);
+        const int np = gb[q].n[0] * gb[q].n[1] * gb[q].n[2];
+        for (int s = 0; s < np; s += TILE) tiles[BOX_GHOST].push_back(Tile{b, id, s});
+      }
     }
     off += npts;
     ioff += B.nint;
@@ -1441,11 +1458,11 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   HIPCALL(hipMemcpy(c->d_blocks, hb.data(), sizeof(DevBlock) * nblocks, hipMemcpyHostToDevice));
   for (int kind = 0; kind < NBOX; ++kind) {
     if (hipMalloc(&c->d_tiles[kind], sizeof(Tile) * tiles[kind].size()) != hipSuccess ||
-        hipMalloc(&c->d_boxes[kind], sizeof(Box) * nblocks) != hipSuccess)
+        hipMalloc(&c->d_boxes[kind], sizeof(Box) * boxes[kind].size()) != hipSuccess)
       return fail(Z4C_ENOMEM, "set_blocks: allocation failed");
     HIPCALL(hipMemcpy(c->d_tiles[kind], tiles[kind].data(), sizeof(Tile) * tiles[kind].size(),
                       hipMemcpyHostToDevice));
-    HIPCALL(hipMemcpy(c->d_boxes[kind], boxes[kind].data(), sizeof(Box) * nblocks,
+    HIPCALL(hipMemcpy(c->d_boxes[kind], boxes[kind].data(), sizeof(Box) * boxes[kind].size(),
                       hipMemcpyHostToDevice));
     c->ntiles[kind] = (int)tiles[kind].size();
   }
@@ -1729,7 +1746,7 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
     }
     HIPCALL(hipGetLastError());
     if (st == 1) {
-      GhostArgs ga{c->d_blocks, launch_of(c, BOX_WHOLE), u, W0, W1, p->eps_floor,
+      GhostArgs ga{c->d_blocks, launch_of(c, BOX_GHOST), u, W0, W1, p->eps_floor,
                    p->tf_force_regularization, p->force_regularization};
       {
         PassTimer t(c, Z4C_PASS_GHOSTS, s);
