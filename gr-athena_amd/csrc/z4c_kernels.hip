@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1074,6 +1075,40 @@ __global__ void __launch_bounds__(TILE) k_periodic(PerArgs a) {
   }
 }
 
+
+// Same-level VC exchange between blocks (src/comm/comm_channel.cpp:1277-1330
+// additive unpack, src/comm/node_multiplicity.cpp:349-420 division): each
+// destination vertex takes the sum of its owners' values (interior vertices
+// of any block at the same lattice point, in block and k, j, i order) from a
+// snapshot, times 1/multiplicity.  One thread per destination.
+struct ExArgs {
+  const DevBlock* blocks;
+  const int* dst;      // 2 ints per destination: block, point
+  const int* src_off;  // CSR offsets into src
+  const int* src;      // 2 ints per source: block, point
+  int ndst;
+  int nvar;
+  double* u;
+  const double* snap;
+};
+
+__global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= a.ndst) return;
+  const DevBlock B = a.blocks[a.dst[2 * d]];
+  const int p = a.dst[2 * d + 1];
+  const int s0 = a.src_off[d], s1 = a.src_off[d + 1];
+  const int cnt = s1 - s0;
+  const double inv = 1.0 / (double)cnt;
+  for (int n = 0; n < a.nvar; ++n) {
+    double acc = 0.0;
+    for (int e = s0; e < s1; ++e) {
+      const DevBlock& S = a.blocks[a.src[2 * e]];
+      acc += a.snap[S.off * a.nvar + (long long)n * S.npts + a.src[2 * e + 1]];
+    }
+    a.u[B.off * a.nvar + (long long)n * B.npts + p] = cnt > 1 ? acc * inv : acc;
+  }
+}
 }  // namespace
 
 // ===========================================================================
@@ -1097,6 +1132,10 @@ struct z4c_ctx {
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
   double* hst = nullptr;   // history partial sums (lazy)
+  int* ex_dst = nullptr;   // exchange map (z4c_set_exchange)
+  int* ex_off = nullptr;
+  int* ex_src = nullptr;
+  int ex_ndst = 0;
   std::vector<double> hst_host;
   int n_cu = 256;
   // per-pass kernel timing (events recorded on the launch stream)
@@ -1131,6 +1170,11 @@ struct z4c_ctx {
     hipFree(snap);
     hipFree(hst);
     hst = nullptr;
+    hipFree(ex_dst);
+    hipFree(ex_off);
+    hipFree(ex_src);
+    ex_dst = ex_off = ex_src = nullptr;
+    ex_ndst = 0;
     hst_host.clear();
     d_blocks = nullptr;
     d1 = nullptr;
@@ -1301,6 +1345,30 @@ int launch_periodic(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   HIPCALL(hipGetLastError());
   PerArgs a{c->d_blocks, launch_of(c, BOX_WHOLE), nvar, u, c->snap};
   hipLaunchKernelGGL(k_periodic, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int ensure_snapshot(z4c_ctx* c, long long n) {
+  if (c->snap_elems >= n) return Z4C_OK;
+  hipFree(c->snap);
+  c->snap = nullptr;
+  c->snap_elems = 0;
+  if (hipMalloc(&c->snap, sizeof(double) * n) != hipSuccess)
+    return fail(Z4C_ENOMEM, "exchange: snapshot allocation failed");
+  c->snap_elems = n;
+  return Z4C_OK;
+}
+
+int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
+  if (!c->ex_ndst) return Z4C_OK;
+  const long long n = c->total_pts * nvar;
+  int rc = ensure_snapshot(c, n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy, dim3(elementwise_grid(n)), dim3(256), 0, s, c->snap, (const double*)u, n);
+  HIPCALL(hipGetLastError());
+  ExArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ex_src, c->ex_ndst, nvar, u, c->snap};
+  hipLaunchKernelGGL(k_exchange, dim3((c->ex_ndst + 255) / 256), dim3(256), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
@@ -1665,6 +1733,114 @@ int z4c_history(z4c_ctx* c, const z4c_params* p, const double* u, const double* 
   return Z4C_OK;
 }
 
+int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* periodic) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!lo || !hi || !periodic) return fail(Z4C_EINVAL, "set_exchange: null argument");
+  const double* h = c->hblocks[0].dx;
+  long long Nd[3];
+  for (int d = 0; d < 3; ++d) {
+    if (!(hi[d] > lo[d])) return fail(Z4C_EINVAL, "set_exchange: empty domain");
+    Nd[d] = std::llround((hi[d] - lo[d]) / h[d]);
+    if (Nd[d] <= 0 || Nd[d] >= (1LL << 20) ||
+        std::fabs((hi[d] - lo[d]) / h[d] - (double)Nd[d]) > 1e-6)
+      return fail(Z4C_EINVAL, "set_exchange: domain is not a whole number of cells");
+  }
+  // lattice origin of every block's interior; one spacing for all blocks
+  const int nb = c->nblocks;
+  std::vector<std::array<long long, 3>> Lo(nb);
+  for (int b = 0; b < nb; ++b) {
+    const z4c_block& B = c->hblocks[b];
+    const int il[3] = {B.il, B.jl, B.kl};
+    for (int d = 0; d < 3; ++d) {
+      if (B.dx[d] != h[d])
+        return fail(Z4C_EINVAL, "set_exchange: blocks of different spacing need prolongation "
+                                "and restriction, which this build does not provide");
+      const double L = (B.x0[d] + il[d] * h[d] - lo[d]) / h[d];
+      Lo[b][d] = std::llround(L);
+      if (std::fabs(L - (double)Lo[b][d]) > 1e-6)
+        return fail(Z4C_EINVAL, "set_exchange: block not aligned with the lattice");
+    }
+  }
+  auto wrap = [&](long long w, int d) {
+    return periodic[d] ? ((w % Nd[d]) + Nd[d]) % Nd[d] : w;
+  };
+  auto key = [&](long long I, long long J, long long K) -> long long {
+    const long long off = 1LL << 20;  // non-periodic coordinates may be negative
+    return ((I + off) << 42) | ((J + off) << 21) | (K + off);
+  };
+  // owners: (key, global point id, block, point), sorted by key then id
+  struct Own {
+    long long key, gid;
+    int blk, p;
+  };
+  std::vector<Own> own;
+  for (int b = 0; b < nb; ++b) {
+    const z4c_block& B = c->hblocks[b];
+    for (int k = B.kl; k <= B.ku; ++k)
+      for (int j = B.jl; j <= B.ju; ++j)
+        for (int i = B.il; i <= B.iu; ++i) {
+          const long long I = wrap(Lo[b][0] + (i - B.il), 0), J = wrap(Lo[b][1] + (j - B.jl), 1),
+                          K = wrap(Lo[b][2] + (k - B.kl), 2);
+          const int p = i + B.nn1 * (j + B.nn2 * k);
+          own.push_back({key(I, J, K), B.offset + p, b, p});
+        }
+  }
+  std::sort(own.begin(), own.end(), [](const Own& x, const Own& y) {
+    return x.key != y.key ? x.key < y.key : x.gid < y.gid;
+  });
+  std::vector<int> dst, off{0}, src;
+  for (int b = 0; b < nb; ++b) {
+    const z4c_block& B = c->hblocks[b];
+    for (int k = 0; k < B.nn3; ++k)
+      for (int j = 0; j < B.nn2; ++j)
+        for (int i = 0; i < B.nn1; ++i) {
+          const long long I = wrap(Lo[b][0] + (i - B.il), 0), J = wrap(Lo[b][1] + (j - B.jl), 1),
+                          K = wrap(Lo[b][2] + (k - B.kl), 2);
+          const long long kk = key(I, J, K);
+          auto lo_it = std::lower_bound(own.begin(), own.end(), kk,
+                                        [](const Own& x, long long v) { return x.key < v; });
+          auto hi_it = lo_it;
+          while (hi_it != own.end() && hi_it->key == kk) ++hi_it;
+          const long cnt = hi_it - lo_it;
+          const bool interior = i >= B.il && i <= B.iu && j >= B.jl && j <= B.ju &&
+                                k >= B.kl && k <= B.ku;
+          if (cnt == 0 || (interior && cnt == 1)) continue;
+          dst.push_back(b);
+          dst.push_back(i + B.nn1 * (j + B.nn2 * k));
+          for (auto it = lo_it; it != hi_it; ++it) {
+            src.push_back(it->blk);
+            src.push_back(it->p);
+          }
+          if (src.size() / 2 > (size_t)std::numeric_limits<int>::max())
+            return fail(Z4C_EINVAL, "set_exchange: map too large");
+          off.push_back((int)(src.size() / 2));
+        }
+  }
+  hipFree(c->ex_dst);
+  hipFree(c->ex_off);
+  hipFree(c->ex_src);
+  c->ex_dst = c->ex_off = c->ex_src = nullptr;
+  c->ex_ndst = 0;
+  if (dst.empty()) return Z4C_OK;
+  if (hipMalloc(&c->ex_dst, sizeof(int) * dst.size()) != hipSuccess ||
+      hipMalloc(&c->ex_off, sizeof(int) * off.size()) != hipSuccess ||
+      hipMalloc(&c->ex_src, sizeof(int) * src.size()) != hipSuccess)
+    return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
+  HIPCALL(hipMemcpy(c->ex_dst, dst.data(), sizeof(int) * dst.size(), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(c->ex_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(c->ex_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+  c->ex_ndst = (int)(dst.size() / 2);
+  return Z4C_OK;
+}
+
+int z4c_exchange(z4c_ctx* c, int nvar, double* u, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u || nvar <= 0) return fail(Z4C_EINVAL, "exchange: bad argument");
+  return launch_exchange(c, nvar, u, (hipStream_t)stream);
+}
+
 int z4c_set_timing(z4c_ctx* c, int enable) {
   if (!c) return fail(Z4C_EINVAL, "set_timing: null context");
   c->timing = enable != 0;
@@ -1780,8 +1956,11 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   else
     rc = launch_bt(c, NVAR, u, un, 4, kk, Bw, dt, s);
   if (rc) return rc;
-  if (periodic) {
+  if (periodic == 1) {
     rc = launch_periodic(c, NVAR, u, s);
+    if (rc) return rc;
+  } else if (periodic == 2) {
+    rc = launch_exchange(c, NVAR, u, s);
     if (rc) return rc;
   }
   rc = launch_alg(c, p, u, nullptr, 0, s);

@@ -208,19 +208,34 @@ class Z4c:
     def periodic_fill(self, u, nvar=Z4C_NVAR, stream=None):
         check(self.L.z4c_periodic_fill(self._ctx, nvar, _ptr(u), _stream(stream)), "periodic_fill")
 
-    def rk4_stage(self, stage, dt, u, un, k, mat=None, periodic=False, stream=None):
+    def set_exchange(self, lo, hi, periodic=(True, True, True)):
+        """Build the same-level VC exchange of the block list on [lo, hi]."""
+        lo_ = (C.c_double * 3)(*np.broadcast_to(np.asarray(lo, float), (3,)))
+        hi_ = (C.c_double * 3)(*np.broadcast_to(np.asarray(hi, float), (3,)))
+        per = (C.c_int * 3)(*[int(bool(v)) for v in np.broadcast_to(periodic, (3,))])
+        check(self.L.z4c_set_exchange(self._ctx, lo_, hi_, per), "set_exchange")
+
+    def exchange(self, u, nvar=Z4C_NVAR, stream=None):
+        """Same-level VC ghost exchange (SEND/RECV/SETB of GR_Z4c)."""
+        check(self.L.z4c_exchange(self._ctx, nvar, _ptr(u), _stream(stream)), "exchange")
+
+    def rk4_stage(self, stage, dt, u, un, k, mat=None, periodic=False, exchange=False,
+                  stream=None):
+        """One GR_Z4c stage; ghosts after SumBT: the single-block periodic
+        fill (``periodic``), the block exchange (``exchange``) or neither."""
         kp = (C.c_void_p * 4)(*[_ptr(x) for x in k])
+        mode = 2 if exchange else (1 if periodic else 0)
         check(self.L.z4c_rk4_stage(self._ctx, C.byref(self.params), int(stage), float(dt), _ptr(u),
-                                   _ptr(un), kp, _ptr(mat), int(periodic), _stream(stream)),
+                                   _ptr(un), kp, _ptr(mat), mode, _stream(stream)),
               "rk4_stage")
 
-    def rk4_step(self, dt, u, un, k, mat=None, periodic=False, stream=None):
+    def rk4_step(self, dt, u, un, k, mat=None, periodic=False, exchange=False, stream=None):
         """One bt_rk4 step in the GR_Z4c stage order; derivatives for ``u``
         must be prepared (PrepareZ4cDerivatives) before the first step; each
         stage re-prepares them for the next."""
         un.copy_(u)  # Butcher::PutScratchBT_u at stage 1
         for s in range(1, 5):
-            self.rk4_stage(s, dt, u, un, k, mat, periodic, stream)
+            self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, stream)
 
     def rk4_step_fused(self, dt, u, work, mat=None, stream=None):
         """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr

@@ -167,13 +167,29 @@ int z4c_new_dt(z4c_ctx* ctx, double cfl, double* dt_out);
    Applies to every registered block independently. */
 int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
+/* Same-level vertex-centred ghost exchange between the registered blocks,
+   as GR_Z4c's SEND_Z4C / RECV_Z4C / SETB_Z4C do for neighbours on one level
+   (additive unpack src/comm/comm_channel.cpp:1277-1330, ranges
+   src/comm/index_utilities.hpp:116-139, division by the node multiplicity
+   src/comm/node_multiplicity.cpp:349-420).  z4c_set_exchange builds the map
+   for the domain [lo, hi] (periodic[d] != 0: axis d wraps); all blocks must
+   share one spacing and sit on its lattice (Z4C_EINVAL otherwise: levels
+   need prolongation/restriction, not provided).  z4c_exchange then sets
+   every vertex that some block owns as an interior vertex (ghosts, and
+   shared boundary vertices) to the mean of its owners' values, summed in
+   block order and k, j, i order, times 1/multiplicity. */
+int z4c_set_exchange(z4c_ctx* ctx, const double* lo, const double* hi, const int* periodic);
+int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
+
 /* One bt_rk4 stage of the GR_Z4c task list
    (src/task_list/gr/task_list_gr_z4c.cpp:45-89, stage s = 1..4) on all
    blocks: k[s-1] = RHS(u_s) (+ Sommerfeld), u_{s+1} = u_n + sum a dt k
    (SumBT_ak / SumBT_bk), optional periodic ghost fill, AlgConstr,
    PrepareZ4cDerivatives.  `un` holds u at the start of the step; `u` is
    overwritten with the next stage state; k[4] are device pointers (host
-   array). */
+   array).  Ghost update after SumBT: periodic = 0 none (ghosts stay u_n),
+   1 single-block periodic fill (z4c_periodic_fill), 2 the exchange set by
+   z4c_set_exchange. */
 int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
                   double* u, const double* un, double* const* k,
                   const double* mat, int periodic, void* stream);

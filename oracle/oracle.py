@@ -137,6 +137,7 @@ def lib():
         L.oracle_new_dt.argtypes = [pg, d]
         L.oracle_periodic_fill.argtypes = [pg, C.c_int, pd]
         L.oracle_history.argtypes = [pp, pg, pd, pd, pd]
+        L.oracle_exchange.argtypes = [C.c_int, pg, P(pd), C.c_int, pd, pd, P(C.c_int)]
     return _lib
 
 
@@ -237,6 +238,19 @@ def history(o, g, u, con, out=None):
     acc = (C.c_double * 8)(*(out if out is not None else [0.0] * 8))
     lib().oracle_history(C.byref(o), C.byref(g), _p(u), _p(con), acc)
     return list(acc)
+
+
+def exchange(geoms, us, lo, hi, periodic):
+    """Same-level VC exchange of the blocks ``geoms`` with arrays ``us``
+    (modified in place)."""
+    n = len(geoms)
+    garr = (Geom * n)(*geoms)
+    uarr = (C.POINTER(C.c_double) * n)(*[_p(u) for u in us])
+    lo_ = (C.c_double * 3)(*lo)
+    hi_ = (C.c_double * 3)(*hi)
+    per = (C.c_int * 3)(*[int(bool(v)) for v in periodic])
+    lib().oracle_exchange(n, garr, uarr, us[0].shape[0], lo_, hi_, per)
+    return us
 
 
 def periodic_fill(g, u):

@@ -1208,3 +1208,82 @@ void oracle_history(const oracle_params* o, const oracle_geom* g,
         out[7] += vol * fabs(con[OR_CON_C * N + p]);
       }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Same-level vertex-centred exchange between blocks (additive unpack of     */
+/* every owner's interior value, then multiplication by 1/multiplicity:     */
+/* src/comm/comm_channel.cpp:1277-1330, src/comm/node_multiplicity.cpp:     */
+/* 349-420, ranges src/comm/index_utilities.hpp:116-139).                   */
+/* Brute force: for every vertex of every block, the owners are the         */
+/* interior vertices of all blocks at the same lattice point (periodic axes */
+/* wrap); the vertex takes the mean of the owners' values, summed in block  */
+/* order and then k, j, i order.  Vertices without an owner are untouched.  */
+/* ------------------------------------------------------------------------ */
+static long lattice(double x, double lo, double h) { return lround((x - lo) / h); }
+
+static int owners_axis(long W, long Lo, long nx, long Nd, int per, long il, long* out) {
+  long d = W - Lo;
+  if (per) d = ((d % Nd) + Nd) % Nd;
+  int n = 0;
+  if (d >= 0 && d <= nx) out[n++] = il + d;
+  if (per && d == 0 && nx == Nd) out[n++] = il + Nd; /* block spans the period */
+  return n;
+}
+
+void oracle_exchange(int nb, const oracle_geom* g, double* const* u, int nvar,
+                     const double* lo, const double* hi, const int* periodic) {
+  const double h[3] = {g[0].dx[0], g[0].dx[1], g[0].dx[2]};
+  long Nd[3];
+  for (int d = 0; d < 3; ++d) Nd[d] = lround((hi[d] - lo[d]) / h[d]);
+  double** snap = (double**)malloc(sizeof(double*) * nb);
+  for (int b = 0; b < nb; ++b) {
+    const long N = (long)g[b].nn1 * g[b].nn2 * g[b].nn3;
+    snap[b] = (double*)malloc(sizeof(double) * N * nvar);
+    memcpy(snap[b], u[b], sizeof(double) * N * nvar);
+  }
+  for (int b = 0; b < nb; ++b) {
+    const oracle_geom* G = &g[b];
+    const long N = (long)G->nn1 * G->nn2 * G->nn3;
+    const int nn[3] = {G->nn1, G->nn2, G->nn3};
+    for (int k = 0; k < nn[2]; ++k)
+      for (int j = 0; j < nn[1]; ++j)
+        for (int i = 0; i < nn[0]; ++i) {
+          const int ijk[3] = {i, j, k};
+          long W[3];
+          for (int d = 0; d < 3; ++d) {
+            W[d] = lattice(G->x0[d] + ijk[d] * h[d], lo[d], h[d]);
+            if (periodic[d]) W[d] = ((W[d] % Nd[d]) + Nd[d]) % Nd[d];
+          }
+          /* owners: all blocks, in block order, then k, j, i */
+          double acc[64];
+          for (int n = 0; n < nvar && n < 64; ++n) acc[n] = 0.0;
+          int cnt = 0;
+          for (int c = 0; c < nb; ++c) {
+            const oracle_geom* C = &g[c];
+            const long il[3] = {C->il, C->jl, C->kl};
+            const long nx[3] = {C->iu - C->il, C->ju - C->jl, C->ku - C->kl};
+            long o[3][2];
+            int no[3];
+            for (int d = 0; d < 3; ++d) {
+              const long Lo = lattice(C->x0[d] + il[d] * h[d], lo[d], h[d]);
+              no[d] = owners_axis(W[d], Lo, nx[d], Nd[d], periodic[d], il[d], o[d]);
+            }
+            const long NC = (long)C->nn1 * C->nn2 * C->nn3;
+            for (int a = 0; a < no[2]; ++a)
+              for (int bb = 0; bb < no[1]; ++bb)
+                for (int cc = 0; cc < no[0]; ++cc) {
+                  const long p = o[0][cc] + (long)C->nn1 * (o[1][bb] + (long)C->nn2 * o[2][a]);
+                  for (int n = 0; n < nvar && n < 64; ++n) acc[n] += snap[c][n * NC + p];
+                  ++cnt;
+                }
+          }
+          if (cnt == 0) continue;
+          const long p = i + (long)nn[0] * (j + (long)nn[1] * k);
+          const double inv = 1.0 / (double)cnt;
+          for (int n = 0; n < nvar && n < 64; ++n)
+            u[b][n * N + p] = (cnt > 1) ? acc[n] * inv : acc[n];
+        }
+  }
+  for (int b = 0; b < nb; ++b) free(snap[b]);
+  free(snap);
+}

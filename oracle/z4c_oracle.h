@@ -142,6 +142,13 @@ double oracle_new_dt(const oracle_geom* g, double cfl);
 void oracle_history(const oracle_params* o, const oracle_geom* g,
                     const double* u, const double* con, double* out);
 
+/* Same-level VC exchange over nb blocks of one spacing on the domain
+   [lo, hi] (periodic[d] != 0: axis d wraps): every vertex with at least
+   one owner (an interior vertex of any block at the same lattice point)
+   takes the owners' mean; u[b] is block b's nvar-field array. */
+void oracle_exchange(int nb, const oracle_geom* g, double* const* u, int nvar,
+                     const double* lo, const double* hi, const int* periodic);
+
 /* Single-block periodic vertex-centred ghost fill (all three axes). */
 void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u);
 

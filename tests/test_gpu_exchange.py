@@ -1,0 +1,87 @@
+"""Same-level VC ghost exchange between blocks on the device (z4c_exchange)
+against the oracle's brute-force exchange, and a bt_rk4 evolution of a
+periodic domain split into blocks against the same domain as one block
+(src/comm/comm_channel.cpp:1277-1330, src/comm/node_multiplicity.cpp)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from z4c_amd import Params, Z4cError  # noqa: E402
+from z4c_amd import awa  # noqa: E402
+from z4c_amd.initial_data import gauge_wave_adm  # noqa: E402
+from z4c_amd.z4c import MeshBlockGeom, Z4c  # noqa: E402
+
+
+def _grid(parts, n, h, lo):
+    sub = [n // p for p in parts]
+    return [MeshBlockGeom.cube(tuple(sub), h, tuple(lo + np.array([a, b, c]) * np.array(sub) * h))
+            for c in range(parts[2]) for b in range(parts[1]) for a in range(parts[0])]
+
+
+@pytest.mark.parametrize("parts,periodic", [((2, 2, 1), (1, 1, 1)), ((2, 1, 2), (1, 0, 1)),
+                                            ((1, 1, 1), (1, 1, 1))])
+def test_exchange_matches_oracle(parts, periodic):
+    h, n, lo = 0.125, 8, -0.5
+    gs = _grid(parts, n, h, lo)
+    rng = np.random.default_rng(31)
+    host = [rng.standard_normal((22,) + g.shape) for g in gs]
+    z = Z4c(gs, Params())
+    z.set_exchange([lo] * 3, [lo + n * h] * 3, periodic)
+    u = z.upload(host)
+    z.exchange(u)
+    torch.cuda.synchronize()
+    got = z.download(u, 22)
+    og = [orc.Geom.block(g.nx, g.dx, g.xmin) for g in gs]
+    exp = orc.exchange(og, [np.ascontiguousarray(x) for x in host], [lo] * 3,
+                       [lo + n * h] * 3, periodic)
+    for b in range(len(gs)):
+        assert np.array_equal(got[b], exp[b]), b
+
+
+def test_exchange_rejects_mixed_spacing():
+    gs = [MeshBlockGeom.cube(4, 0.25, 0.0), MeshBlockGeom.cube(4, 0.125, 1.0)]
+    z = Z4c(gs, Params())
+    with pytest.raises(Z4cError):
+        z.set_exchange([0.0] * 3, [2.0] * 3)
+
+
+def test_split_gauge_wave_evolution_equals_single_block():
+    """The periodic gauge wave on one 16^3 block (periodic self-fill) and on
+    2 x 1 x 2 blocks (exchange) evolve to the same state: two bt_rk4 steps
+    with the ghost update after every stage."""
+    n, nsteps = 16, 2
+    geom, params, z1, u1 = awa.gauge_wave_setup(n=n, gauge="puncture")
+    dt = z1.NewBlockTimeStep(0.25)
+    un1, k1 = z1.make_rk_registers()
+    for _ in range(nsteps):
+        z1.rk4_step(dt, u1, un1, k1, periodic=True)
+    whole = z1.download(u1, 22)[0]
+
+    gs = _grid((2, 1, 2), n, 1.0 / n, -0.5)
+    z = Z4c(gs, params)
+    z.set_exchange([-0.5] * 3, [0.5] * 3, (1, 1, 1))
+    adm = z.upload([gauge_wave_adm(g, A=0.01, d=1.0) for g in gs])
+    u = z.zeros()
+    z.ADMToZ4c(adm, u)
+    z.exchange(u)
+    nn = gs[0].nn
+    z.AlgConstr(u, rng=(0, nn[0] - 1, 0, nn[1] - 1, 0, nn[2] - 1), skip_physical=True)
+    z.PrepareZ4cDerivatives(u)
+    un, k = z.make_rk_registers()
+    for _ in range(nsteps):
+        z.rk4_step(dt, u, un, k, exchange=True)
+    torch.cuda.synchronize()
+    parts = z.download(u, 22)
+    sub = n // 2
+    for b, g in enumerate(gs):
+        a, c = b % 2, b // 2
+        i0, k0 = a * sub, c * sub
+        ref = whole[:, k0:k0 + g.shape[0], :, i0:i0 + g.shape[2]]
+        num = np.abs(parts[b] - ref).reshape(22, -1).max(axis=1)
+        den = np.maximum(np.abs(ref).reshape(22, -1).max(axis=1), 0.01)
+        assert np.all(num / den <= 1e-11), (b, num / den)

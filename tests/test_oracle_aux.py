@@ -267,3 +267,82 @@ def test_history_sums_known_fields():
     c = -1.5 + 0.25 + 0.5 * np.arange(6)
     inside = sum(1 for a in c for b in c for e in c if a * a + b * b + e * e <= 1.0)
     np.testing.assert_allclose(out[0], 4 * inside * 0.125, rtol=1e-14)
+
+
+def _split(g_big, nx_big, parts, h, lo):
+    """Geometries of the parts of a cube split per axis (VC: neighbours share
+    their boundary vertices)."""
+    sub = [nx_big // p for p in parts]
+    geoms, starts = [], []
+    for c in range(parts[2]):
+        for b in range(parts[1]):
+            for a in range(parts[0]):
+                s = (a * sub[0], b * sub[1], c * sub[2])
+                geoms.append(orc.Geom.block(sub, h, [lo + s[d] * h for d in range(3)]))
+                starts.append(s)
+    return geoms, starts
+
+
+def test_exchange_single_periodic_block_equals_periodic_fill():
+    g = orc.Geom.block(6, 0.1, 0.0)
+    rng = np.random.default_rng(21)
+    u = rng.standard_normal((2,) + g.shape)
+    a = orc.periodic_fill(g, u.copy())
+    (b,) = orc.exchange([g], [u.copy()], [0.0] * 3, [0.6] * 3, [1, 1, 1])
+    np.testing.assert_allclose(b, a, rtol=1e-15, atol=1e-15)
+
+
+def test_exchange_of_split_block_equals_whole_block():
+    """A periodic 8^3-cell block split into 2 x 2 x 1 blocks: after the
+    exchange every vertex of every part (ghosts included) equals the whole
+    block's periodic fill at the same place."""
+    h, lo, n = 0.125, -0.5, 8
+    big = orc.Geom.block(n, h, lo)
+    rng = np.random.default_rng(22)
+    u = rng.standard_normal((2,) + big.shape)
+    ref = orc.periodic_fill(big, u.copy())
+    geoms, starts = _split(big, n, (2, 2, 1), h, lo)
+    parts = []
+    for g, s in zip(geoms, starts):
+        v = rng.standard_normal((2,) + g.shape)  # ghosts: garbage to be replaced
+        sl = (slice(None),) + tuple(slice(4 + s[2 - d], 4 + s[2 - d] + (g.ku - g.kl, g.ju - g.jl,
+                                                                              g.iu - g.il)[d] + 1)
+                                    for d in range(3))
+        v[(slice(None),) + g.interior()] = ref[sl]
+        parts.append(v)
+    orc.exchange(geoms, parts, [lo] * 3, [lo + n * h] * 3, [1, 1, 1])
+    for g, s, v in zip(geoms, starts, parts):
+        sl = (slice(None),) + tuple(slice(s[2 - d], s[2 - d] + g.shape[d]) for d in range(3))
+        # the part's padded box, cut from the whole block's periodic extension
+        ext = np.pad(ref[(slice(None),) + big.interior()][:, :-1, :-1, :-1],
+                     ((0, 0),) + ((4, 5),) * 3, mode="wrap")
+        np.testing.assert_allclose(v, ext[sl], rtol=1e-14, atol=1e-15)
+
+
+def test_exchange_brute_force_nonperiodic():
+    """Two blocks side by side in x on a non-periodic domain, distinct data:
+    every vertex is the mean over the interior vertices at the same place,
+    found by coordinates; vertices outside the domain keep their values."""
+    h = 0.25
+    geoms = [orc.Geom.block((4, 4, 4), h, (0.0, 0.0, 0.0)),
+             orc.Geom.block((4, 4, 4), h, (1.0, 0.0, 0.0))]
+    rng = np.random.default_rng(23)
+    us = [rng.standard_normal((1,) + g.shape) for g in geoms]
+    before = [u.copy() for u in us]
+    orc.exchange(geoms, us, [0.0] * 3, [2.0, 1.0, 1.0], [0, 0, 0])
+    owners = {}
+    for b, g in enumerate(geoms):
+        X, Y, Z = g.mesh()
+        for k in range(g.kl, g.ku + 1):
+            for j in range(g.jl, g.ju + 1):
+                for i in range(g.il, g.iu + 1):
+                    key = tuple(np.round(np.array([X[k, j, i], Y[k, j, i], Z[k, j, i]]) / h).astype(int))
+                    owners.setdefault(key, []).append(before[b][0, k, j, i])
+    for b, g in enumerate(geoms):
+        X, Y, Z = g.mesh()
+        for k in range(g.nn3):
+            for j in range(g.nn2):
+                for i in range(g.nn1):
+                    key = tuple(np.round(np.array([X[k, j, i], Y[k, j, i], Z[k, j, i]]) / h).astype(int))
+                    exp = np.mean(owners[key]) if key in owners else before[b][0, k, j, i]
+                    assert abs(us[b][0, k, j, i] - exp) <= 1e-15 * max(1.0, abs(exp)), (b, i, j, k)
