@@ -8,8 +8,10 @@ Brill-Lindquist data (m1 = m2 = 0.5, separation 10, pre-collapsed lapse,
 zero shift, K = 0), moving-puncture gauge.  A step is one classical RK4 step
 (bt_rk4) on every block: per stage Z4cRHS (+KO) with SumBT and AlgConstr
 fused into its assembly kernel, one launch per kernel per stage for all
-blocks, and PrepareZ4cDerivatives of the new state at the end.  Ghost
-vertices stay at u_n through the step (independent blocks, no exchange).
+blocks.  Ghost vertices stay at u_n through the step (independent blocks,
+no exchange).  The reference prepares derivative arrays every stage because
+its RHS reads them; here the RHS forms its own, and the arrays are prepared
+only where the constraints are evaluated (not part of the step).
 
 value = owned point-updates per second: sum over blocks of nx^3 (the
 reference's zone count, src/main.hpp:729-731) per RK4 step, divided by the

@@ -1643,6 +1643,8 @@ int z4c_adm_constraints(z4c_ctx* c, const z4c_params* p, const double* u,
   if (rc) return rc;
   if (!p || !u || !adm || !con) return fail(Z4C_EINVAL, "adm_constraints: null argument");
   hipStream_t s = (hipStream_t)stream;
+  rc = launch_prep(c, p, u, nullptr, s);
+  if (rc) return rc;
   rc = fill(c, con, 0.0, c->total_pts * NCON, s);
   if (rc) return rc;
   ConArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, adm, mat, c->d1, con,
@@ -1931,7 +1933,7 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
       HIPCALL(hipGetLastError());
     }
   }
-  return launch_prep(c, p, u, nullptr, s);
+  return Z4C_OK;
 }
 
 int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double* u,

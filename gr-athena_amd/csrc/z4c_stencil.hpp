@@ -194,22 +194,36 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       }
       __syncthreads();
     }
-    double xv[9], yv[9];
+    // one direction at a time (x, y, then z from the ring), so that a
+    // single 9-point array is live; L and D accumulate in x, y, z order
+    double L = 0.0, D;
+    {
+      double xv[9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) {
-      xv[m] = cur[o + (m - 4)];
-      yv[m] = cur[o + (m - 4) * PX];
+      for (int m = 0; m < 9; ++m) xv[m] = cur[o + (m - 4)];
+      if (grad) bstore(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
+      if (full) bstore(DVB, st_dxx(xv, B.cidx2[0]), q8, (db + 3 + 0) * NS8);
+      L += st_lx(xv, B.lidx[0], bx);
+      D = st_diss(xv, B.cidxd[0], a.diss);
     }
-    if (grad) {
-      bstore(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
-      bstore(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
-      bstore(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
+    {
+      double yv[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) yv[m] = cur[o + (m - 4) * PX];
+      if (grad) bstore(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
+      if (full) bstore(DVB, st_dxx(yv, B.cidx2[1]), q8, (db + 3 + 3) * NS8);
+      L += st_lx(yv, B.lidx[1], by);
+      D += st_diss(yv, B.cidxd[1], a.diss);
     }
+    if (grad) bstore(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
+    if (full) bstore(DVB, st_dxx(zr, B.cidx2[2]), q8, (db + 3 + 5) * NS8);
+    // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
+    L += st_lx(zr, B.lidx[2], bz);
+    bstore(DVB, L, q8, sl * NS8);
+    D += st_diss(zr, B.cidxd[2], a.diss);
+    bstore(DVB, D, q8, sd * NS8);
     if (full) {
       double v[9];
-      bstore(DVB, st_dxx(xv, B.cidx2[0]), q8, (db + 3 + 0) * NS8);
-      bstore(DVB, st_dxx(yv, B.cidx2[1]), q8, (db + 3 + 3) * NS8);
-      bstore(DVB, st_dxx(zr, B.cidx2[2]), q8, (db + 3 + 5) * NS8);
       v[0] = v[8] = 0.0;
 #pragma unroll
       for (int m = 1; m < 8; ++m) v[m] = dyp[w + (m - 4)];
@@ -221,16 +235,6 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       for (int m = 1; m < 8; ++m) v[m] = dzp[w + (m - 4) * EX];
       bstore(DVB, st_dx(v, B.cidx1[1]), q8, (db + 3 + 4) * NS8);  // d_y(d_z u)
     }
-    // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
-    double L = 0.0;
-    L += st_lx(xv, B.lidx[0], bx);
-    L += st_lx(yv, B.lidx[1], by);
-    L += st_lx(zr, B.lidx[2], bz);
-    bstore(DVB, L, q8, sl * NS8);
-    double D = st_diss(xv, B.cidxd[0], a.diss);
-    D += st_diss(yv, B.cidxd[1], a.diss);
-    D += st_diss(zr, B.cidxd[2], a.diss);
-    bstore(DVB, D, q8, sd * NS8);
     q8 += qstep8;
 #pragma unroll
     for (int m = 0; m < 8; ++m) zr[m] = zr[m + 1];

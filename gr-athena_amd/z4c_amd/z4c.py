@@ -230,18 +230,18 @@ class Z4c:
               "rk4_stage")
 
     def rk4_step(self, dt, u, un, k, mat=None, periodic=False, exchange=False, stream=None):
-        """One bt_rk4 step in the GR_Z4c stage order; derivatives for ``u``
-        must be prepared (PrepareZ4cDerivatives) before the first step; each
-        stage re-prepares them for the next."""
+        """One bt_rk4 step in the GR_Z4c stage order (each stage: Z4cRHS,
+        SumBT, the ghost update, AlgConstr, PrepareZ4cDerivatives)."""
         un.copy_(u)  # Butcher::PutScratchBT_u at stage 1
         for s in range(1, 5):
             self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, stream)
 
     def rk4_step_fused(self, dt, u, work, mat=None, stream=None):
         """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr
-        in one kernel), ghosts held at u_n, ending with PrepareZ4cDerivatives
-        of the new state.  Equal to ``rk4_step`` with ``periodic=False``;
-        ``work`` comes from ``make_step_work``."""
+        in one kernel), ghosts held at u_n.  The state equals ``rk4_step``
+        with no ghost update; the derivative arrays are not prepared
+        (ADMConstraints prepares its own).  ``work`` comes from
+        ``make_step_work``."""
         check(self.L.z4c_rk4_step(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                   _ptr(work), _ptr(mat), _stream(stream)), "rk4_step")
 

@@ -100,14 +100,17 @@ int z4c_set_blocks(z4c_ctx* ctx, const z4c_block* blocks, int nblocks,
 
 /* Z4c::PrepareZ4cDerivatives (src/z4c/adm_z4c.cpp:972-1120): first
    derivatives of alpha, chi, beta^i, g~_ij on the extended ranges, kept in
-   the context for the next z4c_rhs / z4c_adm_constraints.  `aux` (may be
-   NULL) receives the 30 physical ADM derivatives (storage.aux, interior). */
+   the context.  `aux` (may be NULL) receives the 30 physical ADM
+   derivatives (storage.aux, interior).  The right-hand side does not need
+   them here (its stencil pass forms every derivative it uses) and
+   z4c_adm_constraints prepares its own. */
 int z4c_prepare_derivatives(z4c_ctx* ctx, const z4c_params* p, const double* u,
                             double* aux, void* stream);
 
 /* Z4c::Z4cRHS (src/z4c/calculate_z4c_rhs.cpp:29-916) incl. KO dissipation,
-   interior points only, from the derivatives prepared for `u`.  `mat` (10
-   fields, may be NULL for vacuum).  Ghosts of rhs are not written. */
+   interior points only; all derivatives are formed from `u` (ghosts
+   included).  `mat` must be NULL (vacuum; matter sources are rejected in
+   this build).  Ghosts of rhs are not written. */
 int z4c_rhs(z4c_ctx* ctx, const z4c_params* p, const double* u,
             const double* mat, double* rhs, void* stream);
 
@@ -136,7 +139,9 @@ int z4c_z4c_to_adm(z4c_ctx* ctx, const z4c_params* p, const double* u,
                    double* adm, double* aux_ext, void* stream);
 
 /* Z4c::ADMConstraints (src/z4c/adm_z4c.cpp:382-930): con (7 fields) zeroed
-   then set on the interior; needs derivatives prepared for `u`. */
+   then set on the interior.  First prepares the derivatives of `u`
+   (PrepareZ4cDerivatives), as PREP_Z4C_DERIV precedes ADM_CONSTR in the
+   task list (src/task_list/gr/task_list_gr_z4c.cpp:67-71). */
 int z4c_adm_constraints(z4c_ctx* ctx, const z4c_params* p, const double* u,
                         const double* adm, const double* mat, double* con,
                         void* stream);
@@ -200,10 +205,11 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
    ghost vertices held at u_n (no exchange) and Sommerfeld faces honoured.
    Per stage: the stencil and curvature passes of Z4cRHS, then one kernel
    that assembles the RHS and applies SumBT and AlgConstr at each interior
-   vertex; the ghosts get AlgConstr(u_n) once.  Ends with
-   PrepareZ4cDerivatives of the new state, as the final stage of the task
-   list does.  The results equal z4c_rk4_stage over stages 1..4 (same sums in
-   the same order) without k registers: `work` is caller-owned device memory
+   vertex; the ghosts get AlgConstr(u_n) once.  The derivative arrays of
+   PrepareZ4cDerivatives are not formed (nothing in the step reads them;
+   z4c_adm_constraints prepares its own).  The state equals z4c_rk4_stage
+   over stages 1..4 (same sums in the same order) without k registers:
+   `work` is caller-owned device memory
    of 3 * 22 * elems_per_var doubles (two stage states and the running sum).
    `u` holds u_n on entry and u_{n+1} on return. */
 int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,

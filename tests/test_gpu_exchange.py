@@ -77,6 +77,33 @@ def test_split_gauge_wave_evolution_equals_single_block():
         z.rk4_step(dt, u, un, k, exchange=True)
     torch.cuda.synchronize()
     parts = z.download(u, 22)
+    # against the oracle's multi-block stage sequence with its exchange
+    og = [orc.Geom.block(g.nx, g.dx, g.xmin) for g in gs]
+    op = orc.Params(**params.as_dict())
+    lo, hi, per = [-0.5] * 3, [0.5] * 3, [1, 1, 1]
+    us = [orc.adm_to_z4c(op, o, np.ascontiguousarray(gauge_wave_adm(g, A=0.01, d=1.0)))
+          for o, g in zip(og, gs)]
+    orc.exchange(og, us, lo, hi, per)
+    for o, x in zip(og, us):
+        orc.algconstr_range(op, o, x, 0, o.nn1 - 1, 0, o.nn2 - 1, 0, o.nn3 - 1, 1)
+    for _ in range(nsteps):
+        un_h = [x.copy() for x in us]
+        ks = [[] for _ in us]
+        for st in range(4):
+            for b, o in enumerate(og):
+                ks[b].append(orc.rhs(op, o, us[b]))
+                w = orc.BT_RK4_A[st + 1][:st + 1] if st < 3 else orc.BT_RK4_B
+                us[b] = orc.bt_sum(o, un_h[b], ks[b], w, dt)
+            orc.exchange(og, us, lo, hi, per)
+            for o, x in zip(og, us):
+                orc.algconstr(op, o, x)
+    for b in range(len(gs)):
+        num = np.abs(parts[b] - us[b]).reshape(22, -1).max(axis=1)
+        den = np.maximum(np.abs(us[b]).reshape(22, -1).max(axis=1), 0.01)
+        assert np.all(num / den <= 1e-11), (b, num / den)
+    # and, physically, the same state as the single block: the two differ
+    # only in the summation order of the 4- and 8-fold shared vertices,
+    # which the second-derivative operators amplify in Theta and A~
     sub = n // 2
     for b, g in enumerate(gs):
         a, c = b % 2, b // 2
@@ -84,4 +111,4 @@ def test_split_gauge_wave_evolution_equals_single_block():
         ref = whole[:, k0:k0 + g.shape[0], :, i0:i0 + g.shape[2]]
         num = np.abs(parts[b] - ref).reshape(22, -1).max(axis=1)
         den = np.maximum(np.abs(ref).reshape(22, -1).max(axis=1), 0.01)
-        assert np.all(num / den <= 1e-11), (b, num / den)
+        assert np.all(num / den <= 1e-10), (b, num / den)
