@@ -170,6 +170,10 @@ def main():
     dt = min(z.NewBlockTimeStep(0.25), 1.0)
     torch.cuda.synchronize()
     log("initial data ready")
+    t_tune = time.perf_counter()
+    plan = z.autotune(u, reps=3)
+    log(f"stencil workgroup sizes by block size (autotuned in {time.perf_counter() - t_tune:.2f} s): "
+        + ", ".join(f"{'x'.join(map(str, k))}: {v}" for k, v in sorted(plan.items())))
 
     def step():
         z.rk4_step_fused(dt, u, work)
@@ -214,7 +218,8 @@ def main():
                                "bt_rk4, one launch per kernel per stage",
                    "nblocks": len(blocks), "owned_points": owned,
                    "interior_points": interior_pts, "padded_points": z.total_points,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "stencil_threads": {"x".join(map(str, k)): v for k, v in sorted(plan.items())}},
     }
     dom = max((k for k in pinfo if "gbs" in pinfo[k]), key=lambda k: pinfo[k]["avg_ms"] * pinfo[k]["launches"])
     d = pinfo[dom]

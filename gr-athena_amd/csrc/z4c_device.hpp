@@ -45,7 +45,7 @@ __host__ __device__ constexpr int sym(int a, int b) {
 // Derivative slots of a block: NDV rows of nis = nint + DV_PAD doubles; the
 // DV_PAD entries past the interior take the stores of stencil threads that
 // lie past their tile (never read).
-constexpr int DV_PAD = 256;
+constexpr int DV_PAD = 384;
 
 struct DevBlock {
   long long off;     // point offset in state buffers (block data at off*nvar)

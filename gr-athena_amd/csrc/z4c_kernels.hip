@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -202,8 +203,9 @@ __device__ __forceinline__ bool on_sommerfeld_face(const DevBlock& B, int i,
 // (tile, field) pairs, so the 22 fields of a tile stream through the same
 // L2 together: the shift planes every field reads, and the halo rows of
 // neighbouring tiles, come from HBM once.
-__global__ void __launch_bounds__(ST_THREADS) k_stencil(StencilArgs a) {
-  __shared__ double sm[ST_LDS];
+template <class Cfg>
+__global__ void __launch_bounds__(Cfg::THREADS) k_stencil(StencilArgs a) {
+  __shared__ double sm[2 * Cfg::BUF];
   const int G = gridDim.x;  // multiple of 8: XCD-contiguous ranges
   const int w = blockIdx.x;
   const int t = (w & 7) * (G >> 3) + (w >> 3);
@@ -212,9 +214,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_stencil(StencilArgs a) {
   const int f = t - tile * NVAR;
   const PlaneTile T = a.tiles[tile];
   switch (dkind(f)) {
-    case DK_FULL: stencil_tile<DK_FULL>(a, T, f, sm); break;
-    case DK_GRAD: stencil_tile<DK_GRAD>(a, T, f, sm); break;
-    default: stencil_tile<DK_ADV>(a, T, f, sm); break;
+    case DK_FULL: stencil_tile<Cfg, DK_FULL>(a, T, f, sm); break;
+    case DK_GRAD: stencil_tile<Cfg, DK_GRAD>(a, T, f, sm); break;
+    default: stencil_tile<Cfg, DK_ADV>(a, T, f, sm); break;
   }
 }
 
@@ -1126,8 +1128,12 @@ struct z4c_ctx {
   double* d1 = nullptr;     // prepared first derivatives (PrepareZ4cDerivatives)
   double* dv = nullptr;     // derivative slots of the RHS stencil pass
   double* cv = nullptr;     // curvature part of the RHS algebra
-  PlaneTile* d_ptiles = nullptr;
-  int nptiles = 0;
+  // stencil plan: the workgroup configuration of every block, and per
+  // configuration the plane tiles of its blocks
+  std::vector<int> blk_cfg;
+  std::vector<DevBlock> hdev;  // host copy of the device block table
+  PlaneTile* st_tiles[NSTCFG] = {};
+  int st_ntiles[NSTCFG] = {};
   long long total_int = 0;  // interior points
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
@@ -1166,7 +1172,11 @@ struct z4c_ctx {
     hipFree(d1);
     hipFree(dv);
     hipFree(cv);
-    hipFree(d_ptiles);
+    for (int q = 0; q < NSTCFG; ++q) {
+      hipFree(st_tiles[q]);
+      st_tiles[q] = nullptr;
+      st_ntiles[q] = 0;
+    }
     hipFree(snap);
     hipFree(hst);
     hst = nullptr;
@@ -1180,8 +1190,7 @@ struct z4c_ctx {
     d1 = nullptr;
     dv = nullptr;
     cv = nullptr;
-    d_ptiles = nullptr;
-    nptiles = 0;
+    blk_cfg.clear();
     snap = nullptr;
     snap_elems = 0;
   }
@@ -1263,16 +1272,101 @@ int launch_prep(z4c_ctx* c, const z4c_params* p, const double* u, double* aux,
   return Z4C_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Stencil-pass plan
+// ---------------------------------------------------------------------------
+constexpr int ST_THREADS_OF[NSTCFG] = {StCfg256::THREADS, StCfg320::THREADS,
+                                       StCfg192::THREADS, StCfg384::THREADS};
+
+bool cfg_fits(int cfg, int ni, int nj) {
+  switch (cfg) {
+    case 0: return StCfg256::fits(ni, nj);
+    case 1: return StCfg320::fits(ni, nj);
+    case 2: return StCfg192::fits(ni, nj);
+    default: return StCfg384::fits(ni, nj);
+  }
+}
+
+// The fewest x-y tiles of an ni x nj plane that fit configuration `cfg`,
+// preferring the widest x extent (coalescing); false if none fits.
+bool plane_tiles(int cfg, int b, int ni, int nj, std::vector<PlaneTile>* out) {
+  int best_tx = 0, best_ty = 0, best_n = 0;
+  for (int ntx = 1; ntx <= ni; ++ntx) {
+    const int tx = (ni + ntx - 1) / ntx;
+    int nty = 1, ty = nj;
+    while (nty <= nj && !cfg_fits(cfg, tx, ty)) {
+      ++nty;
+      ty = (nj + nty - 1) / nty;
+    }
+    if (!cfg_fits(cfg, tx, ty)) continue;
+    if (best_n == 0 || ntx * nty < best_n) {
+      best_n = ntx * nty;
+      best_tx = tx;
+      best_ty = ty;
+    }
+  }
+  if (best_n == 0) return false;
+  if (out)
+    for (int j0 = 0; j0 < nj; j0 += best_ty)
+      for (int i0 = 0; i0 < ni; i0 += best_tx)
+        out->push_back(PlaneTile{b, i0, j0, std::min(best_tx, ni - i0), std::min(best_ty, nj - j0)});
+  return true;
+}
+
+// Thread slots (tiles x threads) of a block under `cfg`, or -1.
+long long cfg_slots(int cfg, int ni, int nj) {
+  std::vector<PlaneTile> t;
+  if (!plane_tiles(cfg, 0, ni, nj, &t)) return -1;
+  return (long long)t.size() * ST_THREADS_OF[cfg];
+}
+
+// Uploads the tile lists of the blocks' configurations.
+int build_stencil_plan(z4c_ctx* c, const std::vector<DevBlock>& hb) {
+  std::vector<PlaneTile> lists[NSTCFG];
+  for (int b = 0; b < (int)hb.size(); ++b)
+    if (!plane_tiles(c->blk_cfg[b], b, hb[b].ni, hb[b].nj, &lists[c->blk_cfg[b]]))
+      return fail(Z4C_EINVAL, "stencil plan: no tile fits block " + std::to_string(b));
+  for (int q = 0; q < NSTCFG; ++q) {
+    hipFree(c->st_tiles[q]);
+    c->st_tiles[q] = nullptr;
+    c->st_ntiles[q] = 0;
+    if (lists[q].empty()) continue;
+    if (hipMalloc(&c->st_tiles[q], sizeof(PlaneTile) * lists[q].size()) != hipSuccess)
+      return fail(Z4C_ENOMEM, "stencil plan: allocation failed");
+    HIPCALL(hipMemcpy(c->st_tiles[q], lists[q].data(), sizeof(PlaneTile) * lists[q].size(),
+                      hipMemcpyHostToDevice));
+    c->st_ntiles[q] = (int)lists[q].size();
+  }
+  return Z4C_OK;
+}
+
+void launch_stencil_cfg(int cfg, const StencilArgs& sa, hipStream_t s) {
+  const dim3 g(grid_for(sa.ntiles * NVAR));
+  switch (cfg) {
+    case 0: hipLaunchKernelGGL(k_stencil<StCfg256>, g, dim3(StCfg256::THREADS), 0, s, sa); break;
+    case 1: hipLaunchKernelGGL(k_stencil<StCfg320>, g, dim3(StCfg320::THREADS), 0, s, sa); break;
+    case 2: hipLaunchKernelGGL(k_stencil<StCfg192>, g, dim3(StCfg192::THREADS), 0, s, sa); break;
+    default: hipLaunchKernelGGL(k_stencil<StCfg384>, g, dim3(StCfg384::THREADS), 0, s, sa); break;
+  }
+}
+
+// The stencil pass over all blocks: one launch per configuration in use.
+int launch_stencil(z4c_ctx* c, const double* u, double diss, hipStream_t s) {
+  PassTimer t(c, Z4C_PASS_STENCIL, s);
+  for (int q = 0; q < NSTCFG; ++q) {
+    if (!c->st_ntiles[q]) continue;
+    StencilArgs sa{c->d_blocks, c->st_tiles[q], c->st_ntiles[q], u, c->dv, diss};
+    launch_stencil_cfg(q, sa, s);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
 int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
                const double* mat, double* rhs, bool sommerfeld, hipStream_t s) {
   if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
-  StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, u, c->dv, p->diss};
-  {
-    PassTimer t(c, Z4C_PASS_STENCIL, s);
-    hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles * NVAR)), dim3(ST_THREADS), 0, s,
-                       sa);
-  }
-  HIPCALL(hipGetLastError());
+  int rc = launch_stencil(c, u, p->diss, s);
+  if (rc) return rc;
   RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
   const int g = grid_for(a.L.ntiles);
   {
@@ -1415,7 +1509,7 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   std::vector<Box> boxes[NBOX];
   std::vector<Tile> tiles[NBOX];
   long long off = 0, ioff = 0, doff = 0;
-  std::vector<PlaneTile> ptiles;
+  std::vector<int> blk_cfg;
   const double diss_scaling = std::pow(2.0, -2 * NG) * (NG % 2 == 0 ? -1 : 1);
   for (int b = 0; b < nblocks; ++b) {
     const z4c_block& in = blocks[b];
@@ -1448,31 +1542,20 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     if ((long long)B.nis * NDV > std::numeric_limits<int>::max())
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
-    // x-y plane tiles of the stencil pass: the fewest tiles that fit the
-    // 256-thread mapping, preferring the widest x extent (coalescing)
+    // stencil configuration: the fewest thread slots (ties: lowest index)
     {
-      int best_tx = 0, best_ty = 0, best_n = 0;
-      for (int ntx = 1; ntx <= B.ni; ++ntx) {
-        const int tx = (B.ni + ntx - 1) / ntx;
-        int nty = 1, ty = B.nj;
-        while (nty <= B.nj && !stencil_tile_fits(tx, ty)) {
-          ++nty;
-          ty = (B.nj + nty - 1) / nty;
-        }
-        if (!stencil_tile_fits(tx, ty)) continue;
-        if (best_n == 0 || ntx * nty < best_n) {
-          best_n = ntx * nty;
-          best_tx = tx;
-          best_ty = ty;
+      int best = -1;
+      long long best_slots = 0;
+      for (int q = 0; q < NSTCFG; ++q) {
+        const long long sl = cfg_slots(q, B.ni, B.nj);
+        if (sl > 0 && (best < 0 || sl < best_slots)) {
+          best = q;
+          best_slots = sl;
         }
       }
-      if (best_n == 0)
+      if (best < 0)
         return fail(Z4C_EINVAL, "set_blocks: no stencil tile fits block " + std::to_string(b));
-      for (int j0 = 0; j0 < B.nj; j0 += best_ty)
-        for (int i0 = 0; i0 < B.ni; i0 += best_tx) {
-          PlaneTile pt{b, i0, j0, std::min(best_tx, B.ni - i0), std::min(best_ty, B.nj - j0)};
-          ptiles.push_back(pt);
-        }
+      blk_cfg.push_back(best);
     }
     for (int d = 0; d < 3; ++d) {
       const double dx = in.dx[d];
@@ -1541,11 +1624,12 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   if (hipMalloc(&c->dv, sizeof(double) * doff) != hipSuccess ||
       hipMalloc(&c->cv, sizeof(double) * ioff * 12) != hipSuccess)
     return fail(Z4C_ENOMEM, "set_blocks: derivative-slot allocation failed");
-  if (hipMalloc(&c->d_ptiles, sizeof(PlaneTile) * ptiles.size()) != hipSuccess)
-    return fail(Z4C_ENOMEM, "set_blocks: allocation failed");
-  HIPCALL(hipMemcpy(c->d_ptiles, ptiles.data(), sizeof(PlaneTile) * ptiles.size(),
-                    hipMemcpyHostToDevice));
-  c->nptiles = (int)ptiles.size();
+  c->blk_cfg = blk_cfg;
+  c->hdev = hb;
+  {
+    const int rc = build_stencil_plan(c, hb);
+    if (rc) return rc;
+  }
   c->hblocks.assign(blocks, blocks + nblocks);
   c->nblocks = nblocks;
   c->total_pts = off;
@@ -1843,6 +1927,91 @@ int z4c_exchange(z4c_ctx* c, int nvar, double* u, void* stream) {
   return launch_exchange(c, nvar, u, (hipStream_t)stream);
 }
 
+int z4c_autotune(z4c_ctx* c, const z4c_params* p, const double* u, int reps, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || reps < 1) return fail(Z4C_EINVAL, "autotune: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  std::map<std::array<int, 3>, std::vector<int>> classes;
+  for (int b = 0; b < c->nblocks; ++b)
+    classes[{c->hdev[b].ni, c->hdev[b].nj, c->hdev[b].nk}].push_back(b);
+  hipEvent_t e0, e1;
+  HIPCALL(hipEventCreate(&e0));
+  HIPCALL(hipEventCreate(&e1));
+  PlaneTile* d = nullptr;
+  rc = Z4C_OK;
+  for (auto& cl : classes) {
+    int best = -1;
+    float best_ms = 0.f;
+    for (int q = 0; q < NSTCFG && rc == Z4C_OK; ++q) {
+      std::vector<PlaneTile> tl;
+      bool ok = true;
+      for (int b : cl.second) ok = ok && plane_tiles(q, b, c->hdev[b].ni, c->hdev[b].nj, &tl);
+      if (!ok || tl.empty()) continue;
+      if (hipMalloc(&d, sizeof(PlaneTile) * tl.size()) != hipSuccess) {
+        rc = fail(Z4C_ENOMEM, "autotune: allocation failed");
+        break;
+      }
+      if (hipMemcpy(d, tl.data(), sizeof(PlaneTile) * tl.size(), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        rc = fail(Z4C_EHIP, "autotune: copy failed");
+        break;
+      }
+      StencilArgs sa{c->d_blocks, d, (int)tl.size(), u, c->dv, p->diss};
+      launch_stencil_cfg(q, sa, s);  // warm-up
+      hipEventRecord(e0, s);
+      for (int r = 0; r < reps; ++r) launch_stencil_cfg(q, sa, s);
+      hipEventRecord(e1, s);
+      float ms = 0.f;
+      if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(Z4C_EHIP, "autotune: timing failed");
+        break;
+      }
+      hipFree(d);
+      d = nullptr;
+      if (best < 0 || ms < best_ms) {
+        best = q;
+        best_ms = ms;
+      }
+    }
+    if (rc) break;
+    if (best >= 0)
+      for (int b : cl.second) c->blk_cfg[b] = best;
+  }
+  hipFree(d);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  return build_stencil_plan(c, c->hdev);
+}
+
+int z4c_set_stencil_threads(z4c_ctx* c, int block, int threads) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (block < -1 || block >= c->nblocks) return fail(Z4C_EINVAL, "set_stencil_threads: bad block");
+  int q = -1;
+  for (int k = 0; k < NSTCFG; ++k)
+    if (ST_THREADS_OF[k] == threads) q = k;
+  if (q < 0) return fail(Z4C_EINVAL, "set_stencil_threads: threads must be 192, 256, 320 or 384");
+  const int b0 = block < 0 ? 0 : block, b1 = block < 0 ? c->nblocks : block + 1;
+  for (int b = b0; b < b1; ++b)
+    if (!plane_tiles(q, b, c->hdev[b].ni, c->hdev[b].nj, nullptr))
+      return fail(Z4C_EINVAL, "set_stencil_threads: no tile of that size fits block " +
+                                  std::to_string(b));
+  for (int b = b0; b < b1; ++b) c->blk_cfg[b] = q;
+  return build_stencil_plan(c, c->hdev);
+}
+
+int z4c_stencil_threads(z4c_ctx* c, int block, int* threads) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!threads || block < 0 || block >= c->nblocks)
+    return fail(Z4C_EINVAL, "stencil_threads: bad argument");
+  *threads = ST_THREADS_OF[c->blk_cfg[block]];
+  return Z4C_OK;
+}
+
 int z4c_set_timing(z4c_ctx* c, int enable) {
   if (!c) return fail(Z4C_EINVAL, "set_timing: null context");
   c->timing = enable != 0;
@@ -1905,13 +2074,8 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
     a.force_reg = p->force_regularization;
     const int g = grid_for(a.r.L.ntiles);
     if (!p->cowling) {
-      StencilArgs sa{c->d_blocks, c->d_ptiles, c->nptiles, in, c->dv, p->diss};
-      {
-        PassTimer t(c, Z4C_PASS_STENCIL, s);
-        hipLaunchKernelGGL(k_stencil, dim3(grid_for(c->nptiles * NVAR)), dim3(ST_THREADS), 0,
-                           s, sa);
-      }
-      HIPCALL(hipGetLastError());
+      int rc2 = launch_stencil(c, in, p->diss, s);
+      if (rc2) return rc2;
       {
         PassTimer t(c, Z4C_PASS_CURVATURE, s);
         hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a.r);

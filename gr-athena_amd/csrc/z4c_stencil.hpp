@@ -34,10 +34,30 @@
 
 namespace z4c {
 
-constexpr int ST_THREADS = 256;  // workgroup size; tiles hold <= 256 points
-constexpr int ST_CUR = 3;        // LDS-plane elements per thread: (ni+8)(nj+8) <= 768
-constexpr int ST_EXT = 2;        // Dx_z columns per thread: (ni+6)(nj+6) <= 512
-static_assert(DV_PAD >= ST_THREADS, "sink slots must cover a workgroup");
+// Workgroup configurations of the stencil pass: THREADS threads, a tile of
+// at most THREADS interior points, CUR LDS-plane elements and EXT Dx_z
+// columns per thread.  The host picks one per block size (by measurement,
+// z4c_autotune, or by the fewest thread slots).
+template <int TH, int CUR_, int EXT_>
+struct StencilCfg {
+  static constexpr int THREADS = TH;
+  static constexpr int CUR = CUR_;
+  static constexpr int EXT = EXT_;
+  // LDS doubles of one buffer: plane, Dx_y plane, Dx_z plane
+  static constexpr int BUF = CUR * THREADS + 2 * EXT * THREADS;
+  // whether a tile of ni x nj interior points fits the thread mapping
+  __host__ __device__ static constexpr bool fits(int ni, int nj) {
+    return ni * nj <= THREADS && (ni + 8) * (nj + 8) <= CUR * THREADS &&
+           (ni + 6) * (nj + 6) <= EXT * THREADS;
+  }
+};
+using StCfg256 = StencilCfg<256, 3, 2>;
+using StCfg320 = StencilCfg<320, 3, 2>;
+using StCfg192 = StencilCfg<192, 4, 3>;
+using StCfg384 = StencilCfg<384, 2, 2>;
+constexpr int NSTCFG = 4;
+constexpr int ST_MAX_THREADS = 384;
+static_assert(DV_PAD >= ST_MAX_THREADS, "sink slots must cover a workgroup");
 
 struct StencilArgs {
   const DevBlock* blocks;
@@ -48,20 +68,13 @@ struct StencilArgs {
   double diss;
 };
 
-// Whether a tile of ni x nj interior points fits the thread mapping.
-__host__ __device__ constexpr bool stencil_tile_fits(int ni, int nj) {
-  return ni * nj <= ST_THREADS && (ni + 8) * (nj + 8) <= ST_CUR * ST_THREADS &&
-         (ni + 6) * (nj + 6) <= ST_EXT * ST_THREADS;
-}
-
-// LDS doubles of one buffer: plane, Dx_y plane, Dx_z plane (fixed sizes).
-constexpr int ST_BUF = ST_CUR * ST_THREADS + 2 * ST_EXT * ST_THREADS;
-// LDS doubles per workgroup (double-buffered).
-constexpr int ST_LDS = 2 * ST_BUF;
-
-template <int KIND>
+template <class Cfg, int KIND>
 __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTile T,
                                              int f, double* sm) {
+  constexpr int ST_THREADS = Cfg::THREADS;
+  constexpr int ST_CUR = Cfg::CUR;
+  constexpr int ST_EXT = Cfg::EXT;
+  constexpr int ST_BUF = Cfg::BUF;
   constexpr bool full = KIND == DK_FULL;
   constexpr bool grad = KIND != DK_ADV;
   const DevBlock B = a.blocks[T.block];

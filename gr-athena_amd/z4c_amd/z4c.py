@@ -256,6 +256,27 @@ class Z4c:
                                  _stream(stream)), "history")
         return list(out)
 
+    def autotune(self, u, reps=3, stream=None):
+        """Time the stencil-pass workgroup configurations per block size on
+        state ``u`` and keep the fastest; returns {block size: threads}."""
+        check(self.L.z4c_autotune(self._ctx, C.byref(self.params), _ptr(u), int(reps),
+                                  _stream(stream)), "autotune")
+        return self.stencil_plan()
+
+    def set_stencil_threads(self, threads, block=-1):
+        """Fix the stencil workgroup size of ``block`` (-1: all blocks)."""
+        check(self.L.z4c_set_stencil_threads(self._ctx, int(block), int(threads)),
+              "set_stencil_threads")
+
+    def stencil_plan(self):
+        """{interior vertex counts (ni, nj, nk): stencil workgroup size}."""
+        out = {}
+        t = C.c_int()
+        for b, g in enumerate(self.blocks):
+            check(self.L.z4c_stencil_threads(self._ctx, b, C.byref(t)), "stencil_threads")
+            out[tuple(n + 1 for n in g.nx)] = t.value
+        return out
+
     def set_timing(self, enable=True):
         """Bracket every pass launch with HIP events on its stream."""
         check(self.L.z4c_set_timing(self._ctx, int(enable)), "set_timing")

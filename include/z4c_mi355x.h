@@ -224,6 +224,20 @@ int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
 int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
                 const double* con, double* out, void* stream);
 
+/* Tile autotuner of the stencil pass (no counterpart in the reference,
+   whose pencil loops have no tiling).  For every block size among the
+   registered blocks, times the stencil pass of those blocks under each
+   workgroup configuration (192, 256, 320 or 384 threads, each with the
+   fewest x-y tiles that fit) on state `u`, `reps` launches each, and keeps
+   the fastest.  Until it is called a block uses the configuration with the
+   fewest thread slots.  Results do not depend on the configuration.
+   Waits for the stream. */
+int z4c_autotune(z4c_ctx* ctx, const z4c_params* p, const double* u, int reps, void* stream);
+/* Workgroup size the stencil pass uses for `block`; setting it (block -1:
+   all blocks) replaces the tuned or default choice. */
+int z4c_stencil_threads(z4c_ctx* ctx, int block, int* threads);
+int z4c_set_stencil_threads(z4c_ctx* ctx, int block, int threads);
+
 /* Passes of the kernel timer. */
 enum {
   Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */
