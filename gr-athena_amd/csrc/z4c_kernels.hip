@@ -1144,6 +1144,13 @@ struct z4c_ctx {
   int ex_ndst = 0;
   std::vector<double> hst_host;
   int n_cu = 256;
+  // captured fused step (z4c_rk4_step_graph) and what it was captured for
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const double* g_u = nullptr;
+  const double* g_work = nullptr;
+  double g_dt = 0.0;
+  z4c_params g_params{};
   // per-pass kernel timing (events recorded on the launch stream)
   struct TimedLaunch {
     int pass;
@@ -1160,7 +1167,14 @@ struct z4c_ctx {
     tev.clear();
   }
 
+  // the captured step holds device pointers of the block table and plans
+  void invalidate_graph() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+  }
+
   void release() {
+    invalidate_graph();
     hipFree(d_blocks);
     for (int b = 0; b < NBOX; ++b) {
       hipFree(d_tiles[b]);
@@ -1322,6 +1336,7 @@ long long cfg_slots(int cfg, int ni, int nj) {
 
 // Uploads the tile lists of the blocks' configurations.
 int build_stencil_plan(z4c_ctx* c, const std::vector<DevBlock>& hb) {
+  c->invalidate_graph();
   std::vector<PlaneTile> lists[NSTCFG];
   for (int b = 0; b < (int)hb.size(); ++b)
     if (!plane_tiles(c->blk_cfg[b], b, hb[b].ni, hb[b].nj, &lists[c->blk_cfg[b]]))
@@ -1497,6 +1512,7 @@ int z4c_ctx_destroy(z4c_ctx* c) {
   hipSetDevice(c->device);
   c->release();
   c->drop_timing();
+  if (c->cap_stream) hipStreamDestroy(c->cap_stream);
   delete c;
   return Z4C_OK;
 }
@@ -2041,13 +2057,12 @@ int z4c_get_timing(z4c_ctx* c, double* ms_total, long long* launches, int npass)
   return rc;
 }
 
-int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
-                 const double* mat, void* stream) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step: bad argument");
-  if (mat) return fail(Z4C_EINVAL, "rk4_step: matter sources are not enabled in this build (vacuum only)");
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                    hipStream_t s) {
+  int rc = Z4C_OK;
   const long long n = c->total_pts * NVAR;
   double* W0 = work;
   double* W1 = work + n;
@@ -2097,6 +2112,55 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
       HIPCALL(hipGetLastError());
     }
   }
+  return Z4C_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                 const double* mat, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step: bad argument");
+  if (mat) return fail(Z4C_EINVAL, "rk4_step: matter sources are not enabled in this build (vacuum only)");
+  return launch_rk4_step(c, p, dt, u, work, (hipStream_t)stream);
+}
+
+int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                       void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step_graph: bad argument");
+  if (c->timing) return fail(Z4C_ESTATE, "rk4_step_graph: the kernel timer cannot be captured");
+  hipStream_t s = (hipStream_t)stream;
+  const bool same = c->graph_exec && c->g_u == u && c->g_work == work && c->g_dt == dt &&
+                    std::memcmp(&c->g_params, p, sizeof(z4c_params)) == 0;
+  if (!same) {
+    c->invalidate_graph();
+    if (!c->cap_stream) HIPCALL(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    HIPCALL(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_rk4_step(c, p, dt, u, work, c->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess || !g) return fail(Z4C_EHIP, "rk4_step_graph: capture failed");
+    const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      c->graph_exec = nullptr;
+      return fail(Z4C_EHIP, "rk4_step_graph: instantiation failed");
+    }
+    c->g_u = u;
+    c->g_work = work;
+    c->g_dt = dt;
+    c->g_params = *p;
+  }
+  HIPCALL(hipGraphLaunch(c->graph_exec, s));
   return Z4C_OK;
 }
 

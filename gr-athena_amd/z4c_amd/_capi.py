@@ -24,7 +24,7 @@ EXPORTS = [
     "z4c_adm_constraints", "z4c_bt_sum", "z4c_weighted_ave", "z4c_add_rhs",
     "z4c_new_dt", "z4c_periodic_fill", "z4c_rk4_stage", "z4c_rk4_step",
     "z4c_set_timing", "z4c_get_timing", "z4c_history", "z4c_set_exchange", "z4c_exchange",
-    "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads",
+    "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads", "z4c_rk4_step_graph",
 ]
 
 # kernel-timer passes (Z4C_PASS_* of the header)
@@ -118,6 +118,7 @@ def load(path: str | None = None):
         "z4c_autotune": [vp, pp, vp, i, vp],
         "z4c_stencil_threads": [vp, i, C.POINTER(i)],
         "z4c_set_stencil_threads": [vp, i, i],
+        "z4c_rk4_step_graph": [vp, pp, d, vp, vp, vp],
         "z4c_get_timing": [vp, C.POINTER(d), C.POINTER(ll), i],
     }
     for name, args in sig.items():

@@ -236,12 +236,20 @@ class Z4c:
         for s in range(1, 5):
             self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, stream)
 
-    def rk4_step_fused(self, dt, u, work, mat=None, stream=None):
+    def rk4_step_fused(self, dt, u, work, mat=None, stream=None, graph=False):
         """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr
         in one kernel), ghosts held at u_n.  The state equals ``rk4_step``
         with no ghost update; the derivative arrays are not prepared
         (ADMConstraints prepares its own).  ``work`` comes from
-        ``make_step_work``."""
+        ``make_step_work``.  ``graph``: replay the step from a HIP graph
+        (captured on first use and whenever u, work, dt or the parameters
+        change)."""
+        if graph:
+            if mat is not None:
+                raise _capi.Z4cError("rk4_step_graph: matter sources are not enabled (vacuum only)")
+            check(self.L.z4c_rk4_step_graph(self._ctx, C.byref(self.params), float(dt), _ptr(u),
+                                            _ptr(work), _stream(stream)), "rk4_step_graph")
+            return
         check(self.L.z4c_rk4_step(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                   _ptr(work), _ptr(mat), _stream(stream)), "rk4_step")
 

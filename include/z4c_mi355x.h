@@ -215,6 +215,15 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
 int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                  double* work, const double* mat, void* stream);
 
+/* z4c_rk4_step replayed from a HIP graph (no counterpart in the reference:
+   its task list launches nothing).  The first call, and any call whose u,
+   work, dt or parameters differ from the captured ones, captures the step's
+   launches on a stream of the context; every call then launches the graph
+   on `stream`.  Z4C_ESTATE while the kernel timer is on.  Results equal
+   z4c_rk4_step's. */
+int z4c_rk4_step_graph(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
+                       double* work, void* stream);
+
 /* Z4c constraint sums of the history output (src/outputs/history.cpp:178-205):
    out[8] = sum over blocks and over the reference's cell-index loop (vertices
    il..iu-1, jl..ju-1, kl..ku-1) of vol*{H^2, |M|, Mx^2, My^2, Mz^2, |Z|,

@@ -52,3 +52,28 @@ def test_stencil_configurations_agree_and_autotune():
     torch.cuda.synchronize()
     for o in outs[1:]:
         assert torch.equal(o, outs[0])
+
+
+def test_graph_step_replays_equal_eager_steps():
+    """The captured fused step, replayed several times with eager work and
+    synchronisations in between, gives the eager step's state every time."""
+    gs = [MeshBlockGeom.cube(8, 0.125, 0.3, faces=63), MeshBlockGeom.cube(12, 0.1, 3.0)]
+    p = Params(diss=0.02, damp_kappa1=0.02, shift_eta=2.0)
+    z = Z4c(gs, p)
+    host = [perturbed_flat(g, seed=40 + b) for b, g in enumerate(gs)]
+    ua, ub = z.upload(host), z.upload(host)
+    wa, wb = z.make_step_work(), z.make_step_work()
+    z.AlgConstr(ua)
+    z.AlgConstr(ub)
+    for rep in range(4):
+        z.rk4_step_fused(0.005, ua, wa)
+        z.rk4_step_fused(0.005, ub, wb, graph=True)
+        scratch = torch.ones(1000, dtype=torch.float64, device=ua.device) * rep  # eager work
+        torch.cuda.synchronize()
+        assert torch.equal(ua, ub), rep
+        assert float(scratch.sum()) == 1000.0 * rep
+    # a new dt is captured anew
+    z.rk4_step_fused(0.0025, ua, wa)
+    z.rk4_step_fused(0.0025, ub, wb, graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(ua, ub)
