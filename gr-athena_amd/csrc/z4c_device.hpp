@@ -58,6 +58,7 @@ struct DevBlock {
   int ni, nj, nk;    // interior extents
   int nint;          // ni*nj*nk
   int faces;         // Sommerfeld face bits
+  int pfaces;        // faces with the physical ghost fill (Sommerfeld | outflow)
   double dx[3];
   double x0[3];
   double cidx1[3];   // 1/(dx*60)        Dx

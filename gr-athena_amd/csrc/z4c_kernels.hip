@@ -1078,6 +1078,68 @@ __global__ void __launch_bounds__(TILE) k_periodic(PerArgs a) {
 }
 
 
+// Physical ghost fill of vertex-centred data (src/comm/physical_bcs.cpp:
+// 187-290 ExtrapolateOutflowBC, ranges :506-676): one thread per ghost line
+// (a field and two transverse coordinates), filling the four ghost layers
+// outward in order, each from the five vertices inside it, with the weights
+// of InterpolateLagrangeUniformBiasR<5>.  One launch per axis.
+struct PhysLine {
+  int block;
+  int lo0, n0, lo1, n1;  // transverse ranges (fast, slow)
+  long long start;       // first line of this entry
+};
+struct PhysArgs {
+  const DevBlock* blocks;
+  const PhysLine* lines;
+  int nent;
+  long long nlines1;  // lines per field
+  long long nlines;   // nlines1 * nvar
+  int axis;
+  int nvar;
+  double* u;
+};
+
+__global__ void __launch_bounds__(256) k_physical(PhysArgs a) {
+  const long long Lg = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (Lg >= a.nlines) return;
+  const int n = (int)(Lg / a.nlines1);  // field
+  const long long L = Lg - (long long)n * a.nlines1;
+  int lo = 0, hi = a.nent - 1;  // last entry with start <= L
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.lines[mid].start <= L) lo = mid; else hi = mid - 1;
+  }
+  const PhysLine E = a.lines[lo];
+  const DevBlock B = a.blocks[E.block];
+  const int r = (int)(L - E.start);
+  const int t1 = E.lo1 + r / E.n0, t0 = E.lo0 + r % E.n0;
+  const long long str[3] = {1, B.nn1, (long long)B.nn1 * B.nn2};
+  const int ax = a.axis;
+  // position of the line's axis-0 point and the other two coordinates
+  const int o0 = ax == 0 ? 1 : 0, o1 = ax == 2 ? 1 : 2;  // axes of t0, t1
+  double* v = a.u + B.off * a.nvar + (long long)n * B.npts + t0 * str[o0] + t1 * str[o1];
+  const long long s = str[ax];
+  const int lo_v = ax == 0 ? B.il : (ax == 1 ? B.jl : B.kl);
+  const int ie = (ax == 0 ? B.iu : (ax == 1 ? B.ju : B.ku)) - 1;  // cell-centred end
+  const double c[5] = {5., -10., 10., -5., 1.};
+  if (B.pfaces & (1 << (2 * ax))) {
+    for (int i = lo_v - 1; i >= lo_v - NG; --i) {
+      double x = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) x += c[p] * v[(i + p + 1) * s];
+      v[i * s] = x;
+    }
+  }
+  if (B.pfaces & (2 << (2 * ax))) {
+    for (int i = ie + 1; i <= ie + NG; ++i) {
+      double x = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) x += c[p] * v[(i - p - 1) * s];
+      v[i * s] = x;
+    }
+  }
+}
+
 // Same-level VC exchange between blocks (src/comm/comm_channel.cpp:1277-1330
 // additive unpack, src/comm/node_multiplicity.cpp:349-420 division): each
 // destination vertex takes the sum of its owners' values (interior vertices
@@ -1138,6 +1200,9 @@ struct z4c_ctx {
   double* snap = nullptr;  // periodic-fill snapshot (lazy)
   long long snap_elems = 0;
   double* hst = nullptr;   // history partial sums (lazy)
+  PhysLine* ph_lines[3] = {};  // physical-fill line tables per axis
+  int ph_nent[3] = {};
+  long long ph_nlines1[3] = {};  // lines per field
   int* ex_dst = nullptr;   // exchange map (z4c_set_exchange)
   int* ex_off = nullptr;
   int* ex_src = nullptr;
@@ -1194,6 +1259,12 @@ struct z4c_ctx {
     hipFree(snap);
     hipFree(hst);
     hst = nullptr;
+    for (int q = 0; q < 3; ++q) {
+      hipFree(ph_lines[q]);
+      ph_lines[q] = nullptr;
+      ph_nent[q] = 0;
+      ph_nlines1[q] = 0;
+    }
     hipFree(ex_dst);
     hipFree(ex_off);
     hipFree(ex_src);
@@ -1469,6 +1540,51 @@ int ensure_snapshot(z4c_ctx* c, long long n) {
   return Z4C_OK;
 }
 
+// Line tables of the physical fill (per field; scaled by nvar at launch).
+int build_physical_tables(z4c_ctx* c, const std::vector<DevBlock>& hb) {
+  for (int ax = 0; ax < 3; ++ax) {
+    std::vector<PhysLine> t;
+    long long start = 0;
+    for (int b = 0; b < (int)hb.size(); ++b) {
+      const DevBlock& B = hb[b];
+      if (!(B.pfaces & (3 << (2 * ax)))) continue;
+      const int is = B.il, ie = B.iu - 1, js = B.jl, je = B.ju - 1, ks = B.kl, ke = B.ku - 1;
+      const int bis = is - NG, bie = ie + NG;
+      const int bjs = (B.pfaces & 4) ? js : js - NG, bje = (B.pfaces & 8) ? je : je + NG;
+      const int bks = (B.pfaces & 16) ? ks : ks - NG, bke = (B.pfaces & 32) ? ke : ke + NG;
+      PhysLine e{};
+      e.block = b;
+      if (ax == 0) { e.lo0 = bjs; e.n0 = bje - bjs + 1; e.lo1 = bks; e.n1 = bke - bks + 1; }
+      else if (ax == 1) { e.lo0 = bis; e.n0 = bie - bis + 1; e.lo1 = bks; e.n1 = bke - bks + 1; }
+      else { e.lo0 = bis; e.n0 = bie - bis + 1; e.lo1 = js - NG; e.n1 = je - js + 1 + 2 * NG; }
+      e.start = start;
+      start += (long long)e.n0 * e.n1;
+      t.push_back(e);
+    }
+    hipFree(c->ph_lines[ax]);
+    c->ph_lines[ax] = nullptr;
+    c->ph_nent[ax] = (int)t.size();
+    c->ph_nlines1[ax] = start;
+    if (t.empty()) continue;
+    if (hipMalloc(&c->ph_lines[ax], sizeof(PhysLine) * t.size()) != hipSuccess)
+      return fail(Z4C_ENOMEM, "physical fill: allocation failed");
+    HIPCALL(hipMemcpy(c->ph_lines[ax], t.data(), sizeof(PhysLine) * t.size(), hipMemcpyHostToDevice));
+  }
+  return Z4C_OK;
+}
+
+int launch_physical(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!c->ph_nent[ax]) continue;
+    // the table holds one field's lines; the field is the slowest index
+    PhysArgs a{c->d_blocks, c->ph_lines[ax], c->ph_nent[ax], c->ph_nlines1[ax],
+               c->ph_nlines1[ax] * nvar, ax, nvar, u};
+    hipLaunchKernelGGL(k_physical, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
 int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   if (!c->ex_ndst) return Z4C_OK;
   const long long n = c->total_pts * nvar;
@@ -1558,6 +1674,7 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     if ((long long)B.nis * NDV > std::numeric_limits<int>::max())
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
+    B.pfaces = (in.sommerfeld_faces | in.outflow_faces) & 63;
     // stencil configuration: the fewest thread slots (ties: lowest index)
     {
       int best = -1;
@@ -1643,7 +1760,9 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
   c->blk_cfg = blk_cfg;
   c->hdev = hb;
   {
-    const int rc = build_stencil_plan(c, hb);
+    int rc = build_stencil_plan(c, hb);
+    if (rc) return rc;
+    rc = build_physical_tables(c, hb);
     if (rc) return rc;
   }
   c->hblocks.assign(blocks, blocks + nblocks);
@@ -1943,6 +2062,13 @@ int z4c_exchange(z4c_ctx* c, int nvar, double* u, void* stream) {
   return launch_exchange(c, nvar, u, (hipStream_t)stream);
 }
 
+int z4c_physical_bc(z4c_ctx* c, int nvar, double* u, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u || nvar <= 0) return fail(Z4C_EINVAL, "physical_bc: bad argument");
+  return launch_physical(c, nvar, u, (hipStream_t)stream);
+}
+
 int z4c_autotune(z4c_ctx* c, const z4c_params* p, const double* u, int reps, void* stream) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -2186,11 +2312,15 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   else
     rc = launch_bt(c, NVAR, u, un, 4, kk, Bw, dt, s);
   if (rc) return rc;
-  if (periodic == 1) {
+  if (periodic & 1) {
     rc = launch_periodic(c, NVAR, u, s);
     if (rc) return rc;
-  } else if (periodic == 2) {
+  } else if (periodic & 2) {
     rc = launch_exchange(c, NVAR, u, s);
+    if (rc) return rc;
+  }
+  if (periodic & 4) {
+    rc = launch_physical(c, NVAR, u, s);
     if (rc) return rc;
   }
   rc = launch_alg(c, p, u, nullptr, 0, s);

@@ -25,6 +25,7 @@ EXPORTS = [
     "z4c_new_dt", "z4c_periodic_fill", "z4c_rk4_stage", "z4c_rk4_step",
     "z4c_set_timing", "z4c_get_timing", "z4c_history", "z4c_set_exchange", "z4c_exchange",
     "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads", "z4c_rk4_step_graph",
+    "z4c_physical_bc",
 ]
 
 # kernel-timer passes (Z4C_PASS_* of the header)
@@ -68,7 +69,7 @@ class Block(C.Structure):
                 ("nn3", C.c_int), ("il", C.c_int), ("iu", C.c_int), ("jl", C.c_int),
                 ("ju", C.c_int), ("kl", C.c_int), ("ku", C.c_int),
                 ("dx", C.c_double * 3), ("x0", C.c_double * 3), ("level", C.c_int),
-                ("sommerfeld_faces", C.c_int)]
+                ("sommerfeld_faces", C.c_int), ("outflow_faces", C.c_int), ("pad_", C.c_int)]
 
 
 class Z4cError(RuntimeError):
@@ -119,6 +120,7 @@ def load(path: str | None = None):
         "z4c_stencil_threads": [vp, i, C.POINTER(i)],
         "z4c_set_stencil_threads": [vp, i, i],
         "z4c_rk4_step_graph": [vp, pp, d, vp, vp, vp],
+        "z4c_physical_bc": [vp, i, vp, vp],
         "z4c_get_timing": [vp, C.POINTER(d), C.POINTER(ll), i],
     }
     for name, args in sig.items():

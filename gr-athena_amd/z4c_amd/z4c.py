@@ -39,6 +39,7 @@ class MeshBlockGeom:
     xmin: tuple
     level: int = 0
     sommerfeld_faces: int = 0
+    outflow_faces: int = 0
 
     @property
     def nn(self):
@@ -58,10 +59,10 @@ class MeshBlockGeom:
                      for d in range(3))
 
     @staticmethod
-    def cube(nx, dx, xmin, level=0, faces=0):
+    def cube(nx, dx, xmin, level=0, faces=0, outflow=0):
         t = lambda v: tuple(float(x) for x in np.broadcast_to(v, (3,)))
         return MeshBlockGeom(tuple(int(x) for x in np.broadcast_to(nx, (3,))), t(dx), t(xmin),
-                             level, faces)
+                             level, faces, outflow)
 
 
 def _ptr(t):
@@ -106,6 +107,7 @@ class Z4c:
                 x.x0[d] = g.xmin[d] - NGHOST * g.dx[d]
             x.level = g.level
             x.sommerfeld_faces = g.sommerfeld_faces
+            x.outflow_faces = g.outflow_faces
             self.offsets.append(off)
             off += g.npts
         total = C.c_longlong()
@@ -219,22 +221,28 @@ class Z4c:
         """Same-level VC ghost exchange (SEND/RECV/SETB of GR_Z4c)."""
         check(self.L.z4c_exchange(self._ctx, nvar, _ptr(u), _stream(stream)), "exchange")
 
+    def physical_bc(self, u, nvar=Z4C_NVAR, stream=None):
+        """Physical ghost fill on the flagged faces (PHY_BVAL of GR_Z4c)."""
+        check(self.L.z4c_physical_bc(self._ctx, nvar, _ptr(u), _stream(stream)), "physical_bc")
+
     def rk4_stage(self, stage, dt, u, un, k, mat=None, periodic=False, exchange=False,
-                  stream=None):
+                  physical=False, stream=None):
         """One GR_Z4c stage; ghosts after SumBT: the single-block periodic
-        fill (``periodic``), the block exchange (``exchange``) or neither."""
+        fill (``periodic``) or the block exchange (``exchange``), then the
+        physical ghost fill (``physical``); none of them: ghosts stay u_n."""
         kp = (C.c_void_p * 4)(*[_ptr(x) for x in k])
-        mode = 2 if exchange else (1 if periodic else 0)
+        mode = (2 if exchange else (1 if periodic else 0)) + (4 if physical else 0)
         check(self.L.z4c_rk4_stage(self._ctx, C.byref(self.params), int(stage), float(dt), _ptr(u),
                                    _ptr(un), kp, _ptr(mat), mode, _stream(stream)),
               "rk4_stage")
 
-    def rk4_step(self, dt, u, un, k, mat=None, periodic=False, exchange=False, stream=None):
+    def rk4_step(self, dt, u, un, k, mat=None, periodic=False, exchange=False, physical=False,
+                 stream=None):
         """One bt_rk4 step in the GR_Z4c stage order (each stage: Z4cRHS,
         SumBT, the ghost update, AlgConstr, PrepareZ4cDerivatives)."""
         un.copy_(u)  # Butcher::PutScratchBT_u at stage 1
         for s in range(1, 5):
-            self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, stream)
+            self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, physical, stream)
 
     def rk4_step_fused(self, dt, u, work, mat=None, stream=None, graph=False):
         """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr

@@ -76,7 +76,11 @@ typedef struct {
   double x0[3];                /* coordinate of padded index 0 on each axis     */
   int level;                   /* refinement level (bookkeeping only)           */
   int sommerfeld_faces;        /* bit f set: face f (ix1,ox1,ix2,ox2,ix3,ox3)
-                                  carries gr_sommerfeld (Z4cBoundaryRHS)       */
+                                  carries gr_sommerfeld (Z4cBoundaryRHS and the
+                                  physical ghost fill)                          */
+  int outflow_faces;           /* bit f set: face f carries extrapolate_outflow
+                                  (physical ghost fill only)                    */
+  int pad_;
 } z4c_block;
 
 typedef struct z4c_ctx z4c_ctx;
@@ -186,15 +190,28 @@ int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
 int z4c_set_exchange(z4c_ctx* ctx, const double* lo, const double* hi, const int* periodic);
 int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
+/* Physical ghost fill (PHY_BVAL of GR_Z4c: CommRegistry::ApplyPhysicalBCs,
+   src/comm/comm_registry.cpp:516-527 -> comm::ApplyPhysicalBCs,
+   src/comm/physical_bcs.cpp:506-676) on every face flagged in
+   sommerfeld_faces | outflow_faces: the 5-point Lagrange extrapolation
+   ExtrapolateOutflowBC (:187-290) that gr_sommerfeld and
+   extrapolate_outflow faces of vertex-centred data get (:439-453), with the
+   reference's ranges: it indexes the vertex arrays with the cell-centred
+   is..ie = il..iu-1, so an outer face fills iu..iu+3 (the boundary vertex
+   included, the outermost ghost not), and x2 / x3 faces span the x1 / x1
+   and x2 ghosts.  Faces in order x1, x2, x3. */
+int z4c_physical_bc(z4c_ctx* ctx, int nvar, double* u, void* stream);
+
 /* One bt_rk4 stage of the GR_Z4c task list
    (src/task_list/gr/task_list_gr_z4c.cpp:45-89, stage s = 1..4) on all
    blocks: k[s-1] = RHS(u_s) (+ Sommerfeld), u_{s+1} = u_n + sum a dt k
    (SumBT_ak / SumBT_bk), optional periodic ghost fill, AlgConstr,
    PrepareZ4cDerivatives.  `un` holds u at the start of the step; `u` is
    overwritten with the next stage state; k[4] are device pointers (host
-   array).  Ghost update after SumBT: periodic = 0 none (ghosts stay u_n),
-   1 single-block periodic fill (z4c_periodic_fill), 2 the exchange set by
-   z4c_set_exchange. */
+   array).  Ghost update after SumBT, `periodic` = sum of: 1 single-block
+   periodic fill (z4c_periodic_fill) or 2 the exchange set by
+   z4c_set_exchange, and 4 the physical ghost fill (z4c_physical_bc);
+   0 = none (ghosts stay u_n). */
 int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
                   double* u, const double* un, double* const* k,
                   const double* mat, int periodic, void* stream);

@@ -137,6 +137,7 @@ def lib():
         L.oracle_new_dt.argtypes = [pg, d]
         L.oracle_periodic_fill.argtypes = [pg, C.c_int, pd]
         L.oracle_history.argtypes = [pp, pg, pd, pd, pd]
+        L.oracle_physical_bc.argtypes = [pg, C.c_int, pd, P(C.c_int)]
         L.oracle_exchange.argtypes = [C.c_int, pg, P(pd), C.c_int, pd, pd, P(C.c_int)]
     return _lib
 
@@ -238,6 +239,13 @@ def history(o, g, u, con, out=None):
     acc = (C.c_double * 8)(*(out if out is not None else [0.0] * 8))
     lib().oracle_history(C.byref(o), C.byref(g), _p(u), _p(con), acc)
     return list(acc)
+
+
+def physical_bc(g, u, faces):
+    """Physical ghost fill on the flagged faces (ix1, ox1, ix2, ox2, ix3, ox3)."""
+    f = (C.c_int * 6)(*[int(bool(v)) for v in faces])
+    lib().oracle_physical_bc(C.byref(g), u.shape[0], _p(u), f)
+    return u
 
 
 def exchange(geoms, us, lo, hi, periodic):

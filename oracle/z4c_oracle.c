@@ -1287,3 +1287,85 @@ void oracle_exchange(int nb, const oracle_geom* g, double* const* u, int nvar,
   for (int b = 0; b < nb; ++b) free(snap[b]);
   free(snap);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Physical ghost fill of a vertex-centred block (comm::ApplyPhysicalBCs,   */
+/* src/comm/physical_bcs.cpp:506-676, with ExtrapolateOutflowBC :187-290,   */
+/* which gr_sommerfeld and extrapolate_outflow faces of VC data use,        */
+/* :439-453).  NEXTRAPOLATE = nghost + 1 = 5 points (configure.py:891-895), */
+/* weights InterpolateLagrangeUniformBiasR<5> = {5,-10,10,-5,1}             */
+/* (src/utils/interp_univariate_stencils.tpp:26-28).  The reference indexes */
+/* with the cell-centred ranges is..ie = ng..ng+nx-1 (meshblock.cpp:874-    */
+/* 877) on the vertex arrays, so an outer face fills ie+1..ie+ng (the       */
+/* boundary vertex through the second-outermost ghost) and the transverse   */
+/* ranges stop at je / ke.  Faces in order x1, x2, x3; x2 faces span the x1 */
+/* ghosts, x3 faces the x1 and x2 ghosts.  faces: 6 flags ix1..ox3.         */
+/* ------------------------------------------------------------------------ */
+static const double EXTRAP5[5] = {5., -10., 10., -5., 1.};
+
+void oracle_physical_bc(const oracle_geom* g, int nvar, double* u, const int* faces) {
+  const int ng = g->ng;
+  const int is = g->il, ie = g->iu - 1, js = g->jl, je = g->ju - 1, ks = g->kl, ke = g->ku - 1;
+  const long s1 = 1, s2 = g->nn1, s3 = (long)g->nn1 * g->nn2;
+  const long N = s3 * g->nn3;
+  const int bis = is - ng, bie = ie + ng;
+  int bjs = faces[2] ? js : js - ng, bje = faces[3] ? je : je + ng;
+  const int bks = faces[4] ? ks : ks - ng, bke = faces[5] ? ke : ke + ng;
+  for (int n = 0; n < nvar; ++n) {
+    double* v = u + n * N;
+#define AT(i, j, k) v[(long)(i) * s1 + (long)(j) * s2 + (long)(k) * s3]
+    if (faces[0])
+      for (int k = bks; k <= bke; ++k)
+        for (int j = bjs; j <= bje; ++j)
+          for (int i = is - 1; i >= is - ng; --i) {
+            double x = 0.0;
+            for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i + p + 1, j, k);
+            AT(i, j, k) = x;
+          }
+    if (faces[1])
+      for (int k = bks; k <= bke; ++k)
+        for (int j = bjs; j <= bje; ++j)
+          for (int i = ie + 1; i <= ie + ng; ++i) {
+            double x = 0.0;
+            for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i - p - 1, j, k);
+            AT(i, j, k) = x;
+          }
+    if (faces[2])
+      for (int k = bks; k <= bke; ++k)
+        for (int j = js - 1; j >= js - ng; --j)
+          for (int i = bis; i <= bie; ++i) {
+            double x = 0.0;
+            for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i, j + p + 1, k);
+            AT(i, j, k) = x;
+          }
+    if (faces[3])
+      for (int k = bks; k <= bke; ++k)
+        for (int j = je + 1; j <= je + ng; ++j)
+          for (int i = bis; i <= bie; ++i) {
+            double x = 0.0;
+            for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i, j - p - 1, k);
+            AT(i, j, k) = x;
+          }
+    {
+      const int tjs = js - ng, tje = je + ng;  /* x3 faces span the x2 ghosts */
+      if (faces[4])
+        for (int k = ks - 1; k >= ks - ng; --k)
+          for (int j = tjs; j <= tje; ++j)
+            for (int i = bis; i <= bie; ++i) {
+              double x = 0.0;
+              for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i, j, k + p + 1);
+              AT(i, j, k) = x;
+            }
+      if (faces[5])
+        for (int k = ke + 1; k <= ke + ng; ++k)
+          for (int j = tjs; j <= tje; ++j)
+            for (int i = bis; i <= bie; ++i) {
+              double x = 0.0;
+              for (int p = 0; p < 5; ++p) x += EXTRAP5[p] * AT(i, j, k - p - 1);
+              AT(i, j, k) = x;
+            }
+    }
+#undef AT
+  }
+  (void)bjs;
+}

@@ -149,6 +149,11 @@ void oracle_history(const oracle_params* o, const oracle_geom* g,
 void oracle_exchange(int nb, const oracle_geom* g, double* const* u, int nvar,
                      const double* lo, const double* hi, const int* periodic);
 
+/* Physical ghost fill of one block on the flagged faces (6 flags,
+   ix1..ox3): 5-point extrapolation as the reference applies to VC data on
+   gr_sommerfeld / extrapolate_outflow faces, with its cell-centred ranges. */
+void oracle_physical_bc(const oracle_geom* g, int nvar, double* u, const int* faces);
+
 /* Single-block periodic vertex-centred ghost fill (all three axes). */
 void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u);
 

@@ -60,4 +60,4 @@ def test_binary_is_gfx950(tmp_path):
 def test_params_layout_matches_header():
     # 17 doubles + 4 ints
     assert ctypes.sizeof(_capi.Params) == 17 * 8 + 4 * 4
-    assert ctypes.sizeof(_capi.Block) == 8 + 9 * 4 + 4 + 6 * 8 + 2 * 4
+    assert ctypes.sizeof(_capi.Block) == 8 + 9 * 4 + 4 + 6 * 8 + 4 * 4

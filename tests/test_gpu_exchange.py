@@ -112,3 +112,48 @@ def test_split_gauge_wave_evolution_equals_single_block():
         num = np.abs(parts[b] - ref).reshape(22, -1).max(axis=1)
         den = np.maximum(np.abs(ref).reshape(22, -1).max(axis=1), 0.01)
         assert np.all(num / den <= 1e-10), (b, num / den)
+
+
+@pytest.mark.parametrize("faces,outflow", [(63, 0), (2 | 16, 0), (0, 1 | 4 | 8), (1, 32)])
+def test_physical_fill_matches_oracle(faces, outflow):
+    gs = [MeshBlockGeom.cube((6, 7, 5), 0.1, (0.2, -0.1, 0.3), faces=faces, outflow=outflow),
+          MeshBlockGeom.cube(4, 0.2, (3.0, 3.0, 3.0)),
+          MeshBlockGeom.cube(5, 0.1, (-2.0, 1.0, 0.5), faces=faces)]
+    rng = np.random.default_rng(41)
+    host = [rng.standard_normal((22,) + g.shape) for g in gs]
+    z = Z4c(gs, Params())
+    u = z.upload(host)
+    z.physical_bc(u)
+    torch.cuda.synchronize()
+    got = z.download(u, 22)
+    for b, g in enumerate(gs):
+        bits = g.sommerfeld_faces | g.outflow_faces
+        fl = [(bits >> f) & 1 for f in range(6)]
+        exp = orc.physical_bc(orc.Geom.block(g.nx, g.dx, g.xmin), np.ascontiguousarray(host[b]), fl)
+        assert np.array_equal(got[b], exp), b
+
+
+def test_sommerfeld_block_evolution_with_physical_fill():
+    """One block with gr_sommerfeld on every face: bt_rk4 steps with the
+    physical ghost fill after every stage, against the oracle's steps."""
+    g = MeshBlockGeom.cube(12, 1.0 / 12, (0.3, -0.45, 0.2), faces=63)
+    p = Params(lapse_oplog=2.0, shift_Gamma=1.0, shift_eta=2.0, diss=0.02, damp_kappa1=0.02,
+               chi_div_floor=1e-5)
+    from z4c_amd.initial_data import perturbed_flat
+    op = orc.Params(**p.as_dict())
+    og = orc.Geom.block(g.nx, g.dx, g.xmin)
+    u0 = orc.algconstr(op, og, np.ascontiguousarray(perturbed_flat(g, seed=9)))
+    z = Z4c([g], p)
+    u = z.upload([u0])
+    un, k = z.make_rk_registers()
+    dt = 0.25 / 12
+    exp = u0.copy()
+    for _ in range(2):
+        z.rk4_step(dt, u, un, k, physical=True)
+        exp, _ = orc.rk4_step(op, og, exp, dt,
+                              between_stages=lambda c: orc.physical_bc(og, c, [1] * 6))
+    torch.cuda.synchronize()
+    got = z.download(u, 22)[0]
+    num = np.abs(got - exp).reshape(22, -1).max(axis=1)
+    den = np.maximum(np.abs(exp).reshape(22, -1).max(axis=1), 1e-3)
+    assert np.all(num / den <= 1e-11), num / den

@@ -346,3 +346,60 @@ def test_exchange_brute_force_nonperiodic():
                     key = tuple(np.round(np.array([X[k, j, i], Y[k, j, i], Z[k, j, i]]) / h).astype(int))
                     exp = np.mean(owners[key]) if key in owners else before[b][0, k, j, i]
                     assert abs(us[b][0, k, j, i] - exp) <= 1e-15 * max(1.0, abs(exp)), (b, i, j, k)
+
+
+def _fill_set(g, faces):
+    """Vertices the reference's VC physical fill writes (cell-centred ranges
+    is..ie = il..iu-1 on the vertex arrays, physical_bcs.cpp:506-676)."""
+    ng = 4
+    is_, ie, js, je, ks, ke = g.il, g.iu - 1, g.jl, g.ju - 1, g.kl, g.ku - 1
+    bjs = js if faces[2] else js - ng
+    bje = je if faces[3] else je + ng
+    bks = ks if faces[4] else ks - ng
+    bke = ke if faces[5] else ke + ng
+    pts = set()
+    for k in range(bks, bke + 1):
+        for j in range(bjs, bje + 1):
+            if faces[0]:
+                pts |= {(i, j, k) for i in range(is_ - ng, is_)}
+            if faces[1]:
+                pts |= {(i, j, k) for i in range(ie + 1, ie + ng + 1)}
+    for k in range(bks, bke + 1):
+        for i in range(is_ - ng, ie + ng + 1):
+            if faces[2]:
+                pts |= {(i, j, k) for j in range(js - ng, js)}
+            if faces[3]:
+                pts |= {(i, j, k) for j in range(je + 1, je + ng + 1)}
+    for j in range(js - ng, je + ng + 1):
+        for i in range(is_ - ng, ie + ng + 1):
+            if faces[4]:
+                pts |= {(i, j, k) for k in range(ks - ng, ks)}
+            if faces[5]:
+                pts |= {(i, j, k) for k in range(ke + 1, ke + ng + 1)}
+    return pts
+
+
+@pytest.mark.parametrize("faces", [(1, 1, 1, 1, 1, 1), (0, 1, 0, 0, 1, 0), (1, 0, 1, 1, 0, 0)])
+def test_physical_bc_extrapolates_quartics_over_reference_ranges(faces):
+    """5-point extrapolation is exact for degree <= 4: a separable quartic
+    set everywhere except the fill set (garbage there) comes back whole on
+    the fill set; every other vertex keeps its value."""
+    g = orc.Geom.block((6, 7, 5), 0.1, (0.2, -0.1, 0.3))
+    X, Y, Z = g.mesh()
+    poly = (1 + X - 2 * X ** 2 + 0.5 * X ** 4) * (2 - Y + Y ** 3) * (1 + 0.3 * Z ** 2 - Z ** 4)
+    fill = _fill_set(g, faces)
+    u = np.stack([poly, 2 * poly])
+    rng = np.random.default_rng(5)
+    for (i, j, k) in fill:
+        u[:, k, j, i] = rng.standard_normal(2) * 1e3
+    before = u.copy()
+    orc.physical_bc(g, u, faces)
+    mask = np.zeros(g.shape, bool)
+    for (i, j, k) in fill:
+        mask[k, j, i] = True
+    # exact up to rounding lifted by the weights (sum |c| = 31 per layer,
+    # nested over up to four layers and three faces at the corners)
+    ref = np.stack([poly, 2 * poly])
+    np.testing.assert_allclose(u[:, mask], ref[:, mask], rtol=1e-6,
+                               atol=1e-6 * np.abs(ref).max())
+    np.testing.assert_array_equal(u[:, ~mask], before[:, ~mask])
