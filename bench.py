@@ -82,6 +82,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-perblock", action="store_true", help="skip the per-block launch comparison")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default stencil plan (no autotuner trial launches, e.g. under a profiler)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
@@ -171,9 +173,10 @@ def main():
     torch.cuda.synchronize()
     log("initial data ready")
     t_tune = time.perf_counter()
-    plan = z.autotune(u, reps=3)
-    log(f"stencil workgroup sizes by block size (autotuned in {time.perf_counter() - t_tune:.2f} s): "
-        + ", ".join(f"{'x'.join(map(str, k))}: {v}" for k, v in sorted(plan.items())))
+    plan = z.stencil_plan() if args.no_tune else z.autotune(u, reps=3)
+    log(f"stencil workgroup sizes by block size ("
+        + ("default plan" if args.no_tune else f"autotuned in {time.perf_counter() - t_tune:.2f} s")
+        + "): " + ", ".join(f"{'x'.join(map(str, k))}: {v}" for k, v in sorted(plan.items())))
 
     def step():
         z.rk4_step_fused(dt, u, work)
