@@ -1675,7 +1675,8 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
     B.pfaces = (in.sommerfeld_faces | in.outflow_faces) & 63;
-    // stencil configuration: the fewest thread slots (ties: lowest index)
+    // stencil configuration until tuned: 256 threads, unless another size
+    // needs at least a quarter fewer thread slots (idle lanes)
     {
       int best = -1;
       long long best_slots = 0;
@@ -1688,6 +1689,8 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
       }
       if (best < 0)
         return fail(Z4C_EINVAL, "set_blocks: no stencil tile fits block " + std::to_string(b));
+      const long long s256 = cfg_slots(0, B.ni, B.nj);
+      if (s256 > 0 && 4 * best_slots > 3 * s256) best = 0;
       blk_cfg.push_back(best);
     }
     for (int d = 0; d < 3; ++d) {

@@ -255,8 +255,10 @@ int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
    registered blocks, times the stencil pass of those blocks under each
    workgroup configuration (192, 256, 320 or 384 threads, each with the
    fewest x-y tiles that fit) on state `u`, `reps` launches each, and keeps
-   the fastest.  Until it is called a block uses the configuration with the
-   fewest thread slots.  Results do not depend on the configuration.
+   the fastest.  Until it is called a block uses the 256-thread
+   configuration, or one that needs at least a quarter fewer
+   thread slots (tiles x threads).  Results do not depend on the
+   configuration.
    Waits for the stream. */
 int z4c_autotune(z4c_ctx* ctx, const z4c_params* p, const double* u, int reps, void* stream);
 /* Workgroup size the stencil pass uses for `block`; setting it (block -1:

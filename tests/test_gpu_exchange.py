@@ -151,8 +151,18 @@ def test_sommerfeld_block_evolution_with_physical_fill():
     exp = u0.copy()
     for _ in range(2):
         z.rk4_step(dt, u, un, k, physical=True)
-        exp, _ = orc.rk4_step(op, og, exp, dt,
-                              between_stages=lambda c: orc.physical_bc(og, c, [1] * 6))
+        # the oracle's stage sequence with the Sommerfeld face RHS
+        # (Z4cBoundaryRHS after Z4cRHS) and the physical fill after SumBT
+        n0, ks, cur = exp.copy(), [], exp
+        for st in range(4):
+            kk = orc.rhs(op, og, cur)
+            orc.boundary_rhs(op, og, cur, [1] * 6, kk)
+            ks.append(kk)
+            w = orc.BT_RK4_A[st + 1][:st + 1] if st < 3 else orc.BT_RK4_B
+            cur = orc.bt_sum(og, n0, ks, w, dt)
+            orc.physical_bc(og, cur, [1] * 6)
+            orc.algconstr(op, og, cur)
+        exp = cur
     torch.cuda.synchronize()
     got = z.download(u, 22)[0]
     num = np.abs(got - exp).reshape(22, -1).max(axis=1)
