@@ -71,31 +71,40 @@ struct DevBlock {
 // ---------------------------------------------------------------------------
 // Derivative slots per interior point, written by the stencil pass and read
 // by the algebra pass (field stride = interior points of the block).
-//   FULL fields (chi, g~_ij, alpha, beta^i): d_a (3), d_a d_b (6, SYM2
-//        order; off-diagonal = nested Dx(a, Dx(b, .)) with a < b), L, D
+//   FULL fields (chi, alpha, beta^i): d_a (3), d_a d_b (6, SYM2 order;
+//        off-diagonal = nested Dx(a, Dx(b, .)) with a < b), L, D
+//   METRIC fields (g~_ab): d_a (3), P_ab, L, D with
+//        P_ab = sum_{c<=d} w_cd g~^cd d_c d_d g~_ab  (w = 1/2 on the
+//        diagonal, 1 off it; SYM2 order), the second-derivative part of the
+//        Ricci tensor (calculate_z4c_rhs.cpp:270-286), formed where the
+//        second derivatives are
 //   GRAD fields (Khat, Gam~^i, Theta):       d_a (3), L, D
 //   ADV  fields (A~_ij):                      L, D
 // L = sum_a Lx(a, beta^a, .) in a order; D = (Diss_x + Diss_y) + Diss_z.
 // ---------------------------------------------------------------------------
-enum : int { DK_FULL = 0, DK_GRAD = 1, DK_ADV = 2 };
+enum : int { DK_FULL = 0, DK_GRAD = 1, DK_ADV = 2, DK_METRIC = 3 };
 __host__ __device__ constexpr int dkind(int n) {
-  return (n == V_CHI || (n >= V_GXX && n < V_GXX + 6) || n == V_ALPHA || n >= V_BETAX)
-             ? DK_FULL
-             : ((n == V_KHAT || n == V_THETA || (n >= V_GAMX && n < V_GAMX + 3)) ? DK_GRAD
-                                                                                 : DK_ADV);
+  return (n >= V_GXX && n < V_GXX + 6)
+             ? DK_METRIC
+             : ((n == V_CHI || n == V_ALPHA || n >= V_BETAX)
+                    ? DK_FULL
+                    : ((n == V_KHAT || n == V_THETA || (n >= V_GAMX && n < V_GAMX + 3))
+                           ? DK_GRAD
+                           : DK_ADV));
 }
 __host__ __device__ constexpr int dnslots(int kind) {
-  return kind == DK_FULL ? 11 : (kind == DK_GRAD ? 5 : 2);
+  return kind == DK_FULL ? 11 : (kind == DK_METRIC ? 6 : (kind == DK_GRAD ? 5 : 2));
 }
 __host__ __device__ constexpr int dbase(int n) {
   int s = 0;
   for (int m = 0; m < n; ++m) s += dnslots(dkind(m));
   return s;
 }
-constexpr int NDV = 158;
+constexpr int NDV = 128;
 static_assert(dbase(NVAR) == NDV, "derivative slot layout");
 __host__ __device__ constexpr int s_d(int n, int a) { return dbase(n) + a; }
 __host__ __device__ constexpr int s_dd(int n, int ab) { return dbase(n) + 3 + ab; }
+__host__ __device__ constexpr int s_lap(int n) { return dbase(n) + 3; }
 __host__ __device__ constexpr int s_L(int n) { return dbase(n) + dnslots(dkind(n)) - 2; }
 __host__ __device__ constexpr int s_D(int n) { return dbase(n) + dnslots(dkind(n)) - 1; }
 

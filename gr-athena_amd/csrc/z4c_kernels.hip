@@ -214,6 +214,7 @@ __global__ void __launch_bounds__(Cfg::THREADS) k_stencil(StencilArgs a) {
   const int f = t - tile * NVAR;
   const PlaneTile T = a.tiles[tile];
   switch (dkind(f)) {
+    case DK_METRIC: stencil_tile<Cfg, DK_METRIC>(a, T, f, sm); break;
     case DK_FULL: stencil_tile<Cfg, DK_FULL>(a, T, f, sm); break;
     case DK_GRAD: stencil_tile<Cfg, DK_GRAD>(a, T, f, sm); break;
     default: stencil_tile<Cfg, DK_ADV>(a, T, f, sm); break;

@@ -93,11 +93,8 @@ __device__ __forceinline__ void rhs_curvature_point(const RhsParams& o, const Po
         r += 0.5 * (gdd[sym(c, a)] * X.d(s_d(V_GAMX + c, b)) +
                     gdd[sym(c, b)] * X.d(s_d(V_GAMX + c, a)) +
                     Gu[c] * (Gddd[a][sym(b, c)] + Gddd[b][sym(a, c)]));
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int d = c; d < 3; ++d)
-          r -= ((c == d) ? 0.5 : 1.0) * guu[sym(c, d)] * X.d(s_dd(V_GXX + sym(a, b), sym(c, d)));
+      // sum_{c<=d} w_cd g~^cd d_c d_d g~_ab, formed by the stencil pass
+      r -= X.d(s_lap(V_GXX + sym(a, b)));
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll

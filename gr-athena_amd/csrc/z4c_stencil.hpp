@@ -75,7 +75,8 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   constexpr int ST_CUR = Cfg::CUR;
   constexpr int ST_EXT = Cfg::EXT;
   constexpr int ST_BUF = Cfg::BUF;
-  constexpr bool full = KIND == DK_FULL;
+  constexpr bool metric = KIND == DK_METRIC;
+  constexpr bool full = KIND == DK_FULL || metric;  // second derivatives
   constexpr bool grad = KIND != DK_ADV;
   const DevBlock B = a.blocks[T.block];
   const int N = B.npts;
@@ -84,6 +85,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   // offsets in SGPRs
   const Buf U = make_buf(a.u + B.off * NVAR + (long long)f * N, (long long)N * 8);
   const Buf BETA = make_buf(a.u + B.off * NVAR + (long long)V_BETAX * N, 3LL * N * 8);
+  const Buf GM = make_buf(a.u + B.off * NVAR + (long long)V_GXX * N, 6LL * N * 8);
   const Buf DVB = make_buf(a.dv + B.doff, (long long)NDV * B.nis * 8);
   const int NS8 = B.nis * 8;
   const int db = dbase(f), sl = s_L(f), sd = s_D(f);
@@ -131,6 +133,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   // registers: next plane, z rings (zr: z-4..z+4 at the point; er: z-3..z+3)
   double pc[ST_CUR], zr[9], er[ST_EXT][7], pz, pe[ST_EXT], pb[3];
   double bx, by, bz;  // shift at the point, current plane
+  double pg[6], guu[6];  // metric fields: g~_ab at the point (next plane), g~^ab (current)
 
   // Places a loaded plane (pc, pz, pe, pb) into LDS buffer `buf` and the
   // rings.  The loop is rotated so that loads are issued at the top of an
@@ -144,6 +147,10 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
     bx = pb[0];
     by = pb[1];
     bz = pb[2];
+    if (metric) {
+      const double detg = det3(pg);
+      inv3(1.0 / detg, pg, guu);
+    }
 #pragma unroll
     for (int r = 0; r < ST_CUR; ++r) cur[tid + r * ST_THREADS] = pc[r];
     if (full) {
@@ -167,6 +174,10 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
     if (full) {
 #pragma unroll
       for (int r = 0; r < ST_EXT; ++r) pe[r] = bload(U, eofs[r], 8 * sz * (k1 + 3));
+    }
+    if (metric) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) pg[c] = bload(GM, pxy8, 8 * (c * N + sz * k1));
     }
   };
 
@@ -210,12 +221,13 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
     // one direction at a time (x, y, then z from the ring), so that a
     // single 9-point array is live; L and D accumulate in x, y, z order
     double L = 0.0, D;
+    double dd[6];  // second derivatives, SYM2 order
     {
       double xv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) xv[m] = cur[o + (m - 4)];
       if (grad) bstore(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
-      if (full) bstore(DVB, st_dxx(xv, B.cidx2[0]), q8, (db + 3 + 0) * NS8);
+      if (full) dd[0] = st_dxx(xv, B.cidx2[0]);
       L += st_lx(xv, B.lidx[0], bx);
       D = st_diss(xv, B.cidxd[0], a.diss);
     }
@@ -224,12 +236,12 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
 #pragma unroll
       for (int m = 0; m < 9; ++m) yv[m] = cur[o + (m - 4) * PX];
       if (grad) bstore(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
-      if (full) bstore(DVB, st_dxx(yv, B.cidx2[1]), q8, (db + 3 + 3) * NS8);
+      if (full) dd[3] = st_dxx(yv, B.cidx2[1]);
       L += st_lx(yv, B.lidx[1], by);
       D += st_diss(yv, B.cidxd[1], a.diss);
     }
     if (grad) bstore(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
-    if (full) bstore(DVB, st_dxx(zr, B.cidx2[2]), q8, (db + 3 + 5) * NS8);
+    if (full) dd[5] = st_dxx(zr, B.cidx2[2]);
     // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
     L += st_lx(zr, B.lidx[2], bz);
     bstore(DVB, L, q8, sl * NS8);
@@ -240,13 +252,26 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       v[0] = v[8] = 0.0;
 #pragma unroll
       for (int m = 1; m < 8; ++m) v[m] = dyp[w + (m - 4)];
-      bstore(DVB, st_dx(v, B.cidx1[0]), q8, (db + 3 + 1) * NS8);  // d_x(d_y u)
+      dd[1] = st_dx(v, B.cidx1[0]);  // d_x(d_y u)
 #pragma unroll
       for (int m = 1; m < 8; ++m) v[m] = dzp[w + (m - 4)];
-      bstore(DVB, st_dx(v, B.cidx1[0]), q8, (db + 3 + 2) * NS8);  // d_x(d_z u)
+      dd[2] = st_dx(v, B.cidx1[0]);  // d_x(d_z u)
 #pragma unroll
       for (int m = 1; m < 8; ++m) v[m] = dzp[w + (m - 4) * EX];
-      bstore(DVB, st_dx(v, B.cidx1[1]), q8, (db + 3 + 4) * NS8);  // d_y(d_z u)
+      dd[4] = st_dx(v, B.cidx1[1]);  // d_y(d_z u)
+      if (metric) {
+        // sum_{c<=d} w_cd g~^cd d_c d_d g~_ab in the reference's c, d order
+        double P = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = c; d < 3; ++d)
+            P += ((c == d) ? 0.5 : 1.0) * guu[sym(c, d)] * dd[sym(c, d)];
+        bstore(DVB, P, q8, (db + 3) * NS8);
+      } else {
+#pragma unroll
+        for (int ab = 0; ab < 6; ++ab) bstore(DVB, dd[ab], q8, (db + 3 + ab) * NS8);
+      }
     }
     q8 += qstep8;
 #pragma unroll
