@@ -28,6 +28,7 @@ import numpy as np
 
 from ._capi import Params
 from .initial_data import gauge_wave_adm
+from .outputs import write_hst
 from .z4c import MeshBlockGeom, Z4c
 
 GAUGES = {
@@ -97,17 +98,6 @@ def run_gauge_wave(n=64, t_end=10.0, cfl=0.25, amplitude=0.01, d=1.0, gauge="har
     if hst_path:
         write_hst(hst_path, times, rows, dt)
     return np.array(times), np.array(rows), z.download(u, 22)[0], geom
-
-
-def write_hst(path, times, rows, dt):
-    """History file in the reference's layout (history.cpp:385-445): header
-    lines, then time, dt and the Z4c columns, %.18g."""
-    cols = ["time", "dt"] + Z4c.HISTORY_COLUMNS
-    with open(path, "w") as f:
-        f.write("# Athena++ history data\n")
-        f.write("# " + " ".join(f"[{i + 1}]={c}" for i, c in enumerate(cols)) + " \n")
-        for t, r in zip(times, rows):
-            f.write(" ".join("%.18g" % v for v in [t, dt] + list(r)) + "\n")
 
 
 def exact_lapse(geom, t, amplitude=0.01, d=1.0):
