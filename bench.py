@@ -56,9 +56,11 @@ PASS_BYTES = {
 
 
 def measured_traffic(kernel):
-    """HBM bytes per launch of ``kernel`` from the committed counter summary
-    (the block between 'traffic-bytes-per-launch' markers: kernel, read
-    bytes, write bytes), or None."""
+    """HBM bytes per pass launch of ``kernel`` from the committed counter
+    summary (the block between 'traffic-bytes-per-launch' markers: kernel,
+    read bytes, write bytes per dispatch), or None.  A pass that dispatches
+    one kernel instance per workgroup configuration (the stencil pass) lists
+    one line per instance; the pass's traffic is their sum."""
     try:
         text = open(TRAFFIC_FILE).read()
     except OSError:
@@ -66,11 +68,13 @@ def measured_traffic(kernel):
     if "traffic-bytes-per-launch" not in text:
         return None
     body = text.split("traffic-bytes-per-launch", 1)[1].split("-->", 1)[0]
+    total, found = 0.0, False
     for line in body.splitlines():
         f = line.split()
         if len(f) == 3 and f[0] == kernel:
-            return float(f[1]) + float(f[2])
-    return None
+            total += float(f[1]) + float(f[2])
+            found = True
+    return total if found else None
 
 
 def parse():
