@@ -88,6 +88,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default stencil plan (no autotuner trial launches, e.g. under a profiler)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
@@ -246,9 +247,75 @@ def main():
     if not args.no_perblock:
         out["per_block_launch"] = per_block(z, blocks, params, u, dt, owned)
         out["batched_speedup_vs_per_block"] = rate / out["per_block_launch"]["value"]
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_cost(params)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def adjoint_cost(params, nw=64, reps=5):
+    """Adjoint cost on one MeshBlock of 32^3 cells (33^3 interior vertices,
+    41^3 padded) holding the perturbed flat state (8 Gaussian bumps per
+    variable, amplitudes 1e-6..1e-3, AlgConstr applied) with ``nw`` seeded
+    Gaussian cotangents over all 22 variables, timed with events on the
+    launch stream (average of ``reps`` calls after one warmup call):
+      * one forward Z4cRHS of the block, and the same RHS batched over nw
+        copies of the block (one launch per kernel);
+      * z4c_rhs_vjp with the nw cotangents in one call;
+      * one fused bt_rk4 step and z4c_rk4_step_vjp with one cotangent."""
+    import torch
+    from z4c_amd.initial_data import perturbed_flat
+    from z4c_amd.z4c import MeshBlockGeom, Z4c
+    g = MeshBlockGeom.cube(32, 1.0 / 32, (-0.5, -0.5, -0.5))
+    z = Z4c([g], params)
+    u = z.upload([perturbed_flat(g, seed=0)])
+    z.AlgConstr(u)
+    gen = torch.Generator(device=u.device).manual_seed(1)
+    w = torch.randn(nw * u.numel(), dtype=torch.float64, device=u.device, generator=gen)
+    rhs = z.zeros()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    t_rhs = timed(lambda: z.Z4cRHS(u, None, rhs))
+    t_vjp = timed(lambda: z.rhs_vjp(u, w, nw))
+    zb = Z4c([g] * nw, params)
+    ub = u.repeat(nw)
+    rb = zb.zeros()
+    t_rhs_b = timed(lambda: zb.Z4cRHS(ub, None, rb))
+    zb.close()
+    del ub, rb
+    dt = 0.25 / 32
+    work = z.make_step_work()
+    us = u.clone()
+    t_step = timed(lambda: z.rk4_step_fused(dt, us, work))
+    t_step_vjp = timed(lambda: z.rk4_step_vjp(dt, u, w[:u.numel()], 1))
+    z.close()
+    res = {
+        "workload": f"one 32^3-cell block (33^3 interior vertices), perturbed flat state, "
+                    f"{nw} Gaussian cotangents over 22 variables",
+        "rhs_ms": t_rhs, "rhs_vjp_ms": t_vjp, "nw": nw,
+        "rhs_vjp_ms_per_cotangent": t_vjp / nw,
+        "vjp_per_cotangent_over_rhs": t_vjp / nw / t_rhs,
+        "rhs_batched_ms": t_rhs_b,
+        "vjp_over_batched_rhs": t_vjp / t_rhs_b,
+        "step_ms": t_step, "step_vjp_ms": t_step_vjp,
+        "step_vjp_over_step": t_step_vjp / t_step,
+    }
+    log(f"adjoint: rhs {t_rhs:.3f} ms, rhs_vjp x{nw} {t_vjp:.3f} ms "
+        f"({res['vjp_per_cotangent_over_rhs']:.2f}x rhs per cotangent, "
+        f"{res['vjp_over_batched_rhs']:.2f}x the batched rhs), step {t_step:.3f} ms, "
+        f"step_vjp {t_step_vjp:.3f} ms ({res['step_vjp_over_step']:.2f}x)")
+    return res
 
 
 def per_block(z, blocks, params, u, dt, owned):

@@ -319,14 +319,14 @@ __device__ __forceinline__ double st_lx(const double* v, double lidx, double vx)
 // ---------------------------------------------------------------------------
 // 3x3 symmetric algebra (SYM2 order)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double det3(const double* g) {
+__host__ __device__ __forceinline__ double det3(const double* g) {
   const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
                gzz = g[5];
   return (-(gxz * gxz) * gyy + 2 * gxy * gxz * gyz - gxx * (gyz * gyz) -
           (gxy * gxy) * gzz + gxx * gyy * gzz);
 }
 
-__device__ __forceinline__ void inv3(double oodetg, const double* g, double* u) {
+__host__ __device__ __forceinline__ void inv3(double oodetg, const double* g, double* u) {
   const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
                gzz = g[5];
   u[0] = (-(gyz * gyz) + gyy * gzz) * oodetg;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void inv3(double oodetg, const double* g, double* u) 
   u[5] = (-(gxy * gxy) + gxx * gyy) * oodetg;
 }
 
-__device__ __forceinline__ double trace_rank2(double detginv, const double* g,
+__host__ __device__ __forceinline__ double trace_rank2(double detginv, const double* g,
                                               const double* A) {
   const double gxx = g[0], gxy = g[1], gxz = g[2], gyy = g[3], gyz = g[4],
                gzz = g[5];
@@ -350,11 +350,11 @@ __device__ __forceinline__ double trace_rank2(double detginv, const double* g,
                     Azz * (gxy * gxy) - Ayy * (gxz * gxz) - Axx * (gyz * gyz));
 }
 
-__device__ __forceinline__ double dmax(double a, double b) {
+__host__ __device__ __forceinline__ double dmax(double a, double b) {
   return (a < b) ? b : a;  // std::max semantics
 }
 
-__device__ __forceinline__ double regularize_nth_root(double value,
+__host__ __device__ __forceinline__ double regularize_nth_root(double value,
                                                       double value_reg,
                                                       double eps_floor,
                                                       double npow) {
@@ -362,7 +362,7 @@ __device__ __forceinline__ double regularize_nth_root(double value,
   return (eps < eps_floor) ? (1.0 + eps_floor / npow) : pow(value, 1.0 / npow);
 }
 
-__device__ __forceinline__ double regularize_near_zero(double v, double e) {
+__host__ __device__ __forceinline__ double regularize_near_zero(double v, double e) {
   return (v >= 0) ? ((v < e) ? e : v) : ((v > -e) ? -e : v);
 }
 

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "z4c_adjoint.hpp"
 #include "z4c_device.hpp"
 #include "z4c_mi355x.h"
 #include "z4c_rhs_point.hpp"
@@ -1174,6 +1175,232 @@ __global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
     a.u[B.off * a.nvar + (long long)n * B.npts + p] = cnt > 1 ? acc * inv : acc;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Adjoint of the right-hand side (no counterpart in the reference, which has
+// no derivative code): linearisation, pointwise reverse mode, transposed
+// stencils, parameter-gradient reduction.  Cotangent c of nw is blockIdx.y
+// (times NVAR fields in the stencil kernel).
+// ---------------------------------------------------------------------------
+struct AdjArgs {
+  const DevBlock* blocks;
+  Launch L;
+  RhsParams o;
+  const double* u;   // state the right-hand side is linearised at
+  double* lin;       // NLINF per interior vertex
+  const double* w;   // nw cotangent registers (NVAR * total_pts each)
+  double* ubar;      // nw output registers
+  double* sb;        // nw derivative-cotangent sets (NADJ * total_int each)
+  double* part;      // nw * gridDim.x * NPBAR workgroup partial sums
+  long long reg;     // NVAR * total_pts
+  long long sreg;    // NADJ * total_int
+};
+
+__global__ void __launch_bounds__(TILE) k_adj_lin(AdjArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  adj_lin_at(a.o, B, a.u + B.off * NVAR, a.lin + B.ioff * NLINF, i, j, k);
+}
+
+// Buffer accessors of the pointwise adjoint: one 32-bit lane offset per
+// buffer, the slot / field offsets uniform (SGPR), so the hundreds of slot
+// addresses cost no vector registers.
+struct AdjXBuf {
+  Buf U, L;
+  unsigned vp, vq;  // p*8, q*8
+  int fu, fl;       // npts*8, nint*8
+  __device__ double u(int n) const { return bload(U, vp, n * fu); }
+  __device__ double d1(int n, int a) const { return bload(L, vq, l_d1(n, a) * fl); }
+  __device__ double d2(int n, int ab) const { return bload(L, vq, l_d2(n, ab) * fl); }
+  __device__ double ls(int n, int a) const { return bload(L, vq, l_ls(n, a) * fl); }
+};
+struct AdjFBuf {
+  Buf L;
+  unsigned vq;
+  int fl;
+  __device__ double operator()(int s) const { return bload(L, vq, (NLIN + s) * fl); }
+};
+struct AdjSBuf {
+  Buf S;
+  unsigned vq;
+  int fl;
+  __device__ void d1(int n, int a, double v) const { bstore(S, v, vq, l_d1(n, a) * fl); }
+  __device__ void d2(int n, int ab, double v) const { bstore(S, v, vq, l_d2(n, ab) * fl); }
+};
+
+__global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
+  __shared__ double red[NPBAR][TILE / 64];
+  __shared__ double shG[42 * TILE];  // per-thread Christoffel adjoints
+  const long long c = blockIdx.y;
+  double pb[NPBAR] = {0.0, 0.0, 0.0};
+  int b, i, j, k;
+  if (tile_point(a.L, b, i, j, k)) {
+    const DevBlock B = a.blocks[b];
+    const long long N = B.npts, nint = B.nint;
+    const unsigned vp = (unsigned)flat(B, i, j, k) * 8u, vq = (unsigned)iidx(B, i, j, k) * 8u;
+    const AdjXBuf X{make_buf(a.u + B.off * NVAR, NVAR * N * 8),
+                    make_buf(a.lin + B.ioff * NLINF, NLINF * nint * 8), vp, vq, (int)(N * 8),
+                    (int)(nint * 8)};
+    const AdjFBuf F{X.L, vq, (int)(nint * 8)};
+    const AdjSBuf S{make_buf(a.sb + c * a.sreg + B.ioff * NADJ, NADJ * nint * 8), vq,
+                    (int)(nint * 8)};
+    const Buf W = make_buf(a.w + c * a.reg + B.off * NVAR, NVAR * N * 8);
+    double wp[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) wp[n] = bload(W, vp, n * X.fu);
+    const Buf UB = make_buf(a.ubar + c * a.reg + B.off * NVAR, NVAR * N * 8);
+    const int fu = X.fu;
+    rhs_vjp_point(a.o, X, F, wp, [&](int n, double v) { bstore(UB, v, vp, n * fu); }, S, pb,
+                  shG + threadIdx.x, TILE);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NPBAR; ++t) {
+    double v = pb[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[t][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NPBAR) {
+    double v = 0.0;
+    for (int q = 0; q < TILE / 64; ++q) v += red[threadIdx.x][q];
+    a.part[(c * gridDim.x + blockIdx.x) * NPBAR + threadIdx.x] = v;
+  }
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_stencil(AdjArgs a) {
+  const long long c = blockIdx.y / NVAR;
+  const int n = blockIdx.y % NVAR;
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  double* ub = a.ubar + c * a.reg + B.off * NVAR + (long long)n * B.npts + flat(B, i, j, k);
+  const double pt = is_interior(B, i, j, k) ? *ub : 0.0;
+  *ub = pt + adj_stencil_at(a.o, B, a.u + B.off * NVAR, a.w + c * a.reg + B.off * NVAR,
+                            a.sb + c * a.sreg + B.ioff * NADJ, n, i, j, k);
+}
+
+// pbar[c*NPBAR + t] (+)= sum of the partials of cotangent c, in order
+__global__ void k_adj_pbar(const double* part, int G, int nw, double* pbar, int accumulate) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nw * NPBAR) return;
+  const int c = e / NPBAR, t = e % NPBAR;
+  double v = 0.0;
+  for (int g = 0; g < G; ++g) v += part[((long long)c * G + g) * NPBAR + t];
+  pbar[e] = accumulate ? pbar[e] + v : v;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of one bt_rk4 step with ghosts held at u_n and AlgConstr after
+// every stage (the fused step's semantics):
+//   u_1 = u_n, u_s = AlgConstr(u_n + a_s dt k_{s-1}) (s = 2..4),
+//   k_s = rhs(u_s), u_{n+1} = AlgConstr(u_n + dt sum_s b_s k_s),
+// with k_s = 0 at ghosts (so ghosts of u_s and u_{n+1} are AlgConstr(u_n)).
+// ---------------------------------------------------------------------------
+struct AdjStepArgs {
+  const DevBlock* blocks;
+  Launch L;              // BOX_WHOLE
+  const double* un;      // u_n
+  const double* k[4];    // stage right-hand sides (zero at ghosts)
+  const double* bar;     // final: cotangent of u_{n+1}; stage: cotangent of u_s
+  double* ubar_in;       // cotangent of u_n (accumulated)
+  double* ybar;          // AlgConstr cotangent at the final sum (interior)
+  double* kbar;          // cotangent of k_{s-1} (interior), the next stage's w
+  long long reg;
+  double dt;
+  int stage;             // 5 = final sum, 4..1 = stage input u_s
+  double eps_floor;
+  int tf_force, force_reg;
+};
+
+__device__ __forceinline__ void adj_alg_point(const AdjStepArgs& a, const double* un, long long p,
+                                              long long N, bool interior, double wk, int kreg,
+                                              double* bar) {
+  double g[6], A[6], bg[6], bA[6];
+  double chi = un[V_CHI * N + p], alpha = un[V_ALPHA * N + p];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    g[ab] = un[(V_GXX + ab) * N + p];
+    A[ab] = un[(V_AXX + ab) * N + p];
+  }
+  if (interior) {
+    // the forward state entering AlgConstr: u_n + w k (one register) or the
+    // final sum ((u_n + b1 dt k1) + b2 dt k2 + b3 dt k3) + b4 dt k4
+    if (kreg >= 0) {
+      const double* kk = a.k[kreg];
+      chi = chi + wk * kk[V_CHI * N + p];
+      alpha = alpha + wk * kk[V_ALPHA * N + p];
+#pragma unroll
+      for (int ab = 0; ab < 6; ++ab) {
+        g[ab] = g[ab] + wk * kk[(V_GXX + ab) * N + p];
+        A[ab] = A[ab] + wk * kk[(V_AXX + ab) * N + p];
+      }
+    } else {
+      const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double ws = bw[s] * a.dt;
+        const double* kk = a.k[s];
+        chi = chi + ws * kk[V_CHI * N + p];
+        alpha = alpha + ws * kk[V_ALPHA * N + p];
+#pragma unroll
+        for (int ab = 0; ab < 6; ++ab) {
+          g[ab] = g[ab] + ws * kk[(V_GXX + ab) * N + p];
+          A[ab] = A[ab] + ws * kk[(V_AXX + ab) * N + p];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    bg[ab] = bar[V_GXX + ab];
+    bA[ab] = bar[V_AXX + ab];
+  }
+  algconstr_vjp(g, A, chi, alpha, bg, bA, bar[V_CHI], bar[V_ALPHA], a.eps_floor, a.tf_force,
+                a.force_reg);
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    bar[V_GXX + ab] = bg[ab];
+    bar[V_AXX + ab] = bA[ab];
+  }
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_step(AdjStepArgs a) {
+  const long long c = blockIdx.y;
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const long long N = B.npts, p = flat(B, i, j, k);
+  const bool interior = is_interior(B, i, j, k);
+  const double* un = a.un + B.off * NVAR;
+  const long long cb = c * a.reg + B.off * NVAR;
+  double bar[NVAR];
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) bar[n] = a.bar[cb + n * N + p];
+  const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  const double aw[5] = {0., 0., 0.5, 0.5, 1.0};  // a_s of stage s
+  if (a.stage == 5) {
+    adj_alg_point(a, un, p, N, interior, 0.0, -1, bar);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+      a.ubar_in[cb + n * N + p] = bar[n];
+      if (interior) {
+        a.ybar[cb + n * N + p] = bar[n];
+        a.kbar[cb + n * N + p] = (bw[3] * a.dt) * bar[n];
+      }
+    }
+    return;
+  }
+  const int s = a.stage;
+  if (s >= 2) adj_alg_point(a, un, p, N, interior, aw[s] * a.dt, s - 2, bar);
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) {
+    a.ubar_in[cb + n * N + p] += bar[n];
+    if (s >= 2 && interior)
+      a.kbar[cb + n * N + p] = (bw[s - 2] * a.dt) * a.ybar[cb + n * N + p] + (aw[s] * a.dt) * bar[n];
+  }
+}
 }  // namespace
 
 // ===========================================================================
@@ -1224,6 +1451,17 @@ struct z4c_ctx {
   };
   bool timing = false;
   std::vector<TimedLaunch> tev;
+  // adjoint scratch (lazy): linearisation, derivative cotangents, partial
+  // sums, the step's stage right-hand sides and stage state, and three
+  // per-cotangent registers (ybar, kbar, stage cotangent)
+  double* adj_lin = nullptr;
+  double* adj_sb = nullptr;
+  double* adj_part = nullptr;
+  double* adj_k = nullptr;
+  double* adj_stage = nullptr;
+  double* adj_reg = nullptr;
+  long long adj_lin_n = 0, adj_sb_n = 0, adj_part_n = 0, adj_k_n = 0, adj_stage_n = 0,
+            adj_reg_n = 0;
 
   void drop_timing() {
     for (auto& t : tev) {
@@ -1272,6 +1510,12 @@ struct z4c_ctx {
     ex_dst = ex_off = ex_src = nullptr;
     ex_ndst = 0;
     hst_host.clear();
+    double** adj[6] = {&adj_lin, &adj_sb, &adj_part, &adj_k, &adj_stage, &adj_reg};
+    for (double** q : adj) {
+      hipFree(*q);
+      *q = nullptr;
+    }
+    adj_lin_n = adj_sb_n = adj_part_n = adj_k_n = adj_stage_n = adj_reg_n = 0;
     d_blocks = nullptr;
     d1 = nullptr;
     dv = nullptr;
@@ -1596,6 +1840,116 @@ int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   ExArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ex_src, c->ex_ndst, nvar, u, c->snap};
   hipLaunchKernelGGL(k_exchange, dim3((c->ex_ndst + 255) / 256), dim3(256), 0, s, a);
   HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int ensure_buf(double*& p, long long& have, long long need, const char* what) {
+  if (have >= need) return Z4C_OK;
+  hipFree(p);
+  p = nullptr;
+  have = 0;
+  if (hipMalloc(&p, sizeof(double) * (size_t)need) != hipSuccess)
+    return fail(Z4C_ENOMEM, std::string(what) + ": scratch allocation failed");
+  have = need;
+  return Z4C_OK;
+}
+
+constexpr int MAX_NW = 65535 / NVAR;  // grid.y of the transposed-stencil kernel
+
+int check_vjp_args(const z4c_ctx* c, const z4c_params* p, int nw, const char* what) {
+  if (nw < 1 || nw > MAX_NW)
+    return fail(Z4C_EINVAL, std::string(what) + ": nw must be in [1, " + std::to_string(MAX_NW) + "]");
+  for (const DevBlock& B : c->hdev)
+    if ((long long)NLINF * B.nint * 8 > 0x7fffffffLL || (long long)NVAR * B.npts * 8 > 0x7fffffffLL)
+      return fail(Z4C_EINVAL, std::string(what) + ": block too large for the adjoint's 32-bit buffer offsets");
+  if (p->cowling) return fail(Z4C_EINVAL, std::string(what) + ": the Cowling right-hand side has no adjoint here");
+  return Z4C_OK;
+}
+
+// ubar_c = (d rhs / d u)^T w_c for the nw cotangents; pbar (+)= parameter gradients
+int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const double* w,
+                   double* ubar, double* pbar, int accumulate, hipStream_t s) {
+  const long long reg = c->total_pts * NVAR, sreg = c->total_int * NADJ;
+  const int Gi = grid_for(c->ntiles[BOX_INTERIOR]);
+  int rc = ensure_buf(c->adj_lin, c->adj_lin_n, c->total_int * NLINF, "rhs_vjp");
+  if (!rc) rc = ensure_buf(c->adj_sb, c->adj_sb_n, (long long)nw * sreg, "rhs_vjp");
+  if (!rc) rc = ensure_buf(c->adj_part, c->adj_part_n, (long long)nw * Gi * NPBAR, "rhs_vjp");
+  if (rc) return rc;
+  AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, w, ubar,
+            c->adj_sb, c->adj_part, reg, sreg};
+  hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  a.L = launch_of(c, BOX_WHOLE);
+  hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), nw * NVAR), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_pbar, dim3((nw * NPBAR + 255) / 256), dim3(256), 0, s,
+                     (const double*)c->adj_part, Gi, nw, pbar, accumulate);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+// Adjoint of the fused bt_rk4 step: forward pass storing k_1..k_4, then the
+// reverse sweep stage by stage, each stage input u_s recomputed from u_n and
+// k_{s-1} (stage checkpointing).
+int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,
+                        const double* ubar_out, double* ubar_in, double* pbar, hipStream_t s) {
+  const long long reg = c->total_pts * NVAR;
+  int rc = ensure_buf(c->adj_k, c->adj_k_n, 4 * reg, "rk4_step_vjp");
+  if (!rc) rc = ensure_buf(c->adj_stage, c->adj_stage_n, reg, "rk4_step_vjp");
+  if (!rc) rc = ensure_buf(c->adj_reg, c->adj_reg_n, 3LL * nw * reg, "rk4_step_vjp");
+  if (rc) return rc;
+  double* K[4] = {c->adj_k, c->adj_k + reg, c->adj_k + 2 * reg, c->adj_k + 3 * reg};
+  double* ybar = c->adj_reg;
+  double* kbar = c->adj_reg + (long long)nw * reg;
+  double* usbar = c->adj_reg + 2LL * nw * reg;
+  const double aw[5] = {0., 0., 0.5, 0.5, 1.0};
+  // the right-hand side leaves ghosts untouched: they must read zero
+  rc = fill(c, c->adj_k, 0.0, 4 * reg, s);
+  if (rc) return rc;
+  auto stage_state = [&](int st) -> const double* {  // u_s
+    if (st == 1) return u;
+    const double* kk[1] = {K[st - 2]};
+    const double wa[1] = {aw[st]};
+    if (launch_bt(c, NVAR, c->adj_stage, u, 1, kk, wa, dt, s)) return nullptr;
+    if (launch_alg(c, p, c->adj_stage, nullptr, 0, s)) return nullptr;
+    return c->adj_stage;
+  };
+  for (int st = 1; st <= 4; ++st) {
+    const double* in = stage_state(st);
+    if (!in) return Z4C_EHIP;
+    rc = launch_rhs(c, p, in, nullptr, K[st - 1], false, s);
+    if (rc) return rc;
+  }
+  AdjStepArgs a{};
+  a.blocks = c->d_blocks;
+  a.L = launch_of(c, BOX_WHOLE);
+  a.un = u;
+  for (int q = 0; q < 4; ++q) a.k[q] = K[q];
+  a.ubar_in = ubar_in;
+  a.ybar = ybar;
+  a.kbar = kbar;
+  a.reg = reg;
+  a.dt = dt;
+  a.eps_floor = p->eps_floor;
+  a.tf_force = p->tf_force_regularization;
+  a.force_reg = p->force_regularization;
+  const dim3 g(grid_for(a.L.ntiles), nw);
+  a.bar = ubar_out;
+  a.stage = 5;
+  hipLaunchKernelGGL(k_adj_step, g, dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  for (int st = 4; st >= 1; --st) {
+    const double* in = stage_state(st);
+    if (!in) return Z4C_EHIP;
+    rc = launch_rhs_vjp(c, p, in, nw, kbar, usbar, pbar, st == 4 ? 0 : 1, s);
+    if (rc) return rc;
+    a.bar = usbar;
+    a.stage = st;
+    hipLaunchKernelGGL(k_adj_step, g, dim3(TILE), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
   return Z4C_OK;
 }
 
@@ -2330,6 +2684,30 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   rc = launch_alg(c, p, u, nullptr, 0, s);
   if (rc) return rc;
   return launch_prep(c, p, u, nullptr, s);
+}
+
+int z4c_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const double* w,
+                double* ubar, double* pbar, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !w || !ubar || !pbar) return fail(Z4C_EINVAL, "rhs_vjp: null argument");
+  rc = check_vjp_args(c, p, nw, "rhs_vjp");
+  if (rc) return rc;
+  return launch_rhs_vjp(c, p, u, nw, w, ubar, pbar, 0, (hipStream_t)stream);
+}
+
+int z4c_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,
+                     const double* ubar_out, double* ubar_in, double* pbar, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !ubar_out || !ubar_in || !pbar)
+    return fail(Z4C_EINVAL, "rk4_step_vjp: null argument");
+  rc = check_vjp_args(c, p, nw, "rk4_step_vjp");
+  if (rc) return rc;
+  for (const z4c_block& b : c->hblocks)
+    if (b.sommerfeld_faces & 63)
+      return fail(Z4C_EINVAL, "rk4_step_vjp: Sommerfeld faces have no adjoint here");
+  return launch_rk4_step_vjp(c, p, dt, u, nw, ubar_out, ubar_in, pbar, (hipStream_t)stream);
 }
 
 }  // extern "C"

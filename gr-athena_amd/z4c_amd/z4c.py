@@ -307,6 +307,30 @@ class Z4c:
         check(self.L.z4c_get_timing(self._ctx, ms, cnt, n), "get_timing")
         return {name: (ms[q], cnt[q]) for q, name in enumerate(PASSES)}
 
+    # -- adjoints (no counterpart in the reference) -------------------------------
+    def rhs_vjp(self, u, w, nw=1, params=None, stream=None):
+        """VJP of Z4cRHS at state ``u`` for ``nw`` cotangent registers ``w``
+        (flat, nw*22*total_points): returns (ubar, pbar) with ubar the nw
+        cotangents of u (every vertex) and pbar (nw, 3) the gradients with
+        respect to shift_eta, damp_kappa1, damp_kappa2."""
+        p = params if params is not None else self.params
+        ubar = torch.empty(nw * Z4C_NVAR * self.total_points, dtype=torch.float64, device=self.device)
+        pbar = torch.empty(nw * len(_capi.PBAR), dtype=torch.float64, device=self.device)
+        check(self.L.z4c_rhs_vjp(self._ctx, C.byref(p), _ptr(u), int(nw), _ptr(w), _ptr(ubar),
+                                 _ptr(pbar), _stream(stream)), "rhs_vjp")
+        return ubar, pbar.view(nw, len(_capi.PBAR))
+
+    def rk4_step_vjp(self, dt, u, ubar_out, nw=1, params=None, stream=None):
+        """VJP of one fused bt_rk4 step (``rk4_step_fused``) from u_n = ``u``:
+        ``ubar_out`` holds nw cotangents of u_{n+1}; returns (ubar_in, pbar)."""
+        p = params if params is not None else self.params
+        ubar = torch.empty(nw * Z4C_NVAR * self.total_points, dtype=torch.float64, device=self.device)
+        pbar = torch.empty(nw * len(_capi.PBAR), dtype=torch.float64, device=self.device)
+        check(self.L.z4c_rk4_step_vjp(self._ctx, C.byref(p), float(dt), _ptr(u), int(nw),
+                                      _ptr(ubar_out), _ptr(ubar), _ptr(pbar), _stream(stream)),
+              "rk4_step_vjp")
+        return ubar, pbar.view(nw, len(_capi.PBAR))
+
     def make_step_work(self):
         """Work registers of ``rk4_step_fused``: two stage states and the
         running sum of the final combination."""

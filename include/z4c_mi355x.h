@@ -286,6 +286,35 @@ enum {
 int z4c_set_timing(z4c_ctx* ctx, int enable);
 int z4c_get_timing(z4c_ctx* ctx, double* ms_total, long long* launches, int npass);
 
+/* ------------------------------------------------------------------------
+   Adjoints.  The reference has no derivative code; these are the
+   vector-Jacobian products of its vacuum Z4cRHS (calculate_z4c_rhs.cpp:29-916,
+   without Z4cBoundaryRHS, exactly what z4c_rhs computes) and of one fused
+   bt_rk4 step (z4c_rk4_step), for sensitivities to the initial data and to
+   the gauge parameters.  Parameter gradients per cotangent, in this order: */
+enum { Z4C_PBAR_SHIFT_ETA = 0, Z4C_PBAR_DAMP_KAPPA1 = 1, Z4C_PBAR_DAMP_KAPPA2 = 2, Z4C_NPBAR = 3 };
+
+/* VJP of z4c_rhs at state u for nw cotangents at once (1 <= nw <= 2978):
+   w holds nw registers of 22 * elems_per_var doubles (interior values are
+   read); ubar receives nw registers, ubar_c = (d rhs/d u)^T w_c at every
+   vertex of u, ghosts included; pbar (device, nw * Z4C_NPBAR doubles)
+   receives d<w_c, rhs>/d(shift_eta, damp_kappa1, damp_kappa2).  Vacuum only;
+   Z4C_EINVAL with cowling set.  The context allocates its scratch on the
+   first call (about (180 + 114 nw) doubles per interior vertex). */
+int z4c_rhs_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, int nw, const double* w,
+                double* ubar, double* pbar, void* stream);
+
+/* VJP of one z4c_rk4_step from u_n = u (unchanged): ubar_out holds nw
+   cotangents of u_{n+1}, ubar_in receives those of u_n (every vertex) and
+   pbar (nw * Z4C_NPBAR, device) the parameter gradients of <ubar_out, u_{n+1}>.
+   The forward step is recomputed with its four stage right-hand sides kept
+   (stage checkpointing); the reverse sweep rebuilds each stage state from
+   u_n and the previous right-hand side and applies the AlgConstr adjoint at
+   every stage.  Z4C_EINVAL with Sommerfeld faces or cowling.  Scratch:
+   5 + 3 nw registers of 22 * elems_per_var doubles on top of z4c_rhs_vjp's. */
+int z4c_rk4_step_vjp(z4c_ctx* ctx, const z4c_params* p, double dt, const double* u, int nw,
+                     const double* ubar_out, double* ubar_in, double* pbar, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
