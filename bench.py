@@ -89,6 +89,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default stencil plan (no autotuner trial launches, e.g. under a profiler)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
+    ap.add_argument("--no-sensitivity", action="store_true",
+                    help="skip the 100-step evolution + 10-step adjoint sensitivity leg")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
@@ -249,9 +251,95 @@ def main():
         out["batched_speedup_vs_per_block"] = rate / out["per_block_launch"]["value"]
     if not args.no_adjoint:
         out["adjoint"] = adjoint_cost(params)
+    if not args.no_sensitivity:
+        z.close()
+        del u, work
+        torch.cuda.empty_cache()
+        out["sensitivity"] = sensitivity_run(adm_fn, params)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def sensitivity_run(adm_fn, params, nsteps=100, window=10):
+    """Brill-Lindquist binary on 2048 blocks of 16^3 cells at levels 0-7
+    (octree refined toward the punctures), moving-puncture gauge (1+log,
+    Gamma-driver, eta = 2, kappa1 = 0.02), CFL 0.25: ``nsteps`` fused steps
+    replayed from a HIP graph, then J = sum dx dy dz Ht^2 and the adjoint
+    through the last ``window`` steps (dJ/d eta, kappa1, kappa2 with the
+    window's start state held fixed).  Also times single forward and adjoint
+    steps and checks dJ/d eta and dJ/d kappa1 against central differences
+    over the window (step 1e-3 of the parameter scale)."""
+    import torch
+    from z4c_amd import Params
+    from z4c_amd.initial_data import octree_blocks, owned_points
+    from z4c_amd.sensitivity import hamiltonian_sensitivity
+    from z4c_amd.z4c import Z4c
+    blocks = octree_blocks(nblocks=2048, seed=0, sizes=(16,), mix=(1.0,))
+    z = Z4c(blocks, params)
+    adm = z.empty(17)
+    for b, g in enumerate(blocks):
+        z.block_view(adm, b, 17).copy_(torch.from_numpy(adm_fn(g)))
+    u0 = z.zeros()
+    z.ADMToZ4c(adm, u0)
+    del adm
+    dt = min(z.NewBlockTimeStep(0.25), 1.0)
+    z.autotune(u0, reps=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = hamiltonian_sensitivity(z, u0, dt, nsteps, window)
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    # single-step costs on the window's start state
+    w = res["u_window"]
+    work = z.make_step_work()
+    x = w.clone()
+    z.rk4_step_fused(dt, x, work, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        z.rk4_step_fused(dt, x, work, graph=True)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    z.rk4_step_vjp(dt, w, res["ubar"], 1)
+    torch.cuda.synchronize()
+    t_vjp = time.perf_counter() - t0
+
+    def J_of(**kw):
+        q = Params(**dict(params.as_dict(), **kw))
+        saved = z.params
+        z.params = q
+        y = w.clone()
+        for _ in range(window):
+            z.rk4_step_fused(dt, y, work)
+        J = z.hamiltonian_norm(y)
+        z.params = saved
+        return J
+
+    fd = {}
+    for name in ("shift_eta", "damp_kappa1"):
+        x0 = getattr(params, name)
+        h = 1e-3 * max(abs(x0), 1e-2)
+        d = (J_of(**{name: x0 + h}) - J_of(**{name: x0 - h})) / (2 * h)
+        an = res["pbar"][name]
+        fd[name] = {"adjoint": an, "central_difference": d,
+                    "rel_diff": abs(an - d) / max(abs(d), 1e-300)}
+    z.close()
+    out = {
+        "workload": f"Brill-Lindquist binary, {len(blocks)} blocks of 16^3 cells, levels 0-7, "
+                    f"moving-puncture gauge, CFL 0.25, {nsteps} steps (HIP graph), adjoint "
+                    f"through the last {window}",
+        "owned_points": owned_points(blocks), "dt": dt, "J": res["J"], "dJ": res["pbar"],
+        "seconds_total": t_total, "forward_step_ms": t_step * 1e3, "adjoint_step_ms": t_vjp * 1e3,
+        "adjoint_step_over_forward_step": t_vjp / t_step, "fd_check": fd,
+    }
+    log(f"sensitivity: {nsteps} steps + {window}-step adjoint in {t_total:.2f} s, "
+        f"J = {res['J']:.6e}, dJ/deta = {res['pbar']['shift_eta']:.6e}, "
+        f"dJ/dkappa1 = {res['pbar']['damp_kappa1']:.6e}; step {t_step*1e3:.2f} ms, "
+        f"adjoint step {t_vjp*1e3:.2f} ms; FD rel diff "
+        + ", ".join(f"{k} {v['rel_diff']:.2e}" for k, v in fd.items()))
+    return out
 
 
 def adjoint_cost(params, nw=64, reps=5):

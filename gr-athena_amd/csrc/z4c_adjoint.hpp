@@ -312,6 +312,8 @@ __host__ __device__ __forceinline__ void rhs_fwd_point(const RhsParams& o, const
 //   X.d1(n, a)   d_a u_n, X.d2(n, ab) d_a d_b u_n, X.ls(n, a) Lsel_a(u_n)
 //   Fw(s)        forward intermediate s (rhs_fwd_point)
 //   w[n]         cotangent of rhs_n at the point
+//   wH           cotangent of the Hamiltonian constraint Ht = R + 2/3 K^2 -
+//                A~_ab A~^ab formed inside the right-hand side (:606-624)
 // Outputs: ub(n, v) stores the pointwise cotangent of u_n(p), S.d1(n, a, v)
 // and S.d2(n, ab, v) the derivative cotangents (each exactly once, as soon
 // as final); pb[0..2] += parameter partials.  Forward values are read where
@@ -320,7 +322,7 @@ template <class XIn, class FIn, class UOut, class SOut>
 __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const XIn& X,
                                                        const FIn& Fw, const double* w,
                                                        UOut&& ub, SOut&& S, double* pb,
-                                                       double* bG, int bgs) {
+                                                       double* bG, int bgs, double wH) {
   const double pw = o.chi_psi_power;
   const double k1 = o.damp_kappa1, k2 = o.damp_kappa2;
   auto g = [&](int ab) { return X.u(V_GXX + ab); };
@@ -479,7 +481,7 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     const double Ht = R + (2. / 3.) * (K * K) - trAA;
     const double wt = w[V_THETA];
     b_alpha += wt * (0.5 * Ht - (2. + k2) * k1 * Theta);
-    const double b_Ht = 0.5 * alpha * wt;
+    const double b_Ht = 0.5 * alpha * wt + wH;
     b_Theta -= alpha * (2. + k2) * k1 * wt;
     pb[1] -= alpha * (2. + k2) * Theta * wt;
     pb[2] -= alpha * k1 * Theta * wt;
@@ -811,6 +813,18 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
   for (int ab = 0; ab < 6; ++ab) ub(V_GXX + ab, b_g[ab]);
 }
 
+// Hamiltonian constraint of the Z4c right-hand side at a vertex
+// (calculate_z4c_rhs.cpp:606-624), from the stored forward intermediates.
+__host__ __device__ __forceinline__ double ham_from_fwd(double R, double trAA, double Khat,
+                                                        double Theta) {
+  const double K = Khat + 2. * Theta;
+  return R + (2. / 3.) * (K * K) - trAA;
+}
+// the reference's cell loop of the history sums: vertices il..iu-1 (per axis)
+__host__ __device__ __forceinline__ bool in_cell_loop(const DevBlock& B, int i, int j, int k) {
+  return i < B.iu && j < B.ju && k < B.ku;
+}
+
 // Reverse mode of algconstr_vals (z4c.cpp:677-799) at one point: on entry
 // bg, bA, bchi, balpha hold the cotangents of the outputs, on exit those of
 // the inputs g, A, chi, alpha (the other variables pass through).
@@ -995,18 +1009,18 @@ __host__ __device__ inline void adj_point_at(const RhsParams& o, const DevBlock&
                                              const double* u, const double* lin,
                                              const double* w, double* ubar, double* sb,
                                              int i, int j, int k, double* pb, double* bG,
-                                             int bgs) {
+                                             int bgs, double wH = 0.0) {
   const int N = B.npts;
   const int p = i + B.nn1 * (j + B.nn2 * k);
   const int q = iidx(B, i, j, k);
   double wp[NVAR];
-  for (int n = 0; n < NVAR; ++n) wp[n] = w[(long long)n * N + p];
+  for (int n = 0; n < NVAR; ++n) wp[n] = w ? w[(long long)n * N + p] : 0.0;
   auto ub = [&](int n, double v) { ubar[(long long)n * N + p] = v; };
   LinIn X{LinPoint{u, lin, N, p, B.nint, q}};
   const double* fw = lin + (long long)NLIN * B.nint + q;
   const int nint = B.nint;
   auto Fw = [&](int s) { return fw[(long long)s * nint]; };
-  rhs_vjp_point(o, X, Fw, wp, ub, SlotOut{sb, B.nint, q}, pb, bG, bgs);
+  rhs_vjp_point(o, X, Fw, wp, ub, SlotOut{sb, B.nint, q}, pb, bG, bgs, wH);
 }
 
 // Transposed stencils: the stencil part of ubar_n at vertex (i, j, k) of the
@@ -1034,7 +1048,7 @@ __host__ __device__ inline double adj_stencil_at(const RhsParams& o, const DevBl
   const AdjAxis ax[3] = {{ci, B.ni, 1, 1, iy && iz},
                          {cj, B.nj, B.ni, B.nn1, ix && iz},
                          {ck, B.nk, B.ni * B.nj, B.nn1 * B.nn2, ix && iy}};
-  const double* un = w + (long long)n * N;
+  const double* un = w ? w + (long long)n * N : nullptr;
   double acc = 0.0;
   if (lin1(n) >= 0) {
 #pragma unroll
@@ -1087,6 +1101,7 @@ __host__ __device__ inline double adj_stencil_at(const RhsParams& o, const DevBl
         acc += t * (B.cidx1[a] * B.cidx1[b]);
       }
   }
+  if (!w) return acc;  // zero right-hand-side cotangent: no advective / KO part
   const double lc = adv_coef(o, n);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {

@@ -1194,6 +1194,7 @@ struct AdjArgs {
   double* part;      // nw * gridDim.x * NPBAR workgroup partial sums
   long long reg;     // NVAR * total_pts
   long long sreg;    // NADJ * total_int
+  int hseed;         // 1: seed = d/du of sum vol Ht^2 (w unused, may be null)
 };
 
 __global__ void __launch_bounds__(TILE) k_adj_lin(AdjArgs a) {
@@ -1245,14 +1246,23 @@ __global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
     const AdjFBuf F{X.L, vq, (int)(nint * 8)};
     const AdjSBuf S{make_buf(a.sb + c * a.sreg + B.ioff * NADJ, NADJ * nint * 8), vq,
                     (int)(nint * 8)};
-    const Buf W = make_buf(a.w + c * a.reg + B.off * NVAR, NVAR * N * 8);
     double wp[NVAR];
+    double wH = 0.0;
+    if (a.hseed) {
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) wp[n] = bload(W, vp, n * X.fu);
+      for (int n = 0; n < NVAR; ++n) wp[n] = 0.0;
+      if (in_cell_loop(B, i, j, k))
+        wH = 2.0 * (B.dx[0] * B.dx[1] * B.dx[2]) *
+             ham_from_fwd(F(F_R), F(F_TRAA), X.u(V_KHAT), X.u(V_THETA));
+    } else {
+      const Buf W = make_buf(a.w + c * a.reg + B.off * NVAR, NVAR * N * 8);
+#pragma unroll
+      for (int n = 0; n < NVAR; ++n) wp[n] = bload(W, vp, n * X.fu);
+    }
     const Buf UB = make_buf(a.ubar + c * a.reg + B.off * NVAR, NVAR * N * 8);
     const int fu = X.fu;
     rhs_vjp_point(a.o, X, F, wp, [&](int n, double v) { bstore(UB, v, vp, n * fu); }, S, pb,
-                  shG + threadIdx.x, TILE);
+                  shG + threadIdx.x, TILE, wH);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1277,8 +1287,35 @@ __global__ void __launch_bounds__(TILE) k_adj_stencil(AdjArgs a) {
   const DevBlock B = a.blocks[b];
   double* ub = a.ubar + c * a.reg + B.off * NVAR + (long long)n * B.npts + flat(B, i, j, k);
   const double pt = is_interior(B, i, j, k) ? *ub : 0.0;
-  *ub = pt + adj_stencil_at(a.o, B, a.u + B.off * NVAR, a.w + c * a.reg + B.off * NVAR,
+  *ub = pt + adj_stencil_at(a.o, B, a.u + B.off * NVAR,
+                            a.hseed ? nullptr : a.w + c * a.reg + B.off * NVAR,
                             a.sb + c * a.sreg + B.ioff * NADJ, n, i, j, k);
+}
+
+// Partial sums of vol * Ht^2 over the cell loop, one per workgroup
+__global__ void __launch_bounds__(TILE) k_hnorm(AdjArgs a) {
+  __shared__ double red[TILE / 64];
+  double v = 0.0;
+  int b, i, j, k;
+  if (tile_point(a.L, b, i, j, k)) {
+    const DevBlock B = a.blocks[b];
+    if (in_cell_loop(B, i, j, k)) {
+      const long long N = B.npts, nint = B.nint;
+      const double* un = a.u + B.off * NVAR + flat(B, i, j, k);
+      const double* fw = a.lin + B.ioff * NLINF + (long long)NLIN * nint + iidx(B, i, j, k);
+      const double H = ham_from_fwd(fw[F_R * nint], fw[F_TRAA * nint], un[V_KHAT * N],
+                                    un[V_THETA * N]);
+      v = (B.dx[0] * B.dx[1] * B.dx[2]) * (H * H);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < TILE / 64; ++q) t += red[q];
+    a.part[blockIdx.x] = t;
+  }
 }
 
 // pbar[c*NPBAR + t] (+)= sum of the partials of cotangent c, in order
@@ -1876,7 +1913,7 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   if (!rc) rc = ensure_buf(c->adj_part, c->adj_part_n, (long long)nw * Gi * NPBAR, "rhs_vjp");
   if (rc) return rc;
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, w, ubar,
-            c->adj_sb, c->adj_part, reg, sreg};
+            c->adj_sb, c->adj_part, reg, sreg, w ? 0 : 1};
   hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
@@ -1887,6 +1924,27 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   hipLaunchKernelGGL(k_adj_pbar, dim3((nw * NPBAR + 255) / 256), dim3(256), 0, s,
                      (const double*)c->adj_part, Gi, nw, pbar, accumulate);
   HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+// J = sum over blocks and the cell loop of dx dy dz Ht^2 (host result)
+int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hipStream_t s) {
+  const int Gi = grid_for(c->ntiles[BOX_INTERIOR]);
+  int rc = ensure_buf(c->adj_lin, c->adj_lin_n, c->total_int * NLINF, "hamiltonian_norm");
+  if (!rc) rc = ensure_buf(c->adj_part, c->adj_part_n, (long long)Gi * NPBAR, "hamiltonian_norm");
+  if (rc) return rc;
+  AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, nullptr,
+            nullptr, nullptr, c->adj_part, 0, 0, 1};
+  hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_hnorm, dim3(Gi), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  std::vector<double> part(Gi);
+  HIPCALL(hipMemcpyAsync(part.data(), c->adj_part, sizeof(double) * Gi, hipMemcpyDeviceToHost, s));
+  HIPCALL(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (double v : part) t += v;
+  *J = t;
   return Z4C_OK;
 }
 
@@ -2694,6 +2752,30 @@ int z4c_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const 
   rc = check_vjp_args(c, p, nw, "rhs_vjp");
   if (rc) return rc;
   return launch_rhs_vjp(c, p, u, nw, w, ubar, pbar, 0, (hipStream_t)stream);
+}
+
+int z4c_hamiltonian_norm(z4c_ctx* c, const z4c_params* p, const double* u, double* J,
+                         void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !J) return fail(Z4C_EINVAL, "hamiltonian_norm: null argument");
+  rc = check_vjp_args(c, p, 1, "hamiltonian_norm");
+  if (rc) return rc;
+  return launch_hnorm(c, p, u, J, (hipStream_t)stream);
+}
+
+int z4c_hamiltonian_norm_vjp(z4c_ctx* c, const z4c_params* p, const double* u, double* ubar,
+                             void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !ubar) return fail(Z4C_EINVAL, "hamiltonian_norm_vjp: null argument");
+  rc = check_vjp_args(c, p, 1, "hamiltonian_norm_vjp");
+  if (rc) return rc;
+  double* pbar = nullptr;
+  rc = ensure_buf(c->adj_stage, c->adj_stage_n, c->total_pts * NVAR, "hamiltonian_norm_vjp");
+  if (rc) return rc;
+  pbar = c->adj_stage;  // parameter partials are zero here; scratch target
+  return launch_rhs_vjp(c, p, u, 1, nullptr, ubar, pbar, 0, (hipStream_t)stream);
 }
 
 int z4c_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,

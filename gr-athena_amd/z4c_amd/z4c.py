@@ -331,6 +331,23 @@ class Z4c:
               "rk4_step_vjp")
         return ubar, pbar.view(nw, len(_capi.PBAR))
 
+    def hamiltonian_norm(self, u, params=None, stream=None):
+        """sum over blocks and the history's cell loop of dx dy dz Ht^2, with
+        Ht the Hamiltonian constraint of the right-hand side (waits)."""
+        p = params if params is not None else self.params
+        J = C.c_double()
+        check(self.L.z4c_hamiltonian_norm(self._ctx, C.byref(p), _ptr(u), C.byref(J),
+                                          _stream(stream)), "hamiltonian_norm")
+        return J.value
+
+    def hamiltonian_norm_vjp(self, u, params=None, stream=None):
+        """dJ/du of ``hamiltonian_norm`` at every vertex (one register)."""
+        p = params if params is not None else self.params
+        ubar = torch.empty(Z4C_NVAR * self.total_points, dtype=torch.float64, device=self.device)
+        check(self.L.z4c_hamiltonian_norm_vjp(self._ctx, C.byref(p), _ptr(u), _ptr(ubar),
+                                              _stream(stream)), "hamiltonian_norm_vjp")
+        return ubar
+
     def make_step_work(self):
         """Work registers of ``rk4_step_fused``: two stage states and the
         running sum of the final combination."""

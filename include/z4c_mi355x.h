@@ -315,6 +315,20 @@ int z4c_rhs_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, int nw, cons
 int z4c_rk4_step_vjp(z4c_ctx* ctx, const z4c_params* p, double dt, const double* u, int nw,
                      const double* ubar_out, double* ubar_in, double* pbar, void* stream);
 
+/* Volume integral of the squared Hamiltonian constraint of the Z4c
+   right-hand side, Ht = R + 2/3 (Khat + 2 Theta)^2 - A~_ab A~^ab
+   (calculate_z4c_rhs.cpp:606-624): J = sum over blocks and over the
+   history's cell loop (vertices il..iu-1, jl..ju-1, kl..ku-1,
+   src/outputs/history.cpp:178-205) of dx*dy*dz*Ht^2.  The objective of the
+   adjoint sensitivity runs (the reference's H-norm2 column integrates the
+   H of ADMConstraints, the same constraint discretised through the ADM
+   variables).  Waits for the stream; J is host memory. */
+int z4c_hamiltonian_norm(z4c_ctx* ctx, const z4c_params* p, const double* u, double* J,
+                         void* stream);
+/* ubar (one register of 22 * elems_per_var doubles) = dJ/du at every vertex. */
+int z4c_hamiltonian_norm_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, double* ubar,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,6 +142,20 @@ def _p(p, name):
 def rhs(u, geom, p):
     """Vacuum Z4cRHS of the padded state ``u`` (22, nn3, nn2, nn1): the
     right-hand side on the interior vertices, zero elsewhere."""
+    return _rhs(u, geom, p)[0]
+
+
+def hamiltonian_norm(u, geom, p):
+    """sum over the cell loop (vertices il..iu-1 per axis, as the history
+    output integrates, src/outputs/history.cpp:178-205) of dx dy dz Ht^2,
+    with Ht = R + 2/3 K^2 - A~_ab A~^ab the Hamiltonian constraint formed
+    inside the right-hand side (calculate_z4c_rhs.cpp:606-624)."""
+    _, Ht = _rhs(u, geom, p)
+    vol = float(geom.dx[0]) * float(geom.dx[1]) * float(geom.dx[2])
+    return vol * (Ht[:-1, :-1, :-1] ** 2).sum()
+
+
+def _rhs(u, geom, p):
     pw = _p(p, "chi_psi_power")
     diss = _p(p, "diss")
     k1, k2 = _p(p, "damp_kappa1"), _p(p, "damp_kappa2")
@@ -313,7 +327,7 @@ def rhs(u, geom, p):
 
     full = torch.zeros_like(u)
     full[:, B.lo[2]:B.hi[2] + 1, B.lo[1]:B.hi[1] + 1, B.lo[0]:B.hi[0] + 1] = torch.stack(out)
-    return full
+    return full, Ht
 
 
 def algconstr(u, p):

@@ -81,3 +81,25 @@ def test_autograd_vjp_equals_c_oracle_finite_differences():
     fd = (float((orc.rhs(op, og, u + eps * v) * w).sum())
           - float((orc.rhs(op, og, u - eps * v) * w).sum())) / (2 * eps)
     assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_hamiltonian_norm_of_exact_solutions():
+    """Ht vanishes for Minkowski; for the periodic gauge wave (an exact
+    solution, src/z4c/awa_z4c.cpp:298-351) it converges to zero at 6th
+    order."""
+    from z4c_amd.initial_data import gauge_wave_adm
+    p = Params(**PUNCTURE)
+    op = orc.Params(**p.as_dict())
+    g = MeshBlockGeom.cube(8, 1.0 / 8, (0.1, -0.2, 0.05))
+    og = orc.Geom.block(g.nx, g.dx, g.xmin)
+    u = np.zeros((22,) + og.shape)
+    u[0] = u[1] = u[4] = u[6] = u[18] = 1.0
+    assert zt.hamiltonian_norm(torch.from_numpy(u), og, p.as_dict()).item() < 1e-26
+    errs = []
+    for n in (16, 32):
+        g = MeshBlockGeom.cube(n, 1.0 / n, (-0.5, -0.5, -0.5))
+        og = orc.Geom.block(g.nx, g.dx, g.xmin)
+        u = orc.adm_to_z4c(op, og, np.ascontiguousarray(gauge_wave_adm(g, A=0.1, d=1.0)))
+        orc.periodic_fill(og, u)  # ADMToZ4c forms Gam~ on the interior only
+        errs.append(np.sqrt(zt.hamiltonian_norm(torch.from_numpy(u), og, p.as_dict()).item()))
+    assert 40 < errs[0] / errs[1] < 100, errs
