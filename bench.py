@@ -91,6 +91,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--no-sensitivity", action="store_true",
                     help="skip the 100-step evolution + 10-step adjoint sensitivity leg")
+    ap.add_argument("--sensitivity-only", action="store_true",
+                    help="run only the sensitivity leg and print its object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
@@ -164,6 +166,9 @@ def main():
     from z4c_amd.z4c import Z4c
 
     blocks, params, owned, adm_fn = make_workload(args.nblocks)
+    if args.sensitivity_only:
+        print(json.dumps({"sensitivity": sensitivity_run(adm_fn, params)}), flush=True)
+        return
     interior_pts = int(sum((g.nx[0] + 1) * (g.nx[1] + 1) * (g.nx[2] + 1) for g in blocks))
     log(f"{len(blocks)} blocks, owned points {owned}, interior points {interior_pts}")
     z = Z4c(blocks, params)

@@ -23,11 +23,6 @@
 #include "z4c_device.hpp"
 #include "z4c_rhs_point.hpp"
 
-// Compiler memory fence between sections of the reverse sweep: the loads of
-// a section's forward values are not hoisted above it, which bounds the
-// number of values live at once (no instruction is emitted).
-#define Z4C_ADJ_FENCE() ((void)0)
-
 namespace z4c {
 
 // Linearisation of the state at an interior vertex (the finite-difference
@@ -394,7 +389,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     b_beta[a] -= o.shift_eta * wa;
     pb[0] -= wa * beta(a);
   }
-  Z4C_ADJ_FENCE();
   // lapse (:757-772)
   {
     const double wl = w[V_ALPHA];
@@ -404,7 +398,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     b_Khat -= wl * f * alpha;
     b_Theta -= wl * f * alpha * 2 * o.lapse_K;
   }
-  Z4C_ADJ_FENCE();
   // g~ and A~ (:671-754)
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -434,7 +427,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
       b_K += alpha * A(ab) * wA;
       b_AA[ab] -= 2. * alpha * wA;
     }
-  Z4C_ADJ_FENCE();
   // Gamma~ (:640-729); second-derivative cotangents of beta
   {
     double b_ddbd[3] = {0, 0, 0};
@@ -475,7 +467,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
 #pragma unroll
       for (int bc = 0; bc < 6; ++bc) S.d2(V_BETAX + a, bc, b_ddb[a][bc]);
   }
-  Z4C_ADJ_FENCE();
   // Theta, Khat, chi (:694-711)
   {
     const double Ht = R + (2. / 3.) * (K * K) - trAA;
@@ -513,7 +504,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     ub(V_GAMX + a, b_Gam[a]);
     ub(V_BETAX + a, b_beta[a]);
   }
-  Z4C_ADJ_FENCE();
   // R = sum oopsi4 g^ab RR_ab
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -537,7 +527,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
   // of LDS on the device (42 doubles)
 #pragma unroll
   for (int e = 0; e < 42; ++e) bG[e * bgs] = 0.0;
-  Z4C_ADJ_FENCE();
   // DA (:580-601)
   {
     double b_dK[3] = {0, 0, 0}, b_dTh[3] = {0, 0, 0};
@@ -566,7 +555,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
       S.d1(V_THETA, b, b_dTh[b]);
     }
   }
-  Z4C_ADJ_FENCE();
   // trAA, Auu, AA, Aud (:536-575)
   {
 #pragma unroll
@@ -598,7 +586,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
           b_A[sym(d, b)] += gu(sym(c, d)) * b_Aud[c][b];
         }
   }
-  Z4C_ADJ_FENCE();
 #pragma unroll
   for (int ab = 0; ab < 6; ++ab) ub(V_AXX + ab, b_A[ab]);
   // D_aD_b alpha and its trace (:484-532)
@@ -635,7 +622,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
 #pragma unroll
   for (int a = 0; a < 3; ++a) S.d1(V_ALPHA, a, b_dalpha[a]);
 
-  Z4C_ADJ_FENCE();
   // Ricci (:357-458)
   double b_Ddphi[6] = {};
   double b_trphi = 0.0;
@@ -645,8 +631,7 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     for (int a = 0; a < 3; ++a)
 #pragma unroll
       for (int b = a; b < 3; ++b) {
-        Z4C_ADJ_FENCE();
-        const int ab = sym(a, b);
+              const int ab = sym(a, b);
         const double r = b_RR[ab];
         b_dphi[a] += 4. * dphi(b) * r;
         b_dphi[b] += 4. * dphi(a) * r;
@@ -699,7 +684,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
   }
 #pragma unroll
   for (int cd = 0; cd < 6; ++cd) b_gu[cd] += bG[(36 + cd) * bgs];
-  Z4C_ADJ_FENCE();
   // trphi, Ddphi, dphi
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -734,7 +718,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
     S.d1(V_CHI, a, b_dchi[a]);
   }
   b_chig += -oochi * oochi * b_oochi + ((pw == -4.) ? b_oopsi4 : (-4. / pw) * oopsi4 * oochi * b_oopsi4);
-  Z4C_ADJ_FENCE();
   // Christoffels: Gu, Gudd, Gddd (:300-340)
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -772,7 +755,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
 #pragma unroll
       for (int c = 0; c < 3; ++c) S.d1(V_GXX + ab, c, b_dg[c][ab]);
   }
-  Z4C_ADJ_FENCE();
   // inverse metric: b_g -= G^-1 Sbar G^-1 (symmetric packing)
   {
     double T[3][3];
@@ -806,7 +788,6 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
       for (int j = i; j < 3; ++j) b_g[sym(i, j)] -= mult(i, j) * T[i][j];
   }
 
-  Z4C_ADJ_FENCE();
   // point cotangents of chi and g~
   ub(V_CHI, chi_floored ? 0.0 : b_chig);
 #pragma unroll
@@ -1023,104 +1004,99 @@ __host__ __device__ inline void adj_point_at(const RhsParams& o, const DevBlock&
   rhs_vjp_point(o, X, Fw, wp, ub, SlotOut{sb, B.nint, q}, pb, bG, bgs, wH);
 }
 
-// Transposed stencils: the stencil part of ubar_n at vertex (i, j, k) of the
-// padded block, gathered from the derivative cotangents (first and second
-// derivatives), the advective cotangent adv_coef*w (times beta^a and the
-// lop-sided weights selected at the source vertex) and the KO cotangent w at
-// the interior vertices whose stencils reach (i, j, k).
+// Transposed stencils: the stencil parts of ubar_n (all 22 fields) at vertex
+// (i, j, k) of the padded block, gathered from the derivative cotangents
+// (first and second derivatives), the advective cotangent adv_coef*w (times
+// beta^a and the lop-sided weights selected at the source vertex) and the KO
+// cotangent w at the interior vertices whose stencils reach (i, j, k).
 // Axis a of the gather: source coordinate c_a - o must lie in [0, n_a) with
 // the other two coordinates interior; qs/ps are the interior / padded
-// strides of the axis.
+// strides of the axis.  The bounds, the shift values and the weights of a
+// source are formed once for all fields.  Accessors: S(slot, q) derivative
+// cotangent at interior index q, (*W)(n, p) the cotangent of rhs_n at padded
+// index p (W == nullptr: zero, no advective / KO part), Bx(a, p) beta^a.
+// out(n, v) receives the sums.
 struct AdjAxis {
   int c, n, qs, ps;
   bool others;  // the other two coordinates are interior
 };
 
-__host__ __device__ inline double adj_stencil_at(const RhsParams& o, const DevBlock& B,
-                                                 const double* u, const double* w,
-                                                 const double* sb, int n, int i, int j,
-                                                 int k) {
-  const int N = B.npts, nint = B.nint;
+template <class SIn, class WIn, class BIn, class Out>
+__host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBlock& B,
+                                               const SIn& S, const WIn* W, const BIn& Bx,
+                                               int i, int j, int k, Out&& out) {
   const int ci = i - B.il, cj = j - B.jl, ck = k - B.kl;
   const bool ix = ci >= 0 && ci < B.ni, iy = cj >= 0 && cj < B.nj, iz = ck >= 0 && ck < B.nk;
   const int q0 = (ck * B.nj + cj) * B.ni + ci;  // used only at interior sources
   const int p0 = i + B.nn1 * (j + B.nn2 * k);
-  const AdjAxis ax[3] = {{ci, B.ni, 1, 1, iy && iz},
-                         {cj, B.nj, B.ni, B.nn1, ix && iz},
-                         {ck, B.nk, B.ni * B.nj, B.nn1 * B.nn2, ix && iy}};
-  const double* un = w ? w + (long long)n * N : nullptr;
-  double acc = 0.0;
-  if (lin1(n) >= 0) {
+  const AdjAxis ax0{ci, B.ni, 1, 1, iy && iz}, ax1{cj, B.nj, B.ni, B.nn1, ix && iz},
+      ax2{ck, B.nk, B.ni * B.nj, B.nn1 * B.nn2, ix && iy};
+  // selected by value (no indexed local array, which would live in scratch)
+  auto AX = [&](int a) -> const AdjAxis& { return a == 0 ? ax0 : (a == 1 ? ax1 : ax2); };
+  double acc[NVAR];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double* s1 = sb + (long long)l_d1(n, a) * nint + q0;
-      double t = 0.0;
-      if (ax[a].others)
-#pragma unroll
-        for (int oa = -3; oa <= 3; ++oa) {
-          const int c = ax[a].c - oa;
-          if (oa != 0 && c >= 0 && c < ax[a].n) t += w_d1(oa) * s1[-oa * ax[a].qs];
-        }
-      acc += t * B.cidx1[a];
-    }
-  }
-  if (lin2(n) >= 0) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const double* s2 = sb + (long long)l_d2(n, sym(a, a)) * nint + q0;
-      double t = 0.0;
-      if (ax[a].others)
-#pragma unroll
-        for (int oa = -3; oa <= 3; ++oa) {
-          const int c = ax[a].c - oa;
-          if (c >= 0 && c < ax[a].n) t += w_d2(oa) * s2[-oa * ax[a].qs];
-        }
-      acc += t * B.cidx2[a];
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = a + 1; b < 3; ++b) {
-        const int e = 3 - a - b;  // the third axis
-        if (!((e == 0) ? ix : (e == 1) ? iy : iz)) continue;
-        const double* s2 = sb + (long long)l_d2(n, sym(a, b)) * nint + q0;
-        double t = 0.0;
-#pragma unroll
-        for (int oa = -3; oa <= 3; ++oa) {
-          const int ca = ax[a].c - oa;
-          if (oa == 0 || ca < 0 || ca >= ax[a].n) continue;
-          double tb = 0.0;
-#pragma unroll
-          for (int ob = -3; ob <= 3; ++ob) {
-            const int cb = ax[b].c - ob;
-            if (ob != 0 && cb >= 0 && cb < ax[b].n)
-              tb += w_d1(ob) * s2[-oa * ax[a].qs - ob * ax[b].qs];
-          }
-          t += w_d1(oa) * tb;
-        }
-        acc += t * (B.cidx1[a] * B.cidx1[b]);
-      }
-  }
-  if (!w) return acc;  // zero right-hand-side cotangent: no advective / KO part
-  const double lc = adv_coef(o, n);
+  for (int n = 0; n < NVAR; ++n) acc[n] = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    if (!ax[a].others) continue;
-    const double* bx = u + (long long)(V_BETAX + a) * N + p0;
-    const double* wn = un + p0;
-    double tl = 0.0, tk = 0.0;
+    if (!AX(a).others) continue;
 #pragma unroll
-    for (int oa = -4; oa <= 4; ++oa) {
-      const int c = ax[a].c - oa;
-      if (c < 0 || c >= ax[a].n) continue;
-      const double wv = wn[-oa * ax[a].ps];
-      const double vx = bx[-oa * ax[a].ps];
-      tl += (vx < 0 ? w_left(oa) : w_right(oa)) * (vx * wv);
-      tk += w_ko(oa) * wv;
+    for (int oa = -3; oa <= 3; ++oa) {
+      const int c = AX(a).c - oa;
+      if (c < 0 || c >= AX(a).n) continue;
+      const int sq = q0 - oa * AX(a).qs;
+      const double c1 = w_d1(oa) * B.cidx1[a], c2 = w_d2(oa) * B.cidx2[a];
+#pragma unroll
+      for (int n = 0; n < NVAR; ++n) {
+        if (lin1(n) >= 0 && oa != 0) acc[n] += c1 * S(l_d1(n, a), sq);
+        if (lin2(n) >= 0) acc[n] += c2 * S(l_d2(n, sym(a, a)), sq);
+      }
     }
-    acc += lc * tl * B.lidx[a] + tk * (B.cidxd[a] * o.diss);
   }
-  return acc;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a + 1; b < 3; ++b) {
+      const int e = 3 - a - b;  // the third axis
+      if (!((e == 0) ? ix : (e == 1) ? iy : iz)) continue;
+      const double cab = B.cidx1[a] * B.cidx1[b];
+#pragma unroll
+      for (int oa = -3; oa <= 3; ++oa) {
+        const int ca = AX(a).c - oa;
+        if (oa == 0 || ca < 0 || ca >= AX(a).n) continue;
+#pragma unroll
+        for (int ob = -3; ob <= 3; ++ob) {
+          const int cb = AX(b).c - ob;
+          if (ob == 0 || cb < 0 || cb >= AX(b).n) continue;
+          const int sq = q0 - oa * AX(a).qs - ob * AX(b).qs;
+          const double cw = (w_d1(oa) * w_d1(ob)) * cab;
+#pragma unroll
+          for (int n = 0; n < NVAR; ++n)
+            if (lin2(n) >= 0) acc[n] += cw * S(l_d2(n, sym(a, b)), sq);
+        }
+      }
+    }
+  if (W) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!AX(a).others) continue;
+#pragma unroll
+      for (int oa = -4; oa <= 4; ++oa) {
+        const int c = AX(a).c - oa;
+        if (c < 0 || c >= AX(a).n) continue;
+        const int ps = p0 - oa * AX(a).ps;
+        const double vx = Bx(a, ps);
+        const double cl = (vx < 0 ? w_left(oa) : w_right(oa)) * vx * B.lidx[a];
+        const double ck2 = w_ko(oa) * (B.cidxd[a] * o.diss);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) {
+          const double wn = (*W)(n, ps);
+          acc[n] += (adv_coef(o, n) * cl + ck2) * wn;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) out(n, acc[n]);
 }
 
 }  // namespace z4c

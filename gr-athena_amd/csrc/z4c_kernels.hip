@@ -1280,16 +1280,43 @@ __global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_stencil(AdjArgs a) {
-  const long long c = blockIdx.y / NVAR;
-  const int n = blockIdx.y % NVAR;
+  const long long c = blockIdx.y;
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
-  double* ub = a.ubar + c * a.reg + B.off * NVAR + (long long)n * B.npts + flat(B, i, j, k);
-  const double pt = is_interior(B, i, j, k) ? *ub : 0.0;
-  *ub = pt + adj_stencil_at(a.o, B, a.u + B.off * NVAR,
-                            a.hseed ? nullptr : a.w + c * a.reg + B.off * NVAR,
-                            a.sb + c * a.sreg + B.ioff * NADJ, n, i, j, k);
+  const long long N = B.npts, nint = B.nint;
+  const int fu = (int)(N * 8), fs = (int)(nint * 8);
+  const Buf SB = make_buf(a.sb + c * a.sreg + B.ioff * NADJ, NADJ * nint * 8);
+  const Buf UB = make_buf(a.u + B.off * NVAR, NVAR * N * 8);
+  const Buf WB = make_buf(a.hseed ? a.u : a.w + c * a.reg + B.off * NVAR, NVAR * N * 8);
+  const Buf OB = make_buf(a.ubar + c * a.reg + B.off * NVAR, NVAR * N * 8);
+  auto S = [&](int slot, int q) { return bload(SB, (unsigned)q * 8u, slot * fs); };
+  auto W = [&](int n, int p) { return bload(WB, (unsigned)p * 8u, n * fu); };
+  auto Bx = [&](int d, int p) { return bload(UB, (unsigned)p * 8u, (V_BETAX + d) * fu); };
+  const unsigned vp = (unsigned)flat(B, i, j, k) * 8u;
+  const bool interior = is_interior(B, i, j, k);
+  adj_stencil_at(a.o, B, S, a.hseed ? nullptr : &W, Bx, i, j, k, [&](int n, double v) {
+    const double pt = interior ? bload(OB, vp, n * fu) : 0.0;
+    bstore(OB, pt + v, vp, n * fu);
+  });
+}
+
+// pbar[c*NPBAR + t] (+)= sum of the partials of cotangent c: one workgroup
+// per (c, t), strided partial sums then a fixed-order tree (deterministic)
+__global__ void __launch_bounds__(256) k_adj_pbar(const double* part, int G, int nw, double* pbar,
+                                                  int accumulate) {
+  __shared__ double red[256];
+  const int e = blockIdx.x;
+  const int c = e / NPBAR, t = e % NPBAR;
+  double v = 0.0;
+  for (int g = threadIdx.x; g < G; g += 256) v += part[((long long)c * G + g) * NPBAR + t];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pbar[e] = accumulate ? pbar[e] + red[0] : red[0];
 }
 
 // Partial sums of vol * Ht^2 over the cell loop, one per workgroup
@@ -1316,16 +1343,6 @@ __global__ void __launch_bounds__(TILE) k_hnorm(AdjArgs a) {
     for (int q = 0; q < TILE / 64; ++q) t += red[q];
     a.part[blockIdx.x] = t;
   }
-}
-
-// pbar[c*NPBAR + t] (+)= sum of the partials of cotangent c, in order
-__global__ void k_adj_pbar(const double* part, int G, int nw, double* pbar, int accumulate) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nw * NPBAR) return;
-  const int c = e / NPBAR, t = e % NPBAR;
-  double v = 0.0;
-  for (int g = 0; g < G; ++g) v += part[((long long)c * G + g) * NPBAR + t];
-  pbar[e] = accumulate ? pbar[e] + v : v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1891,7 +1908,7 @@ int ensure_buf(double*& p, long long& have, long long need, const char* what) {
   return Z4C_OK;
 }
 
-constexpr int MAX_NW = 65535 / NVAR;  // grid.y of the transposed-stencil kernel
+constexpr int MAX_NW = 65535 / NPBAR;  // grid.y / grid.x limits of the adjoint kernels
 
 int check_vjp_args(const z4c_ctx* c, const z4c_params* p, int nw, const char* what) {
   if (nw < 1 || nw > MAX_NW)
@@ -1919,9 +1936,9 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   a.L = launch_of(c, BOX_WHOLE);
-  hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), nw * NVAR), dim3(TILE), 0, s, a);
+  hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
-  hipLaunchKernelGGL(k_adj_pbar, dim3((nw * NPBAR + 255) / 256), dim3(256), 0, s,
+  hipLaunchKernelGGL(k_adj_pbar, dim3(nw * NPBAR), dim3(256), 0, s,
                      (const double*)c->adj_part, Gi, nw, pbar, accumulate);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
