@@ -41,15 +41,17 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 
 # Algorithmic HBM bytes of each pass per launch, from what the pass must read
 # and write once (doubles): per interior vertex (I) and per padded vertex (P).
-#  stencil:   reads the 22 fields over the padded block, writes 128 slots
-#  curvature: reads 7 fields + 51 slots, writes 12 curvature values
-#  assembly:  reads 98 slots + 12 curvature + 22 fields, and per stage on
-#             average 16.5 (u_n) + 16.5 (running sum) doubles, writes 16.5
-#             (running sum) + 22 (stage state)
+#  stencil:   reads the 22 fields over the padded block, writes 106 slots
+#  curvature: (separate pass of z4c_rhs) reads 7 fields + 51 slots, writes
+#             12 curvature values
+#  assembly:  (fused step: curvature, assembly, stage update, AlgConstr)
+#             reads the 106 slots + 22 fields, and per stage on average
+#             16.5 (u_n) + 16.5 (running sum) doubles, writes 16.5 (running
+#             sum) + 22 (stage state)
 PASS_BYTES = {
-    "stencil": lambda I, P: 8.0 * (128 * I + 22 * P),
+    "stencil": lambda I, P: 8.0 * (106 * I + 22 * P),
     "curvature": lambda I, P: 8.0 * (7 + 51 + 12) * I,
-    "assembly": lambda I, P: 8.0 * (98 + 12 + 22 + 16.5 + 16.5 + 16.5 + 22) * I,
+    "assembly": lambda I, P: 8.0 * (106 + 22 + 16.5 + 16.5 + 16.5 + 22) * I,
     "ghosts": lambda I, P: 8.0 * (22 + 14 + 44) * (P - I),
     "prepare": lambda I, P: 8.0 * (11 + 33) * P,
 }

@@ -72,15 +72,19 @@ struct DevBlock {
 // Derivative slots per interior point, written by the stencil pass and read
 // by the algebra pass (field stride = interior points of the block).
 //   FULL fields (chi, alpha, beta^i): d_a (3), d_a d_b (6, SYM2 order;
-//        off-diagonal = nested Dx(a, Dx(b, .)) with a < b), L, D
-//   METRIC fields (g~_ab): d_a (3), P_ab, L, D with
+//        off-diagonal = nested Dx(a, Dx(b, .)) with a < b), LD
+//   METRIC fields (g~_ab): d_a (3), P_ab, LD with
 //        P_ab = sum_{c<=d} w_cd g~^cd d_c d_d g~_ab  (w = 1/2 on the
 //        diagonal, 1 off it; SYM2 order), the second-derivative part of the
 //        Ricci tensor (calculate_z4c_rhs.cpp:270-286), formed where the
 //        second derivatives are
-//   GRAD fields (Khat, Gam~^i, Theta):       d_a (3), L, D
-//   ADV  fields (A~_ij):                      L, D
-// L = sum_a Lx(a, beta^a, .) in a order; D = (Diss_x + Diss_y) + Diss_z.
+//   GRAD fields (Khat, Gam~^i, Theta):       d_a (3), LD
+//   ADV  fields (A~_ij):                      LD
+// LD = c_n L + D with L = sum_a Lx(a, beta^a, .) in a order, D = (Diss_x +
+// Diss_y) + Diss_z and c_n the advection coefficient of the field's
+// right-hand side (lapse_advect for alpha, shift_advect for beta^i, 1
+// otherwise): every right-hand side takes its advective and KO terms as
+// plain additive terms (calculate_z4c_rhs.cpp:694-902, 908-915).
 // ---------------------------------------------------------------------------
 enum : int { DK_FULL = 0, DK_GRAD = 1, DK_ADV = 2, DK_METRIC = 3 };
 __host__ __device__ constexpr int dkind(int n) {
@@ -93,20 +97,19 @@ __host__ __device__ constexpr int dkind(int n) {
                            : DK_ADV));
 }
 __host__ __device__ constexpr int dnslots(int kind) {
-  return kind == DK_FULL ? 11 : (kind == DK_METRIC ? 6 : (kind == DK_GRAD ? 5 : 2));
+  return kind == DK_FULL ? 10 : (kind == DK_METRIC ? 5 : (kind == DK_GRAD ? 4 : 1));
 }
 __host__ __device__ constexpr int dbase(int n) {
   int s = 0;
   for (int m = 0; m < n; ++m) s += dnslots(dkind(m));
   return s;
 }
-constexpr int NDV = 128;
+constexpr int NDV = 106;
 static_assert(dbase(NVAR) == NDV, "derivative slot layout");
 __host__ __device__ constexpr int s_d(int n, int a) { return dbase(n) + a; }
 __host__ __device__ constexpr int s_dd(int n, int ab) { return dbase(n) + 3 + ab; }
 __host__ __device__ constexpr int s_lap(int n) { return dbase(n) + 3; }
-__host__ __device__ constexpr int s_L(int n) { return dbase(n) + dnslots(dkind(n)) - 2; }
-__host__ __device__ constexpr int s_D(int n) { return dbase(n) + dnslots(dkind(n)) - 1; }
+__host__ __device__ constexpr int s_LD(int n) { return dbase(n) + dnslots(dkind(n)) - 1; }
 
 // Tile of the interior x-y plane of a block for the z-marching stencil pass
 // (interior-relative origin i0, j0 and extents ni, nj).

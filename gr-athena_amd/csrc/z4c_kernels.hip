@@ -396,9 +396,12 @@ struct StageArgs {
   int tf_force, force_reg;
 };
 
+// The curvature part (Ricci, D_aD_b alpha) is formed in the same kernel and
+// staged in LDS (a per-thread column) instead of a separate pass through HBM.
 template <bool WITH_SOMMERFELD>
 __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   __shared__ double ko[NVAR * TILE];  // this point's k_s, staged (see k_rhs_alg)
+  __shared__ double cvs[12 * TILE];   // this point's curvature part
   int b, i, j, k;
   if (!tile_point(a.r.L, b, i, j, k)) return;
   const int t = threadIdx.x;
@@ -411,9 +414,8 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   } else {
     const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
     const PointIO X = point_io(a.r, B, p, qi);
-    const Buf CV = make_buf(a.r.cv + B.ioff * 12, (long long)12 * B.nint * 8);
-    const int fc = B.nint * 8;
-    rhs_assembly_point(a.r.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
+    rhs_curvature_point(a.r.o, X, [&](int n, double v) { cvs[n * TILE + t] = v; });
+    rhs_assembly_point(a.r.o, X, [&](int n) { return cvs[n * TILE + t]; },
                        [&](int n, double v) { ko[n * TILE + t] = v; });
     if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
       const Pt P = make_pt(p, B.nn1, B.nn2, N);
@@ -1736,11 +1738,12 @@ void launch_stencil_cfg(int cfg, const StencilArgs& sa, hipStream_t s) {
 }
 
 // The stencil pass over all blocks: one launch per configuration in use.
-int launch_stencil(z4c_ctx* c, const double* u, double diss, hipStream_t s) {
+int launch_stencil(z4c_ctx* c, const double* u, const z4c_params* p, hipStream_t s) {
   PassTimer t(c, Z4C_PASS_STENCIL, s);
   for (int q = 0; q < NSTCFG; ++q) {
     if (!c->st_ntiles[q]) continue;
-    StencilArgs sa{c->d_blocks, c->st_tiles[q], c->st_ntiles[q], u, c->dv, diss};
+    StencilArgs sa{c->d_blocks, c->st_tiles[q], c->st_ntiles[q], u, c->dv, p->diss,
+                   p->lapse_advect, p->shift_advect};
     launch_stencil_cfg(q, sa, s);
     HIPCALL(hipGetLastError());
   }
@@ -1750,7 +1753,7 @@ int launch_stencil(z4c_ctx* c, const double* u, double diss, hipStream_t s) {
 int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
                const double* mat, double* rhs, bool sommerfeld, hipStream_t s) {
   if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
-  int rc = launch_stencil(c, u, p->diss, s);
+  int rc = launch_stencil(c, u, p, s);
   if (rc) return rc;
   RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
   const int g = grid_for(a.L.ntiles);
@@ -2532,7 +2535,8 @@ int z4c_autotune(z4c_ctx* c, const z4c_params* p, const double* u, int reps, voi
         rc = fail(Z4C_EHIP, "autotune: copy failed");
         break;
       }
-      StencilArgs sa{c->d_blocks, d, (int)tl.size(), u, c->dv, p->diss};
+      StencilArgs sa{c->d_blocks, d, (int)tl.size(), u, c->dv, p->diss, p->lapse_advect,
+                     p->shift_advect};
       launch_stencil_cfg(q, sa, s);  // warm-up
       hipEventRecord(e0, s);
       for (int r = 0; r < reps; ++r) launch_stencil_cfg(q, sa, s);
@@ -2648,13 +2652,8 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
     a.force_reg = p->force_regularization;
     const int g = grid_for(a.r.L.ntiles);
     if (!p->cowling) {
-      int rc2 = launch_stencil(c, in, p->diss, s);
+      int rc2 = launch_stencil(c, in, p, s);
       if (rc2) return rc2;
-      {
-        PassTimer t(c, Z4C_PASS_CURVATURE, s);
-        hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a.r);
-      }
-      HIPCALL(hipGetLastError());
     }
     {
       PassTimer t(c, Z4C_PASS_ASSEMBLY, s);

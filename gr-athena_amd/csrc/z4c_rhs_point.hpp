@@ -3,9 +3,11 @@
 // Pointwise: reads the state at the point and the derivative slots written
 // by the stencil pass (coalesced, one buffer offset per thread) and
 // evaluates Z4c::Z4cRHS (src/z4c/calculate_z4c_rhs.cpp:29-916) term by term
-// in the reference's operation order.  The only reordering: the three KO
-// terms, which the reference adds to the finished RHS one direction at a
-// time (:908-915), arrive pre-summed as (D_x + D_y) + D_z.
+// in the reference's operation order.  Reorderings (rounding level): the
+// three KO terms, which the reference adds to the finished RHS one direction
+// at a time (:908-915), arrive pre-summed as (D_x + D_y) + D_z and added to
+// the advective sum in the stencil pass (slot LD = c L + D, entering where
+// the reference adds L); the Ricci second-derivative term arrives as P_ab.
 // Vacuum build and constant shift damping (no Z4C_ETA_CONF /
 // Z4C_ETA_TRACK_TP); matter sources enter through `mat` when present.
 #pragma once
@@ -345,21 +347,21 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
 
   // ----------------------------------------- chi, Khat, Theta (:694-711)
   {
-    double Lchi = X.d(s_L(V_CHI));
+    double Lchi = X.d(s_LD(V_CHI));
     Lchi += (1. / 6.) * pw * chi_guarded * dbeta_div;
     const double r = Lchi - (1. / 6.) * pw * chi_guarded * alpha * K;
-    out(V_CHI, r + X.d(s_D(V_CHI)));
+    out(V_CHI, r);
   }
   {
-    double r = -Ddalpha + alpha * (trAA + (1. / 3.) * (K * K)) + X.d(s_L(V_KHAT)) +
+    double r = -Ddalpha + alpha * (trAA + (1. / 3.) * (K * K)) + X.d(s_LD(V_KHAT)) +
                k1 * (1 - k2) * alpha * Theta;
     r += 4 * M_PI * alpha * (Str + rho);
-    out(V_KHAT, r + X.d(s_D(V_KHAT)));
+    out(V_KHAT, r);
   }
   {
-    double r = X.d(s_L(V_THETA)) + alpha * (0.5 * Ht - (2. + k2) * k1 * Theta);
+    double r = X.d(s_LD(V_THETA)) + alpha * (0.5 * Ht - (2. + k2) * k1 * Theta);
     r -= 8. * M_PI * alpha * rho;
-    out(V_THETA, r + X.d(s_D(V_THETA)));
+    out(V_THETA, r);
   }
 
   // -------------------------------------------------- Gamma~ (:640-729)
@@ -374,7 +376,7 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      double LGam = X.d(s_L(V_GAMX + a));
+      double LGam = X.d(s_LD(V_GAMX + a));
       LGam += (2. / 3.) * Gu[a] * dbeta_div;
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
@@ -390,7 +392,7 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
         t -= 2. * Auu[sym(a, b)] * dalpha[b];
         t -= 16. * M_PI * alpha * guu[sym(a, b)] * Sd[b];
       }
-      out(V_GAMX + a, t + X.d(s_D(V_GAMX + a)));
+      out(V_GAMX + a, t);
     }
   }
 
@@ -400,8 +402,8 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
 #pragma unroll
     for (int b = a; b < 3; ++b) {
       const int ab = sym(a, b);
-      double Lg = X.d(s_L(V_GXX + ab));
-      double LA = X.d(s_L(V_AXX + ab));
+      double Lg = X.d(s_LD(V_GXX + ab));
+      double LA = X.d(s_LD(V_AXX + ab));
       Lg -= (2. / 3.) * gdd[ab] * dbeta_div;
       LA -= (2. / 3.) * Add[ab] * dbeta_div;
 #pragma unroll
@@ -417,21 +419,21 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
       t += alpha * (K * Add[ab] - 2. * AA[ab]);
       t += LA;
       t -= 8. * M_PI * alpha * (oopsi4 * Sdd[ab] - (1. / 3.) * Str * gdd[ab]);
-      out(V_GXX + ab, r_g + X.d(s_D(V_GXX + ab)));
-      out(V_AXX + ab, t + X.d(s_D(V_AXX + ab)));
+      out(V_GXX + ab, r_g);
+      out(V_AXX + ab, t);
     }
 
   // ------------------------------------------- lapse and shift (:757-902)
   {
     const double f = o.lapse_oplog * o.lapse_harmonicf + o.lapse_harmonic * alpha;
-    const double r = (o.lapse_advect * X.d(s_L(V_ALPHA)) -
+    const double r = (X.d(s_LD(V_ALPHA)) -
                       f * alpha * (Khat + 2 * o.lapse_K * Theta));
-    out(V_ALPHA, r + X.d(s_D(V_ALPHA)));
+    out(V_ALPHA, r);
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const double Gam_a = X.u(V_GAMX + a);
-    double r = o.shift_Gamma * Gam_a + o.shift_advect * X.d(s_L(V_BETAX + a));
+    double r = o.shift_Gamma * Gam_a + X.d(s_LD(V_BETAX + a));
     if (fabs(o.shift_alpha2Gamma) > 0.0) r += o.shift_alpha2Gamma * (alpha * alpha) * Gam_a;
     if (fabs(o.shift_H) > 0.0) {
 #pragma unroll
@@ -440,7 +442,7 @@ __device__ __forceinline__ void rhs_assembly_point(const RhsParams& o, const Poi
              guu[sym(a, b)];
     }
     r -= o.shift_eta * X.u(V_BETAX + a);
-    out(V_BETAX + a, r + X.d(s_D(V_BETAX + a)));
+    out(V_BETAX + a, r);
   }
 }
 

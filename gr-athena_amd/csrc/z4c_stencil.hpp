@@ -66,6 +66,7 @@ struct StencilArgs {
   const double* u;  // state, 22 fields
   double* dv;       // derivative slots
   double diss;
+  double adv_alpha, adv_beta;  // lapse_advect, shift_advect
 };
 
 template <class Cfg, int KIND>
@@ -88,7 +89,8 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   const Buf GM = make_buf(a.u + B.off * NVAR + (long long)V_GXX * N, 6LL * N * 8);
   const Buf DVB = make_buf(a.dv + B.doff, (long long)NDV * B.nis * 8);
   const int NS8 = B.nis * 8;
-  const int db = dbase(f), sl = s_L(f), sd = s_D(f);
+  const int db = dbase(f), sld = s_LD(f);
+  const double lc = (f == V_ALPHA) ? a.adv_alpha : ((f >= V_BETAX) ? a.adv_beta : 1.0);
   const int PX = T.ni + 8, PP = PX * (T.nj + 8);
   const int EX = T.ni + 6, EE = EX * (T.nj + 6);
   const int EY = EX * T.nj;  // Dx_y points: tile rows, x in [-3, ni+3)
@@ -244,9 +246,8 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
     if (full) dd[5] = st_dxx(zr, B.cidx2[2]);
     // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
     L += st_lx(zr, B.lidx[2], bz);
-    bstore(DVB, L, q8, sl * NS8);
     D += st_diss(zr, B.cidxd[2], a.diss);
-    bstore(DVB, D, q8, sd * NS8);
+    bstore(DVB, lc * L + D, q8, sld * NS8);
     if (full) {
       double v[9];
       v[0] = v[8] = 0.0;

@@ -269,8 +269,8 @@ int z4c_set_stencil_threads(z4c_ctx* ctx, int block, int threads);
 /* Passes of the kernel timer. */
 enum {
   Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */
-  Z4C_PASS_CURVATURE = 1, /* Ricci and D_i D_j alpha */
-  Z4C_PASS_ASSEMBLY = 2,  /* RHS assembly (fused with SumBT + AlgConstr in z4c_rk4_step) */
+  Z4C_PASS_CURVATURE = 1, /* Ricci and D_i D_j alpha (z4c_rhs; inside the assembly in z4c_rk4_step) */
+  Z4C_PASS_ASSEMBLY = 2,  /* RHS assembly (in z4c_rk4_step fused with the curvature part, SumBT and AlgConstr) */
   Z4C_PASS_GHOSTS = 3,    /* ghost pass of z4c_rk4_step */
   Z4C_PASS_PREPARE = 4,   /* PrepareZ4cDerivatives */
   Z4C_PASS_BT_SUM = 5,    /* SumBT */

@@ -165,11 +165,15 @@ def test_sommerfeld_block_evolution_with_physical_fill():
         exp = cur
     torch.cuda.synchronize()
     got = z.download(u, 22)[0]
-    # Interior vertices: the RHS, the face RHS and the update, 1e-11.
+    # Interior vertices: the RHS, the face RHS and the update.  1e-10: the
+    # vertices next to the faces read ghosts that the fill extrapolated from
+    # the previous stage, so rounding-level differences of the RHS (operation
+    # order of the KO and advective sums) come back amplified by the
+    # extrapolation weights (up to 10 per layer).
     sl = (slice(None),) + (slice(og.il, og.iu + 1),) * 3
     num = np.abs(got[sl] - exp[sl]).reshape(22, -1).max(axis=1)
     den = np.maximum(np.abs(exp[sl]).reshape(22, -1).max(axis=1), 0.01)
-    assert np.all(num / den <= 1e-11), num / den
+    assert np.all(num / den <= 1e-10), num / den
     # Ghosts: the fill reads the outermost ghost layer (iu+4), which it never
     # writes, and repeated 5-point extrapolation (coefficient sum 31 per layer)
     # grows the far ghost corners to O(1e2) within a step; rounding
