@@ -1179,6 +1179,266 @@ __global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): Re / Im psi4 at
+// interior vertices, one thread per vertex, in the reference's operation
+// order: physical metric derivatives by the chain rule from the prepared
+// conformal derivatives (:136-262; the mixed second derivatives are Dx_a of
+// the prepared d_b g~, the mixed d_a d_b chi Dx_a of the prepared d_b chi, as
+// PrepareZ4cDerivatives forms them), Christoffels and Ricci of the physical
+// metric (:264-347), K traces and D_a K_bc (:349-390), the coordinate tetrad
+// orthonormalised with gamma (:392-512) and the Gauss / Codazzi projection
+// onto it (:514-628).
+// ---------------------------------------------------------------------------
+struct WeylArgs {
+  const DevBlock* blocks;
+  Launch L;
+  const double* u;
+  const double* adm;
+  const double* d1;  // prepared first derivatives (k_prep)
+  double* out;       // 2 fields per block
+  double chi_psi_power, chi_div_floor;
+};
+
+__global__ void __launch_bounds__(TILE) k_weyl(WeylArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const int p = flat(B, i, j, k);
+  const Buf U = make_buf(a.u + B.off * NVAR, (long long)NVAR * N * 8);
+  const Buf AD = make_buf(a.adm + B.off * NADM, (long long)NADM * N * 8);
+  const Buf D1 = make_buf(a.d1 + B.off * ND1, (long long)ND1 * N * 8);
+  const Pt P = make_pt(p, B.nn1, B.nn2, N);
+  const double pw = a.chi_psi_power;
+  const double FR4 = 0.25;
+  const double chi_g = dmax(ldc(U, P, V_CHI), a.chi_div_floor);
+  double dchi[3], ddc[6], gt[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) dchi[c] = ldc(D1, P, c * 11 + F_CHI);
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) gt[ab] = ldc(U, P, V_GXX + ab);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    ddc[sym(c, c)] = fd_dxx(U, P, V_CHI, c, B.cidx2[c]);
+#pragma unroll
+    for (int d = c + 1; d < 3; ++d) ddc[sym(c, d)] = fd_dx(D1, P, d * 11 + F_CHI, c, B.cidx1[c]);
+  }
+  double dg[3][6], dK[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      const double dg3 = ldc(D1, P, c * 11 + F_G + ab);
+      if (pw == -4.) {
+        const double oochi = 1.0 / chi_g;
+        dg[c][ab] = oochi * (dg3 - oochi * gt[ab] * dchi[c]);
+      } else {
+        const double fac = 4.0 / pw;
+        const double psi4 = pow(chi_g, fac);
+        const double oochi = 1.0 / chi_g;
+        dg[c][ab] = psi4 * (dg3 + fac * oochi * gt[ab] * dchi[c]);
+      }
+      dK[c][ab] = fd_dx(AD, P, A_KXX + ab, c, B.cidx1[c]);
+    }
+  double ddg[6][6];  // [sym(a,b)][sym(c,d)] = d_a d_b gamma_cd
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = c; d < 3; ++d)
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = x; y < 3; ++y) {
+          const int cd = sym(c, d);
+          const double gtil = gt[cd];
+          const double dgx = ldc(D1, P, x * 11 + F_G + cd), dgy = ldc(D1, P, y * 11 + F_G + cd);
+          double v;
+          if (pw == -4.) {
+            const double oochi = 1.0 / chi_g;
+            if (x == y) {
+              const double conf = fd_dxx(U, P, V_GXX + cd, x, B.cidx2[x]);
+              v = oochi * (conf - oochi * (2.0 * dchi[x] * dgx + ddc[sym(x, x)] * gtil) +
+                           2.0 * oochi * oochi * dchi[x] * dchi[x] * gtil);
+            } else {
+              const double conf = fd_dx(D1, P, y * 11 + F_G + cd, x, B.cidx1[x]);
+              v = oochi * (conf - oochi * (dchi[x] * dgy + dchi[y] * dgx + ddc[sym(x, y)] * gtil) +
+                           2.0 * oochi * oochi * dchi[x] * dchi[y] * gtil);
+            }
+          } else {
+            const double fac = 4.0 / pw, fac2 = fac - 1.0;
+            const double psi4 = pow(chi_g, fac);
+            const double oochi = 1.0 / chi_g;
+            if (x == y) {
+              const double conf = fd_dxx(U, P, V_GXX + cd, x, B.cidx2[x]);
+              v = psi4 * (conf + fac * oochi * (2.0 * dchi[x] * dgx + ddc[sym(x, x)] * gtil) +
+                          fac * fac2 * oochi * oochi * dchi[x] * dchi[x] * gtil);
+            } else {
+              const double conf = fd_dx(D1, P, y * 11 + F_G + cd, x, B.cidx1[x]);
+              v = psi4 * (conf +
+                          fac * oochi * (dchi[x] * dgy + dchi[y] * dgx + ddc[sym(x, y)] * gtil) +
+                          fac * fac2 * oochi * oochi * dchi[x] * dchi[y] * gtil);
+            }
+          }
+          ddg[sym(x, y)][cd] = v;
+        }
+  double gd[6], Kd[6], gu[6];
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    gd[ab] = ldc(AD, P, A_GXX + ab);
+    Kd[ab] = ldc(AD, P, A_KXX + ab);
+  }
+  inv3(1. / det3(gd), gd, gu);
+  double Gd[3][6], Gu[3][6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = x; y < 3; ++y)
+        Gd[c][sym(x, y)] = 0.5 * (dg[x][sym(y, c)] + dg[y][sym(x, c)] - dg[c][sym(x, y)]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      double t = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) t += gu[sym(c, d)] * Gd[d][ab];
+      Gu[c][ab] = t;
+    }
+  double Rd[6], R = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y) {
+      double r = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            r += gu[sym(c, d)] * Gu[e][sym(x, c)] * Gd[e][sym(y, d)];
+            r -= gu[sym(c, d)] * Gu[e][sym(x, y)] * Gd[e][sym(c, d)];
+          }
+          r += 0.5 * gu[sym(c, d)] *
+               (-ddg[sym(c, d)][sym(x, y)] - ddg[sym(x, y)][sym(c, d)] +
+                ddg[sym(x, c)][sym(y, d)] + ddg[sym(y, c)][sym(x, d)]);
+        }
+      Rd[sym(x, y)] = r;
+      R += ((x == y) ? 1.0 : 2.0) * gu[sym(x, y)] * r;
+    }
+  double K = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t += gu[sym(x, c)] * Kd[sym(c, x)];
+    K += t;
+  }
+  double DK[3][6];  // the last write of each symmetric pair, as the reference's loop leaves it
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+      for (int z = 0; z < 3; ++z) {
+        double t = dK[x][sym(y, z)];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          t -= Gu[d][sym(x, y)] * Kd[sym(d, z)];
+          t -= Gu[d][sym(x, z)] * Kd[sym(y, d)];
+        }
+        DK[x][sym(y, z)] = t;
+      }
+  // tetrad
+  const double xc = B.x0[0] + i * B.dx[0], yc = B.x0[1] + j * B.dx[1], zc = B.x0[2] + k * B.dx[2];
+  double xx = xc;
+  if (xc * xc + yc * yc < 1e-10) xx = xx + 1e-8;
+  double uv[3] = {xx, yc, zc}, vv[3] = {xx * zc, yc * zc, -(xx * xx) - yc * yc},
+         wv[3] = {yc * -1.0, xx, 0.0};
+  auto dot = [&](const double* q, const double* r) {
+    double t = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = 0; y < 3; ++y) t += gd[sym(x, y)] * q[x] * r[y];
+    return t;
+  };
+  {
+    const double dp = dot(wv, wv);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) wv[x] = wv[x] / sqrt(dp);
+  }
+  {
+    const double dp = dot(wv, uv);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) uv[x] -= dp * wv[x];
+  }
+  {
+    const double dp = dot(uv, uv);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) uv[x] = uv[x] / sqrt(dp);
+  }
+  {
+    const double dp1 = dot(wv, vv), dp2 = dot(uv, vv);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) vv[x] -= dp1 * wv[x] + dp2 * uv[x];
+  }
+  {
+    const double dp = dot(vv, vv);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) vv[x] = vv[x] / sqrt(dp);
+  }
+  double rp = 0.0, ip = 0.0;
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+      double R4 = Rd[sym(x, y)] + K * Kd[sym(x, y)];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) R4 -= gu[sym(c, d)] * Kd[sym(x, c)] * Kd[sym(d, y)];
+      const double Tr = vv[x] * vv[y] - wv[x] * wv[y];
+      const double Ti = -vv[x] * wv[y] - wv[x] * vv[y];
+      rp += -FR4 * R4 * Tr;
+      ip += -FR4 * R4 * Ti;
+    }
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double R4 = -(DK[y][sym(x, c)] - DK[c][sym(x, y)]);
+        const double Tr = vv[x] * vv[y] - wv[x] * wv[y];
+        const double Ti = -vv[x] * wv[y] - wv[x] * vv[y];
+        rp += 0.5 * R4 * uv[c] * Tr;
+        ip += 0.5 * R4 * uv[c] * Ti;
+      }
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double R4 = gd[sym(d, c)] * Rd[sym(x, y)] + gd[sym(x, y)] * Rd[sym(d, c)] -
+                            gd[sym(d, y)] * Rd[sym(x, c)] - gd[sym(x, c)] * Rd[sym(d, y)] -
+                            0.5 * R * (gd[sym(d, c)] * gd[sym(x, y)] - gd[sym(d, y)] * gd[sym(x, c)]) +
+                            Kd[sym(d, c)] * Kd[sym(x, y)] - Kd[sym(d, y)] * Kd[sym(x, c)];
+          const double Tr = vv[x] * vv[y] - wv[x] * wv[y];
+          const double Ti = -vv[x] * wv[y] - wv[x] * vv[y];
+          rp += -FR4 * R4 * uv[d] * uv[c] * Tr;
+          ip += -FR4 * R4 * uv[d] * uv[c] * Ti;
+        }
+  double* o = a.out + B.off * 2;
+  o[p] = rp;
+  o[N + p] = ip;
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint of the right-hand side (no counterpart in the reference, which has
 // no derivative code): linearisation, pointwise reverse mode, transposed
 // stencils, parameter-gradient reduction.  Cotangent c of nw is blockIdx.y
@@ -2768,6 +3028,23 @@ int z4c_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const 
   rc = check_vjp_args(c, p, nw, "rhs_vjp");
   if (rc) return rc;
   return launch_rhs_vjp(c, p, u, nw, w, ubar, pbar, 0, (hipStream_t)stream);
+}
+
+int z4c_weyl(z4c_ctx* c, const z4c_params* p, const double* u, const double* adm, double* weyl,
+             void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !adm || !weyl) return fail(Z4C_EINVAL, "weyl: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  rc = launch_prep(c, p, u, nullptr, s);
+  if (rc) return rc;
+  rc = fill(c, weyl, std::numeric_limits<double>::quiet_NaN(), 2 * c->total_pts, s);
+  if (rc) return rc;
+  WeylArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, adm, c->d1, weyl, p->chi_psi_power,
+             p->chi_div_floor};
+  hipLaunchKernelGGL(k_weyl, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
 }
 
 int z4c_hamiltonian_norm(z4c_ctx* c, const z4c_params* p, const double* u, double* J,

@@ -26,7 +26,7 @@ EXPORTS = [
     "z4c_set_timing", "z4c_get_timing", "z4c_history", "z4c_set_exchange", "z4c_exchange",
     "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads", "z4c_rk4_step_graph",
     "z4c_physical_bc", "z4c_rhs_vjp", "z4c_rk4_step_vjp", "z4c_hamiltonian_norm",
-    "z4c_hamiltonian_norm_vjp",
+    "z4c_hamiltonian_norm_vjp", "z4c_weyl",
 ]
 
 # parameter gradients of the adjoints (Z4C_PBAR_* of the header)
@@ -130,6 +130,7 @@ def load(path: str | None = None):
         "z4c_rk4_step_vjp": [vp, pp, d, vp, i, vp, vp, vp, vp],
         "z4c_hamiltonian_norm": [vp, pp, vp, C.POINTER(d), vp],
         "z4c_hamiltonian_norm_vjp": [vp, pp, vp, vp, vp],
+        "z4c_weyl": [vp, pp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

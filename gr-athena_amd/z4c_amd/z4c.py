@@ -185,6 +185,22 @@ class Z4c:
         check(self.L.z4c_adm_constraints(self._ctx, C.byref(self.params), _ptr(u), _ptr(adm),
                                          _ptr(mat), _ptr(con), _stream(stream)), "adm_constraints")
 
+    def Z4cWeyl(self, u_adm, u_mat, u_weyl, u, stream=None):
+        """Z4c::Z4cWeyl(u_adm, u_mat, u_weyl) (calculate_weyl_scalars.cpp:44)
+        with the state ``u`` explicit (the reference reads storage.u);
+        ``u_mat`` must be None (vacuum)."""
+        if u_mat is not None:
+            raise _capi.Z4cError("weyl: matter is not enabled in this build (vacuum only)")
+        self.weyl(u, u_adm, u_weyl, stream)
+
+    def weyl(self, u, adm, out=None, stream=None):
+        """Re / Im psi4 at interior vertices (NaN elsewhere) of state ``u``
+        with ADM variables ``adm`` (Z4cToADM of u); returns the 2-field buffer."""
+        out = self.empty(2) if out is None else out
+        check(self.L.z4c_weyl(self._ctx, C.byref(self.params), _ptr(u), _ptr(adm), _ptr(out),
+                              _stream(stream)), "weyl")
+        return out
+
     def AddZ4cRHS(self, rhs, wght, u_out, nvar=Z4C_NVAR, stream=None):
         check(self.L.z4c_add_rhs(self._ctx, nvar, _ptr(rhs), float(wght), _ptr(u_out),
                                  _stream(stream)), "add_rhs")

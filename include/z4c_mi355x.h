@@ -266,6 +266,16 @@ int z4c_autotune(z4c_ctx* ctx, const z4c_params* p, const double* u, int reps, v
 int z4c_stencil_threads(z4c_ctx* ctx, int block, int* threads);
 int z4c_set_stencil_threads(z4c_ctx* ctx, int block, int threads);
 
+/* Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): weyl (2 fields
+   per block, the state-buffer layout) receives Re psi4 and Im psi4 at the
+   interior vertices, NaN elsewhere (as the reference fills u_weyl).  Inputs:
+   the state u and its ADM variables adm (z4c_z4c_to_adm, 17 fields).  First
+   prepares the conformal derivatives (PrepareZ4cDerivatives), which the
+   reference has current when its Weyl task runs; the coordinate tetrad uses
+   the vertex coordinates x0 + i*dx. */
+int z4c_weyl(z4c_ctx* ctx, const z4c_params* p, const double* u, const double* adm,
+             double* weyl, void* stream);
+
 /* Passes of the kernel timer. */
 enum {
   Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */

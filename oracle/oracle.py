@@ -289,3 +289,14 @@ def rk4_step(o, g, u, dt, mat=None, between_stages=None):
             between_stages(cur)
         algconstr(o, g, cur)
     return cur, ks
+
+
+def weyl(o, g, u, adm=None):
+    """Z4c::Z4cWeyl: (2, nn3, nn2, nn1) array of Re, Im psi4 (interior; NaN
+    elsewhere).  The ADM variables default to Z4cToADM of u."""
+    if adm is None:
+        adm = z4c_to_adm(o, g, u)
+    d1, dd = prep_derivs(o, g, u)
+    out = zeros(2, g)
+    lib().oracle_weyl(C.byref(o), C.byref(g), _p(u), _p(adm), _p(d1), _p(dd), _p(out))
+    return out

@@ -1369,3 +1369,213 @@ void oracle_physical_bc(const oracle_geom* g, int nvar, double* u, const int* fa
   }
   (void)bjs;
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): Re/Im psi4 at  */
+/* the interior vertices from the ADM variables, the Z4c state and the      */
+/* prepared conformal derivatives (PrepareZ4cDerivatives, d1 and ddchi of   */
+/* oracle_prep_derivs).  out: 2 fields (rpsi4, ipsi4), NaN outside the      */
+/* interior (u_weyl.Fill(NAN), :64-65).                                     */
+/* ------------------------------------------------------------------------ */
+void oracle_weyl(const oracle_params* o, const oracle_geom* g, const double* u,
+                 const double* adm, const double* d1, const double* ddchi, double* out) {
+  fdop f;
+  fd_init(g, &f);
+  const long N = (long)g->nn1 * g->nn2 * g->nn3;
+  for (long e = 0; e < 2 * N; ++e) out[e] = NAN;
+  const double pw = o->chi_psi_power;
+  const double FR4 = 0.25;
+  for (int k = g->kl; k <= g->ku; ++k)
+    for (int j = g->jl; j <= g->ju; ++j)
+      for (int i = g->il; i <= g->iu; ++i) {
+        const long p = i + (long)g->nn1 * (j + (long)g->nn2 * k);
+        const double chi_g = stdmax(u[OR_CHI * N + p], o->chi_div_floor);
+        double dchi[3], ddc[6], gt[6], dg3[3][6];
+        for (int a = 0; a < 3; ++a) dchi[a] = d1[(a * 11 + 1) * N + p];
+        for (int ab = 0; ab < 6; ++ab) {
+          ddc[ab] = ddchi[ab * N + p];
+          gt[ab] = u[(OR_GXX + ab) * N + p];
+        }
+        for (int c = 0; c < 3; ++c)
+          for (int ab = 0; ab < 6; ++ab) dg3[c][ab] = d1[(c * 11 + 5 + ab) * N + p];
+        /* physical metric derivatives by the chain rule (:136-262) */
+        double dg[3][6], dK[3][6], ddg[6][6]; /* ddg[sym(a,b)][sym(c,d)] */
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b) {
+              const int ab = S[a][b];
+              if (pw == -4.) {
+                const double oochi = 1.0 / chi_g;
+                dg[c][ab] = oochi * (dg3[c][ab] - oochi * gt[ab] * dchi[c]);
+              } else {
+                const double fac = 4.0 / pw;
+                const double psi4 = pow(chi_g, fac);
+                const double oochi = 1.0 / chi_g;
+                dg[c][ab] = psi4 * (dg3[c][ab] + fac * oochi * gt[ab] * dchi[c]);
+              }
+              dK[c][ab] = Dx(&f, adm + (OR_ADM_KXX + ab) * N + p, c);
+            }
+        for (int c = 0; c < 3; ++c)
+          for (int d = c; d < 3; ++d)
+            for (int a = 0; a < 3; ++a)
+              for (int b = a; b < 3; ++b) {
+                const int cd = S[c][d];
+                const double gtil_cd = gt[cd];
+                double v;
+                if (pw == -4.) {
+                  const double oochi = 1.0 / chi_g;
+                  if (a == b) {
+                    const double ddg_conf = Dxx(&f, u + (OR_GXX + cd) * N + p, a);
+                    v = oochi * (ddg_conf -
+                                 oochi * (2.0 * dchi[a] * dg3[a][cd] + ddc[S[a][a]] * gtil_cd) +
+                                 2.0 * oochi * oochi * dchi[a] * dchi[a] * gtil_cd);
+                  } else {
+                    const double ddg_conf = Dx(&f, d1 + (b * 11 + 5 + cd) * N + p, a);
+                    v = oochi * (ddg_conf -
+                                 oochi * (dchi[a] * dg3[b][cd] + dchi[b] * dg3[a][cd] +
+                                          ddc[S[a][b]] * gtil_cd) +
+                                 2.0 * oochi * oochi * dchi[a] * dchi[b] * gtil_cd);
+                  }
+                } else {
+                  const double fac = 4.0 / pw, fac2 = fac - 1.0;
+                  const double psi4 = pow(chi_g, fac);
+                  const double oochi = 1.0 / chi_g;
+                  if (a == b) {
+                    const double ddg_conf = Dxx(&f, u + (OR_GXX + cd) * N + p, a);
+                    v = psi4 * (ddg_conf +
+                                fac * oochi * (2.0 * dchi[a] * dg3[a][cd] + ddc[S[a][a]] * gtil_cd) +
+                                fac * fac2 * oochi * oochi * dchi[a] * dchi[a] * gtil_cd);
+                  } else {
+                    const double ddg_conf = Dx(&f, d1 + (b * 11 + 5 + cd) * N + p, a);
+                    v = psi4 * (ddg_conf +
+                                fac * oochi *
+                                    (dchi[a] * dg3[b][cd] + dchi[b] * dg3[a][cd] +
+                                     ddc[S[a][b]] * gtil_cd) +
+                                fac * fac2 * oochi * oochi * dchi[a] * dchi[b] * gtil_cd);
+                  }
+                }
+                ddg[S[a][b]][cd] = v;
+              }
+        /* inverse metric, Christoffels (:264-306) */
+        double gd[6], Kd[6], gu[6];
+        for (int ab = 0; ab < 6; ++ab) {
+          gd[ab] = adm[(OR_ADM_GXX + ab) * N + p];
+          Kd[ab] = adm[(OR_ADM_KXX + ab) * N + p];
+        }
+        inv3(1. / det3v(gd), gd, gu);
+        double Gd[3][6], Gu[3][6];
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b)
+              Gd[c][S[a][b]] = 0.5 * (dg[a][S[b][c]] + dg[b][S[a][c]] - dg[c][S[a][b]]);
+        for (int c = 0; c < 3; ++c)
+          for (int ab = 0; ab < 6; ++ab) Gu[c][ab] = 0.0;
+        for (int c = 0; c < 3; ++c)
+          for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b)
+              for (int d = 0; d < 3; ++d) Gu[c][S[a][b]] += gu[S[c][d]] * Gd[d][S[a][b]];
+        /* Ricci tensor and scalar (:308-347) */
+        double Rd[6] = {0, 0, 0, 0, 0, 0}, R = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = a; b < 3; ++b) {
+            for (int c = 0; c < 3; ++c)
+              for (int d = 0; d < 3; ++d) {
+                for (int e = 0; e < 3; ++e) {
+                  Rd[S[a][b]] += gu[S[c][d]] * Gu[e][S[a][c]] * Gd[e][S[b][d]];
+                  Rd[S[a][b]] -= gu[S[c][d]] * Gu[e][S[a][b]] * Gd[e][S[c][d]];
+                }
+                Rd[S[a][b]] += 0.5 * gu[S[c][d]] *
+                               (-ddg[S[c][d]][S[a][b]] - ddg[S[a][b]][S[c][d]] +
+                                ddg[S[a][c]][S[b][d]] + ddg[S[b][c]][S[a][d]]);
+              }
+            R += ((a == b) ? 1.0 : 2.0) * gu[S[a][b]] * Rd[S[a][b]];
+          }
+        /* K traces and covariant derivative (:349-390) */
+        double Kud[3][3], K = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          for (int b = 0; b < 3; ++b) {
+            Kud[a][b] = 0.0;
+            for (int c = 0; c < 3; ++c) Kud[a][b] += gu[S[a][c]] * Kd[S[c][b]];
+          }
+          K += Kud[a][a];
+        }
+        double DK[3][6]; /* DK[a][sym(b,c)]: the value of the last write, b >= c */
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+              double t = dK[a][S[b][c]];
+              for (int d = 0; d < 3; ++d) {
+                t -= Gu[d][S[a][b]] * Kd[S[d][c]];
+                t -= Gu[d][S[a][c]] * Kd[S[b][d]];
+              }
+              DK[a][S[b][c]] = t;
+            }
+        /* tetrad (:392-512) */
+        const double x = g->x0[0] + i * g->dx[0], y = g->x0[1] + j * g->dx[1],
+                     z = g->x0[2] + k * g->dx[2];
+        double xx = x;
+        if (x * x + y * y < 1e-10) xx = xx + 1e-8;
+        double uv[3] = {xx, y, z}, vv[3] = {xx * z, y * z, -(xx * xx) - y * y},
+               wv[3] = {y * -1.0, xx, 0.0};
+        double dp = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp += gd[S[a][b]] * wv[a] * wv[b];
+        for (int a = 0; a < 3; ++a) wv[a] = wv[a] / sqrt(dp);
+        dp = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp += gd[S[a][b]] * wv[a] * uv[b];
+        for (int a = 0; a < 3; ++a) uv[a] -= dp * wv[a];
+        dp = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp += gd[S[a][b]] * uv[a] * uv[b];
+        for (int a = 0; a < 3; ++a) uv[a] = uv[a] / sqrt(dp);
+        dp = 0.0;
+        double dp2 = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp += gd[S[a][b]] * wv[a] * vv[b];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp2 += gd[S[a][b]] * uv[a] * vv[b];
+        for (int a = 0; a < 3; ++a) vv[a] -= dp * wv[a] + dp2 * uv[a];
+        dp = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) dp += gd[S[a][b]] * vv[a] * vv[b];
+        for (int a = 0; a < 3; ++a) vv[a] = vv[a] / sqrt(dp);
+        /* psi4 (:514-628) */
+        double rp = 0.0, ip = 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            double R4 = Rd[S[a][b]] + K * Kd[S[a][b]];
+            for (int c = 0; c < 3; ++c)
+              for (int d = 0; d < 3; ++d) R4 -= gu[S[c][d]] * Kd[S[a][c]] * Kd[S[d][b]];
+            const double Tr = vv[a] * vv[b] - wv[a] * wv[b];
+            const double Ti = -vv[a] * wv[b] - wv[a] * vv[b];
+            rp += -FR4 * R4 * Tr;
+            ip += -FR4 * R4 * Ti;
+          }
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) {
+              const double R4 = -(DK[b][S[a][c]] - DK[c][S[a][b]]);
+              const double Tr = vv[a] * vv[b] - wv[a] * wv[b];
+              const double Ti = -vv[a] * wv[b] - wv[a] * vv[b];
+              rp += 0.5 * R4 * uv[c] * Tr;
+              ip += 0.5 * R4 * uv[c] * Ti;
+            }
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c)
+              for (int d = 0; d < 3; ++d) {
+                const double R4 = gd[S[d][c]] * Rd[S[a][b]] + gd[S[a][b]] * Rd[S[d][c]] -
+                                  gd[S[d][b]] * Rd[S[a][c]] - gd[S[a][c]] * Rd[S[d][b]] -
+                                  0.5 * R * (gd[S[d][c]] * gd[S[a][b]] - gd[S[d][b]] * gd[S[a][c]]) +
+                                  Kd[S[d][c]] * Kd[S[a][b]] - Kd[S[d][b]] * Kd[S[a][c]];
+                const double Tr = vv[a] * vv[b] - wv[a] * wv[b];
+                const double Ti = -vv[a] * wv[b] - wv[a] * vv[b];
+                rp += -FR4 * R4 * uv[d] * uv[c] * Tr;
+                ip += -FR4 * R4 * uv[d] * uv[c] * Ti;
+              }
+        out[p] = rp;
+        out[N + p] = ip;
+      }
+}

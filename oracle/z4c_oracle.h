@@ -157,6 +157,12 @@ void oracle_physical_bc(const oracle_geom* g, int nvar, double* u, const int* fa
 /* Single-block periodic vertex-centred ghost fill (all three axes). */
 void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u);
 
+/* Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): out[2][N] =
+   Re, Im psi4 at interior vertices (NaN elsewhere) from the ADM variables
+   adm[17][N], the state and the prepared derivatives of oracle_prep_derivs. */
+void oracle_weyl(const oracle_params* o, const oracle_geom* g, const double* u,
+                 const double* adm, const double* d1, const double* ddchi, double* out);
+
 #ifdef __cplusplus
 }
 #endif
