@@ -247,6 +247,8 @@ def main():
                        "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)}
     out["passes"] = pinfo
 
+    if not args.no_adjoint:
+        out["diagnostics"] = diagnostics_cost(z, u)
     if args.calib_copy:
         tmp = torch.empty_like(u)
         tmp.copy_(u)
@@ -266,6 +268,36 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def diagnostics_cost(z, u, reps=3):
+    """Output-cadence kernels on the same 2048-block state: Z4cToADM and the
+    Weyl scalar psi4 (with its PrepareZ4cDerivatives), ADMConstraints (with
+    its own prepare), timed with events on the launch stream."""
+    import torch
+    adm = z.zeros(17)
+    con = z.zeros(7)
+    w = z.empty(2)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    res = {"z4c_to_adm_ms": timed(lambda: z.Z4cToADM(u, adm)),
+           "weyl_ms": timed(lambda: z.weyl(u, adm, w)),
+           "adm_constraints_ms": timed(lambda: z.ADMConstraints(con, adm, None, u))}
+    if not np.isfinite(z.download(w, 2)[0][:, 4:-4, 4:-4, 4:-4]).all():
+        raise RuntimeError("non-finite psi4 on the workload")
+    log("diagnostics: " + ", ".join(f"{k} {v:.2f}" for k, v in res.items()))
+    del adm, con, w
+    return res
 
 
 def sensitivity_run(adm_fn, params, nsteps=100, window=10):
