@@ -79,6 +79,23 @@ class Geom(C.Structure):
             g.x0[d] = float(xmin[d]) - NGHOST * float(dx[d])
         return g
 
+    def coarse(self):
+        """Coarse VC geometry of this block (mesh_refinement_vc.cpp with
+        NCGHOST = 4): per axis nx/2 + 9 vertices, interior 4 .. 4 + nx/2,
+        spacing 2 dx, coarse vertex 4 at the block's first interior vertex."""
+        c = Geom()
+        nx = [self.iu - self.il, self.ju - self.jl, self.ku - self.kl]
+        nn = [n // 2 + 9 for n in nx]
+        c.nn1, c.nn2, c.nn3 = nn
+        c.ng = 4
+        c.il, c.iu = 4, 4 + nx[0] // 2
+        c.jl, c.ju = 4, 4 + nx[1] // 2
+        c.kl, c.ku = 4, 4 + nx[2] // 2
+        for d in range(3):
+            c.dx[d] = 2.0 * self.dx[d]
+            c.x0[d] = self.x0[d] + NGHOST * self.dx[d] - 4 * c.dx[d]
+        return c
+
     @property
     def shape(self):
         return (self.nn3, self.nn2, self.nn1)
@@ -300,3 +317,37 @@ def weyl(o, g, u, adm=None):
     out = zeros(2, g)
     lib().oracle_weyl(C.byref(o), C.byref(g), _p(u), _p(adm), _p(d1), _p(dd), _p(out))
     return out
+
+
+def restrict_vc(g, fine):
+    """RestrictInterior for VC data (injection onto the coarse interior):
+    returns the coarse buffer (NaN outside the coarse interior)."""
+    cg = g.coarse()
+    nvar = fine.shape[0]
+    coarse = np.full((nvar,) + cg.shape, np.nan)
+    lib().oracle_restrict_vc(C.byref(g), C.byref(cg), nvar, _p(fine), _p(coarse))
+    return coarse
+
+
+def prolongate_vc(g, coarse, fine, rng):
+    """ProlongateVertexCenteredValues over the coarse index box
+    rng = (si, ei, sj, ej, sk, ek), in place on ``fine``."""
+    cg = g.coarse()
+    lib().oracle_prolongate_vc(C.byref(g), C.byref(cg), fine.shape[0], _p(coarse), _p(fine),
+                               *[int(v) for v in rng])
+    return fine
+
+
+def prolongation_range(g, ox, parity):
+    """Coarse index box of the prolongation for a coarser neighbour at offset
+    ox = (ox1, ox2, ox3), block logical-coordinate parities ``parity``."""
+    cg = g.coarse()
+    lo = (cg.il, cg.jl, cg.kl)
+    hi = (cg.iu, cg.ju, cg.ku)
+    out = []
+    for a in range(3):
+        s_, e_ = C.c_int(), C.c_int()
+        lib().oracle_prolongation_range(int(parity[a]), int(ox[a]), lo[a], hi[a], C.byref(s_),
+                                        C.byref(e_))
+        out += [s_.value, e_.value]
+    return tuple(out)

@@ -1579,3 +1579,199 @@ void oracle_weyl(const oracle_params* o, const oracle_geom* g, const double* u,
         out[N + p] = ip;
       }
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* VC restriction and prolongation (src/mesh/mesh_refinement_vc.cpp), 3-D,  */
+/* NGHOST = 4, NCGHOST = 4 (configure.py:876-881 default for nghost 4).     */
+/* Coarse buffer per axis: nx/2 + 2*NCGHOST + 1 vertices, coarse interior   */
+/* civs = NCGHOST .. cive = civs + nx/2, midpoint cimp = civs + (nx/2)/2    */
+/* (meshblock.cpp:611-640, 851-895).  Fine indices ivs = NGHOST, ive.       */
+/* ------------------------------------------------------------------------ */
+#define OR_NCG 4
+#define OR_HNCG ((2 * OR_NCG - 4) / 2) /* H_NCGHOST, :305 */
+#define OR_HSZ (OR_HNCG + 1)           /* H_SZ, :307 */
+/* InterpolateLagrangeUniform<3>::coeff (interp_univariate_stencils.tpp:99-101) */
+static const double LAG3[3] = {3. / 256., -25. / 256., 75. / 128.};
+
+/* RestrictVertexCenteredIndicialHelper (:24-45) */
+static int restrict_fine_index(int c, int cvs, int cve, int vs, int ve) {
+  if (c < cvs) return vs - 2 * (cvs - c);
+  if (c > cve) return ve + 2 * (c - cve);
+  return 2 * (c - cvs) + vs;
+}
+
+/* RestrictVertexCenteredValues (:56-100) over the coarse interior, as
+   RestrictInterior calls it (comm/refinement_ops.cpp:178-205). */
+void oracle_restrict_vc(const oracle_geom* g, const oracle_geom* cg, int nvar,
+                        const double* fine, double* coarse) {
+  const long N = (long)g->nn1 * g->nn2 * g->nn3, CN = (long)cg->nn1 * cg->nn2 * cg->nn3;
+  for (int n = 0; n < nvar; ++n)
+    for (int ck = cg->kl; ck <= cg->ku; ++ck) {
+      const int k = restrict_fine_index(ck, cg->kl, cg->ku, g->kl, g->ku);
+      for (int cj = cg->jl; cj <= cg->ju; ++cj) {
+        const int j = restrict_fine_index(cj, cg->jl, cg->ju, g->jl, g->ju);
+        for (int ci = cg->il; ci <= cg->iu; ++ci) {
+          const int i = restrict_fine_index(ci, cg->il, cg->iu, g->il, g->iu);
+          coarse[n * CN + ci + (long)cg->nn1 * (cj + (long)cg->nn2 * ck)] =
+              fine[n * N + i + (long)g->nn1 * (j + (long)g->nn2 * k)];
+        }
+      }
+    }
+}
+
+/* ProlongateVertexCenteredValues (:283-772), 3-D branch, even NGHOST
+   (eo_offset = 0, injection ranges = si..ei), DBG_SYMMETRIZE_P_OP and
+   DBG_SYMMETRIZE_CHEAP on (defs.hpp.in:17,20): the corner sums are
+   FloatingPoint::sum_corners (utils/floating_point.hpp:44-55) with the
+   argument orders of each case.  cimp/cjmp/ckmp: coarse midpoints. */
+static double sc8(double a, double b, double c, double d, double e, double f, double g8, double h) {
+  return (((a + b) + (c + d)) + ((e + f) + (g8 + h)));
+}
+static double sc4(double a, double b, double c, double d) { return ((a + b) + (c + d)); }
+
+void oracle_prolongate_vc(const oracle_geom* g, const oracle_geom* cg, int nvar,
+                          const double* coarse, double* fine, int si, int ei, int sj,
+                          int ej, int sk, int ek) {
+  const long N = (long)g->nn1 * g->nn2 * g->nn3, CN = (long)cg->nn1 * cg->nn2 * cg->nn3;
+  const int cimp = cg->il + (cg->iu - cg->il) / 2, cjmp = cg->jl + (cg->ju - cg->jl) / 2,
+            ckmp = cg->kl + (cg->ku - cg->kl) / 2;
+#define CO(n, k, j, i) coarse[(n) * CN + (i) + (long)cg->nn1 * ((j) + (long)cg->nn2 * (k))]
+#define FI(n, k, j, i) fine[(n) * N + (i) + (long)g->nn1 * ((j) + (long)g->nn2 * (k))]
+  const int si_prl = (si > cimp) ? si - 1 : si, ei_prl = (ei < cimp) ? ei + 1 : ei;
+  const int sj_prl = (sj > cjmp) ? sj - 1 : sj, ej_prl = (ej < cjmp) ? ej + 1 : ej;
+  const int sk_prl = (sk > ckmp) ? sk - 1 : sk, ek_prl = (ek < ckmp) ? ek + 1 : ek;
+  const int H = OR_HNCG;
+  for (int n = 0; n < nvar; ++n) {
+    /* (prl, prl, prl) */
+    for (int k = sk_prl; k < ek_prl; ++k)
+      for (int j = sj_prl; j < ej_prl; ++j)
+        for (int i = si_prl; i < ei_prl; ++i) {
+          const int fk = 2 * (k - H) + 1, fj = 2 * (j - H) + 1, fi = 2 * (i - H) + 1;
+          double acc = 0.;
+          for (int dk = 0; dk < OR_HSZ; ++dk) {
+            const int ku = k + dk + 1, kl = k - dk;
+            const double lck = LAG3[OR_HSZ - dk - 1];
+            for (int dj = 0; dj < OR_HSZ; ++dj) {
+              const int ju = j + dj + 1, jl = j - dj;
+              const double lckj = lck * LAG3[OR_HSZ - dj - 1];
+              for (int di = 0; di < OR_HSZ; ++di) {
+                const int iu = i + di + 1, il = i - di;
+                const double w = lckj * LAG3[OR_HSZ - di - 1];
+                acc += w * sc8(CO(n, ku, ju, iu), CO(n, kl, jl, il), CO(n, ku, ju, il),
+                               CO(n, kl, jl, iu), CO(n, kl, ju, il), CO(n, ku, jl, iu),
+                               CO(n, kl, ju, iu), CO(n, ku, jl, il));
+              }
+            }
+          }
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (inj, prl, prl) */
+    for (int k = sk; k <= ek; ++k)
+      for (int j = sj_prl; j < ej_prl; ++j)
+        for (int i = si_prl; i < ei_prl; ++i) {
+          const int fk = 2 * (k - H), fj = 2 * (j - H) + 1, fi = 2 * (i - H) + 1;
+          double acc = 0.;
+          for (int dj = 0; dj < OR_HSZ; ++dj) {
+            const int ju = j + dj + 1, jl = j - dj;
+            const double lcj = LAG3[OR_HSZ - dj - 1];
+            for (int di = 0; di < OR_HSZ; ++di) {
+              const int iu = i + di + 1, il = i - di;
+              const double w = lcj * LAG3[OR_HSZ - di - 1];
+              acc += w * sc4(CO(n, k, ju, iu), CO(n, k, jl, il), CO(n, k, jl, iu), CO(n, k, ju, il));
+            }
+          }
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (prl, inj, prl) */
+    for (int k = sk_prl; k < ek_prl; ++k)
+      for (int j = sj; j <= ej; ++j)
+        for (int i = si_prl; i < ei_prl; ++i) {
+          const int fk = 2 * (k - H) + 1, fj = 2 * (j - H), fi = 2 * (i - H) + 1;
+          double acc = 0.;
+          for (int dk = 0; dk < OR_HSZ; ++dk) {
+            const int ku = k + dk + 1, kl = k - dk;
+            const double lck = LAG3[OR_HSZ - dk - 1];
+            for (int di = 0; di < OR_HSZ; ++di) {
+              const int iu = i + di + 1, il = i - di;
+              const double w = lck * LAG3[OR_HSZ - di - 1];
+              acc += w * sc4(CO(n, kl, j, iu), CO(n, ku, j, il), CO(n, ku, j, iu), CO(n, kl, j, il));
+            }
+          }
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (prl, prl, inj) */
+    for (int k = sk_prl; k < ek_prl; ++k)
+      for (int j = sj_prl; j < ej_prl; ++j)
+        for (int i = si; i <= ei; ++i) {
+          const int fk = 2 * (k - H) + 1, fj = 2 * (j - H) + 1, fi = 2 * (i - H);
+          double acc = 0.;
+          for (int dk = 0; dk < OR_HSZ; ++dk) {
+            const int ku = k + dk + 1, kl = k - dk;
+            const double lck = LAG3[OR_HSZ - dk - 1];
+            for (int dj = 0; dj < OR_HSZ; ++dj) {
+              const int ju = j + dj + 1, jl = j - dj;
+              const double w = lck * LAG3[OR_HSZ - dj - 1];
+              acc += w * sc4(CO(n, ku, ju, i), CO(n, kl, jl, i), CO(n, kl, ju, i), CO(n, ku, jl, i));
+            }
+          }
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (inj, inj, prl) */
+    for (int k = sk; k <= ek; ++k)
+      for (int j = sj; j <= ej; ++j)
+        for (int i = si_prl; i < ei_prl; ++i) {
+          const int fk = 2 * (k - H), fj = 2 * (j - H), fi = 2 * (i - H) + 1;
+          double acc = 0.;
+          for (int di = 0; di < OR_HSZ; ++di)
+            acc += LAG3[OR_HSZ - di - 1] * (CO(n, k, j, i - di) + CO(n, k, j, i + di + 1));
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (inj, prl, inj) */
+    for (int k = sk; k <= ek; ++k)
+      for (int j = sj_prl; j < ej_prl; ++j)
+        for (int i = si; i <= ei; ++i) {
+          const int fk = 2 * (k - H), fj = 2 * (j - H) + 1, fi = 2 * (i - H);
+          double acc = 0.;
+          for (int dj = 0; dj < OR_HSZ; ++dj)
+            acc += LAG3[OR_HSZ - dj - 1] * (CO(n, k, j - dj, i) + CO(n, k, j + dj + 1, i));
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (prl, inj, inj) */
+    for (int k = sk_prl; k < ek_prl; ++k)
+      for (int j = sj; j <= ej; ++j)
+        for (int i = si; i <= ei; ++i) {
+          const int fk = 2 * (k - H) + 1, fj = 2 * (j - H), fi = 2 * (i - H);
+          double acc = 0.;
+          for (int dk = 0; dk < OR_HSZ; ++dk)
+            acc += LAG3[OR_HSZ - dk - 1] * (CO(n, k - dk, j, i) + CO(n, k + dk + 1, j, i));
+          FI(n, fk, fj, fi) = acc;
+        }
+    /* (inj, inj, inj) */
+    for (int k = sk; k <= ek; ++k)
+      for (int j = sj; j <= ej; ++j)
+        for (int i = si; i <= ei; ++i)
+          FI(n, 2 * (k - H), 2 * (j - H), 2 * (i - H)) = CO(n, k, j, i);
+  }
+#undef CO
+#undef FI
+}
+
+/* ProlongationRange (comm/refinement_ops.hpp:48-72) for VC: pcng = ng/2. */
+void oracle_prolongation_range(int parity, int ox, int cvs, int cve, int* s, int* e) {
+  const int pcng = 4 / 2;
+  if (ox > 0) {
+    *s = cve + 1;
+    *e = cve + pcng;
+  } else if (ox < 0) {
+    *s = cvs - pcng;
+    *e = cvs - 1;
+  } else {
+    *s = cvs;
+    *e = cve;
+    if ((parity & 1) == 0)
+      *e += pcng;
+    else
+      *s -= pcng;
+  }
+}

@@ -163,6 +163,21 @@ void oracle_periodic_fill(const oracle_geom* g, int nvar, double* u);
 void oracle_weyl(const oracle_params* o, const oracle_geom* g, const double* u,
                  const double* adm, const double* d1, const double* ddchi, double* out);
 
+/* VC refinement (src/mesh/mesh_refinement_vc.cpp; comm/refinement_ops.*):
+   cg is the coarse geometry of block g (oracle.Geom.coarse): per axis
+   nx/2 + 9 vertices, interior 4 .. 4 + nx/2.  Restriction is
+   RestrictVertexCenteredValues over the coarse interior (RestrictInterior);
+   prolongation is ProlongateVertexCenteredValues (3-D, even NGHOST) over the
+   coarse index box [si..ei] x [sj..ej] x [sk..ek]; prolongation_range is
+   ProlongationRange (VC) for one axis of a coarser neighbour at offset ox
+   (parity: the block's logical coordinate & 1). */
+void oracle_restrict_vc(const oracle_geom* g, const oracle_geom* cg, int nvar,
+                        const double* fine, double* coarse);
+void oracle_prolongate_vc(const oracle_geom* g, const oracle_geom* cg, int nvar,
+                          const double* coarse, double* fine, int si, int ei, int sj,
+                          int ej, int sk, int ek);
+void oracle_prolongation_range(int parity, int ox, int cvs, int cve, int* s, int* e);
+
 #ifdef __cplusplus
 }
 #endif
