@@ -24,6 +24,7 @@
 #include "z4c_adjoint.hpp"
 #include "z4c_device.hpp"
 #include "z4c_mi355x.h"
+#include "z4c_refine.hpp"
 #include "z4c_rhs_point.hpp"
 #include "z4c_stencil.hpp"
 
@@ -1179,6 +1180,91 @@ __global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Vertex-centred restriction and prolongation between levels
+// (src/mesh/mesh_refinement_vc.cpp; the index algebra and the per-vertex
+// operator are in z4c_refine.hpp).  Both kernels take a host-built chunk
+// table: workgroup w handles points start .. start+255 of entry ent (a block
+// for the restriction, a prolongation box for the prolongation), one thread
+// per vertex, all fields.
+// ---------------------------------------------------------------------------
+struct RfBlock {
+  long long coff;     // point offset of the block's coarse buffer
+  int cn1, cn2, cn3;  // coarse vertex counts per axis (nx/2 + 9)
+  int cnpts;          // cn1*cn2*cn3
+  int cni, cnj, cnk;  // coarse interior vertex counts (nx/2 + 1)
+};
+
+struct RfChunk {
+  int ent;
+  int start;
+};
+
+struct RestrictArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const RfChunk* chunks;
+  int nvar;
+  const double* u;
+  double* coarse;
+};
+
+// RestrictInterior (src/comm/refinement_ops.cpp:178-205) ->
+// RestrictVertexCenteredValues (src/mesh/mesh_refinement_vc.cpp:56-100):
+// injection of the fine vertex at each coarse interior vertex.
+__global__ void __launch_bounds__(256) k_restrict_vc(RestrictArgs a) {
+  const RfChunk ch = a.chunks[blockIdx.x];
+  const RfBlock R = a.cb[ch.ent];
+  const int t = ch.start + (int)threadIdx.x;
+  if (t >= R.cni * R.cnj * R.cnk) return;
+  const DevBlock& B = a.blocks[ch.ent];
+  const int ci = RF_NCG + t % R.cni, cj = RF_NCG + (t / R.cni) % R.cnj,
+            ck = RF_NCG + t / (R.cni * R.cnj);
+  const int i = rf_fine_of(ci, RF_NCG, RF_NCG + R.cni - 1, B.il, B.iu);
+  const int j = rf_fine_of(cj, RF_NCG, RF_NCG + R.cnj - 1, B.jl, B.ju);
+  const int k = rf_fine_of(ck, RF_NCG, RF_NCG + R.cnk - 1, B.kl, B.ku);
+  const long long p = i + (long long)B.nn1 * (j + (long long)B.nn2 * k);
+  const long long cp = ci + (long long)R.cn1 * (cj + (long long)R.cn2 * ck);
+  const double* src = a.u + B.off * a.nvar + p;
+  double* dst = a.coarse + R.coff * a.nvar + cp;
+  for (int n = 0; n < a.nvar; ++n) dst[(long long)n * R.cnpts] = src[(long long)n * B.npts];
+}
+
+struct ProlongArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const RfEntry* ent;
+  const RfChunk* chunks;
+  int nvar;
+  const double* coarse;
+  double* u;
+};
+
+// ProlongateVertexCenteredValues (src/mesh/mesh_refinement_vc.cpp:283-772)
+// over one coarse index box per entry.  A fine vertex's value depends only on
+// the coarse buffer and its (coarse index, case), so entries whose fine boxes
+// overlap store the same value there.
+__global__ void __launch_bounds__(256) k_prolong_vc(ProlongArgs a) {
+  const RfChunk ch = a.chunks[blockIdx.x];
+  const RfEntry E = a.ent[ch.ent];
+  const int t = ch.start + (int)threadIdx.x;
+  if (t >= E.fn[0] * E.fn[1] * E.fn[2]) return;
+  const int fi = E.f0[0] + t % E.fn[0], fj = E.f0[1] + (t / E.fn[0]) % E.fn[1],
+            fk = E.f0[2] + t / (E.fn[0] * E.fn[1]);
+  int ci, cj, ck, pi, pj, pk;
+  if (!rf_axis(E, 0, fi, ci, pi) || !rf_axis(E, 1, fj, cj, pj) || !rf_axis(E, 2, fk, ck, pk))
+    return;
+  const int m = pi | (pj << 1) | (pk << 2);
+  const DevBlock& B = a.blocks[E.blk];
+  const RfBlock R = a.cb[E.blk];
+  const long long p = fi + (long long)B.nn1 * (fj + (long long)B.nn2 * fk);
+  const double* C = a.coarse + R.coff * a.nvar;
+  const long s2 = R.cn1, s3 = (long)R.cn1 * R.cn2;
+  double* dst = a.u + B.off * a.nvar + p;
+  for (int n = 0; n < a.nvar; ++n)
+    dst[(long long)n * B.npts] = rf_prolong_point(C + (long long)n * R.cnpts, s2, s3, ci, cj, ck, m);
+}
+
+// ---------------------------------------------------------------------------
 // Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): Re / Im psi4 at
 // interior vertices, one thread per vertex, in the reference's operation
 // order: physical metric derivatives by the chain rule from the prepared
@@ -1751,6 +1837,16 @@ struct z4c_ctx {
   int* ex_off = nullptr;
   int* ex_src = nullptr;
   int ex_ndst = 0;
+  // refinement (lazy): coarse-buffer table and restriction chunks, and the
+  // prolongation entries set by z4c_set_prolongation with their chunks
+  std::vector<RfBlock> hrf;
+  long long rf_total = 0;
+  RfBlock* d_rf = nullptr;
+  RfChunk* d_rchunks = nullptr;
+  int n_rchunks = 0;
+  RfEntry* d_pent = nullptr;
+  RfChunk* d_pchunks = nullptr;
+  int n_pchunks = 0;
   std::vector<double> hst_host;
   int n_cu = 256;
   // captured fused step (z4c_rk4_step_graph) and what it was captured for
@@ -1785,6 +1881,25 @@ struct z4c_ctx {
       hipEventDestroy(t.b);
     }
     tev.clear();
+  }
+
+  void drop_prolongation() {
+    hipFree(d_pent);
+    hipFree(d_pchunks);
+    d_pent = nullptr;
+    d_pchunks = nullptr;
+    n_pchunks = 0;
+  }
+
+  void drop_refine() {
+    drop_prolongation();
+    hipFree(d_rf);
+    hipFree(d_rchunks);
+    d_rf = nullptr;
+    d_rchunks = nullptr;
+    n_rchunks = 0;
+    hrf.clear();
+    rf_total = 0;
   }
 
   // the captured step holds device pointers of the block table and plans
@@ -1825,6 +1940,7 @@ struct z4c_ctx {
     hipFree(ex_src);
     ex_dst = ex_off = ex_src = nullptr;
     ex_ndst = 0;
+    drop_refine();
     hst_host.clear();
     double** adj[6] = {&adj_lin, &adj_sb, &adj_part, &adj_k, &adj_stage, &adj_reg};
     for (double** q : adj) {
@@ -2756,6 +2872,189 @@ int z4c_exchange(z4c_ctx* c, int nvar, double* u, void* stream) {
   if (rc) return rc;
   if (!u || nvar <= 0) return fail(Z4C_EINVAL, "exchange: bad argument");
   return launch_exchange(c, nvar, u, (hipStream_t)stream);
+}
+
+namespace {
+
+// Coarse-buffer table of the registered blocks (built on first use).
+int ensure_coarse(z4c_ctx* c) {
+  if (c->d_rf) return Z4C_OK;
+  std::vector<RfBlock> hr(c->nblocks);
+  std::vector<RfChunk> ch;
+  long long off = 0;
+  for (int b = 0; b < c->nblocks; ++b) {
+    const z4c_block& in = c->hblocks[b];
+    const int nx[3] = {in.iu - in.il, in.ju - in.jl, in.ku - in.kl};
+    int cn[3], ni[3];
+    for (int d = 0; d < 3; ++d) {
+      if (nx[d] % 2 != 0)
+        return fail(Z4C_EINVAL, "refinement: block " + std::to_string(b) +
+                                    " needs an even number of cells per axis");
+      cn[d] = nx[d] / 2 + 2 * RF_NCG + 1;
+      ni[d] = nx[d] / 2 + 1;
+    }
+    RfBlock& R = hr[b];
+    R.coff = off;
+    R.cn1 = cn[0];
+    R.cn2 = cn[1];
+    R.cn3 = cn[2];
+    R.cnpts = cn[0] * cn[1] * cn[2];
+    R.cni = ni[0];
+    R.cnj = ni[1];
+    R.cnk = ni[2];
+    off += R.cnpts;
+    const int np = ni[0] * ni[1] * ni[2];
+    for (int s0 = 0; s0 < np; s0 += 256) ch.push_back(RfChunk{b, s0});
+  }
+  if (hipMalloc(&c->d_rf, sizeof(RfBlock) * hr.size()) != hipSuccess ||
+      hipMalloc(&c->d_rchunks, sizeof(RfChunk) * ch.size()) != hipSuccess) {
+    c->drop_refine();
+    return fail(Z4C_ENOMEM, "refinement: allocation failed");
+  }
+  HIPCALL(hipMemcpy(c->d_rf, hr.data(), sizeof(RfBlock) * hr.size(), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(c->d_rchunks, ch.data(), sizeof(RfChunk) * ch.size(), hipMemcpyHostToDevice));
+  c->n_rchunks = (int)ch.size();
+  c->hrf = hr;
+  c->rf_total = off;
+  return Z4C_OK;
+}
+
+// One prolongation box checked against the block's fine and coarse extents:
+// every coarse vertex the 6-point stencils read and every fine vertex the
+// box writes must lie in the buffers.
+int make_entry(const z4c_ctx* c, const z4c_prolong_box& in, int q, RfEntry& E) {
+  const std::string who = "set_prolongation: box " + std::to_string(q);
+  if (in.block < 0 || in.block >= c->nblocks) return fail(Z4C_EINVAL, who + ": bad block index");
+  const z4c_block& B = c->hblocks[in.block];
+  const RfBlock& R = c->hrf[in.block];
+  const int s[3] = {in.si, in.sj, in.sk}, e[3] = {in.ei, in.ej, in.ek};
+  const int cn[3] = {R.cn1, R.cn2, R.cn3}, nn[3] = {B.nn1, B.nn2, B.nn3};
+  const int cni[3] = {R.cni, R.cnj, R.cnk};
+  E.blk = in.block;
+  for (int a = 0; a < 3; ++a) {
+    const int cmp = RF_NCG + (cni[a] - 1) / 2;  // coarse midpoint cimp
+    if (s[a] > e[a]) return fail(Z4C_EINVAL, who + ": empty range");
+    const int sp = s[a] > cmp ? s[a] - 1 : s[a], ep = e[a] < cmp ? e[a] + 1 : e[a];
+    int rlo = s[a], rhi = e[a];            // coarse vertices read
+    int flo = 2 * (s[a] - RF_H), fhi = 2 * (e[a] - RF_H);  // fine vertices written
+    if (sp < ep) {
+      rlo = std::min(rlo, sp - RF_H);
+      rhi = std::max(rhi, ep - 1 + RF_HSZ);
+      flo = std::min(flo, 2 * (sp - RF_H) + 1);
+      fhi = std::max(fhi, 2 * (ep - 1 - RF_H) + 1);
+    }
+    if (rlo < 0 || rhi >= cn[a] || flo < 0 || fhi >= nn[a])
+      return fail(Z4C_EINVAL, who + ": range leaves the coarse or fine buffer");
+    E.s[a] = s[a];
+    E.e[a] = e[a];
+    E.sp[a] = sp;
+    E.ep[a] = ep;
+    E.f0[a] = flo;
+    E.fn[a] = fhi - flo + 1;
+  }
+  return Z4C_OK;
+}
+
+}  // namespace
+
+int z4c_coarse_layout(z4c_ctx* c, long long* offsets, long long* elems_per_var) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  rc = ensure_coarse(c);
+  if (rc) return rc;
+  if (offsets)
+    for (int b = 0; b < c->nblocks; ++b) offsets[b] = c->hrf[b].coff;
+  if (elems_per_var) *elems_per_var = c->rf_total;
+  return Z4C_OK;
+}
+
+int z4c_restrict(z4c_ctx* c, int nvar, const double* u, double* coarse, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u || !coarse || nvar <= 0) return fail(Z4C_EINVAL, "restrict: bad argument");
+  rc = ensure_coarse(c);
+  if (rc) return rc;
+  RestrictArgs a{c->d_blocks, c->d_rf, c->d_rchunks, nvar, u, coarse};
+  hipLaunchKernelGGL(k_restrict_vc, dim3(c->n_rchunks), dim3(256), 0, (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int z4c_prolongation_range(z4c_ctx* c, int block, const int* ox, const long long* lx,
+                           z4c_prolong_box* box) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ox || !lx || !box || block < 0 || block >= c->nblocks)
+    return fail(Z4C_EINVAL, "prolongation_range: bad argument");
+  rc = ensure_coarse(c);
+  if (rc) return rc;
+  const RfBlock& R = c->hrf[block];
+  const int cni[3] = {R.cni, R.cnj, R.cnk};
+  const int pcng = NG / 2;
+  int* se[3][2] = {{&box->si, &box->ei}, {&box->sj, &box->ej}, {&box->sk, &box->ek}};
+  for (int a = 0; a < 3; ++a) {
+    if (ox[a] < -1 || ox[a] > 1) return fail(Z4C_EINVAL, "prolongation_range: offset not in -1..1");
+    const int cvs = RF_NCG, cve = RF_NCG + cni[a] - 1;
+    int s, e;
+    if (ox[a] > 0) {
+      s = cve + 1;
+      e = cve + pcng;
+    } else if (ox[a] < 0) {
+      s = cvs - pcng;
+      e = cvs - 1;
+    } else {
+      s = cvs;
+      e = cve;
+      if ((lx[a] & 1LL) == 0)
+        e += pcng;
+      else
+        s -= pcng;
+    }
+    *se[a][0] = s;
+    *se[a][1] = e;
+  }
+  box->block = block;
+  return Z4C_OK;
+}
+
+int z4c_set_prolongation(z4c_ctx* c, const z4c_prolong_box* boxes, int nbox) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (nbox < 0 || (nbox > 0 && !boxes)) return fail(Z4C_EINVAL, "set_prolongation: bad argument");
+  rc = ensure_coarse(c);
+  if (rc) return rc;
+  std::vector<RfEntry> ent(nbox);
+  std::vector<RfChunk> ch;
+  for (int q = 0; q < nbox; ++q) {
+    rc = make_entry(c, boxes[q], q, ent[q]);
+    if (rc) return rc;
+    const long long np = (long long)ent[q].fn[0] * ent[q].fn[1] * ent[q].fn[2];
+    for (long long s0 = 0; s0 < np; s0 += 256) ch.push_back(RfChunk{q, (int)s0});
+    if (ch.size() > (size_t)std::numeric_limits<int>::max())
+      return fail(Z4C_EINVAL, "set_prolongation: too many boxes");
+  }
+  c->drop_prolongation();
+  if (ch.empty()) return Z4C_OK;
+  if (hipMalloc(&c->d_pent, sizeof(RfEntry) * ent.size()) != hipSuccess ||
+      hipMalloc(&c->d_pchunks, sizeof(RfChunk) * ch.size()) != hipSuccess) {
+    c->drop_prolongation();
+    return fail(Z4C_ENOMEM, "set_prolongation: allocation failed");
+  }
+  HIPCALL(hipMemcpy(c->d_pent, ent.data(), sizeof(RfEntry) * ent.size(), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(c->d_pchunks, ch.data(), sizeof(RfChunk) * ch.size(), hipMemcpyHostToDevice));
+  c->n_pchunks = (int)ch.size();
+  return Z4C_OK;
+}
+
+int z4c_prolongate(z4c_ctx* c, int nvar, const double* coarse, double* u, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!u || !coarse || nvar <= 0) return fail(Z4C_EINVAL, "prolongate: bad argument");
+  if (!c->n_pchunks) return Z4C_OK;
+  ProlongArgs a{c->d_blocks, c->d_rf, c->d_pent, c->d_pchunks, nvar, coarse, u};
+  hipLaunchKernelGGL(k_prolong_vc, dim3(c->n_pchunks), dim3(256), 0, (hipStream_t)stream, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
 }
 
 int z4c_physical_bc(z4c_ctx* c, int nvar, double* u, void* stream) {

@@ -26,7 +26,8 @@ EXPORTS = [
     "z4c_set_timing", "z4c_get_timing", "z4c_history", "z4c_set_exchange", "z4c_exchange",
     "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads", "z4c_rk4_step_graph",
     "z4c_physical_bc", "z4c_rhs_vjp", "z4c_rk4_step_vjp", "z4c_hamiltonian_norm",
-    "z4c_hamiltonian_norm_vjp", "z4c_weyl",
+    "z4c_hamiltonian_norm_vjp", "z4c_weyl", "z4c_coarse_layout", "z4c_restrict",
+    "z4c_prolongation_range", "z4c_set_prolongation", "z4c_prolongate",
 ]
 
 # parameter gradients of the adjoints (Z4C_PBAR_* of the header)
@@ -74,6 +75,11 @@ class Block(C.Structure):
                 ("ju", C.c_int), ("kl", C.c_int), ("ku", C.c_int),
                 ("dx", C.c_double * 3), ("x0", C.c_double * 3), ("level", C.c_int),
                 ("sommerfeld_faces", C.c_int), ("outflow_faces", C.c_int), ("pad_", C.c_int)]
+
+
+class ProlongBox(C.Structure):
+    """z4c_prolong_box: one ProlongateVertexCenteredValues call."""
+    _fields_ = [(n, C.c_int) for n in ("block", "si", "ei", "sj", "ej", "sk", "ek")]
 
 
 class Z4cError(RuntimeError):
@@ -131,6 +137,11 @@ def load(path: str | None = None):
         "z4c_hamiltonian_norm": [vp, pp, vp, C.POINTER(d), vp],
         "z4c_hamiltonian_norm_vjp": [vp, pp, vp, vp, vp],
         "z4c_weyl": [vp, pp, vp, vp, vp, vp],
+        "z4c_coarse_layout": [vp, C.POINTER(ll), C.POINTER(ll)],
+        "z4c_restrict": [vp, i, vp, vp, vp],
+        "z4c_prolongation_range": [vp, i, C.POINTER(i), C.POINTER(ll), C.POINTER(ProlongBox)],
+        "z4c_set_prolongation": [vp, C.POINTER(ProlongBox), i],
+        "z4c_prolongate": [vp, i, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -16,7 +16,8 @@ CSRC = os.path.join(os.path.dirname(PKG), "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libz4c_mi355x.so")
 SOURCES = [os.path.join(CSRC, "z4c_kernels.hip")]
-HEADERS = [os.path.join(CSRC, f) for f in ("z4c_device.hpp", "z4c_rhs_point.hpp", "z4c_stencil.hpp", "z4c_adjoint.hpp")] + [
+HEADERS = [os.path.join(CSRC, f) for f in ("z4c_device.hpp", "z4c_rhs_point.hpp", "z4c_stencil.hpp", "z4c_adjoint.hpp",
+                                           "z4c_refine.hpp")] + [
     os.path.join(INCLUDE, "z4c_mi355x.h")]
 
 # -ffp-contract=off: no fused multiply-add, so every product and sum rounds

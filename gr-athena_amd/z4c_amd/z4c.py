@@ -241,6 +241,58 @@ class Z4c:
         """Physical ghost fill on the flagged faces (PHY_BVAL of GR_Z4c)."""
         check(self.L.z4c_physical_bc(self._ctx, nvar, _ptr(u), _stream(stream)), "physical_bc")
 
+    # -- VC refinement between levels (src/mesh/mesh_refinement_vc.cpp) ---------
+    def coarse_shape(self, b):
+        """(nn3, nn2, nn1) of block b's coarse buffer: nx/2 + 9 vertices per axis."""
+        g = self.blocks[b]
+        return tuple(int(g.nx[d]) // 2 + 2 * NGHOST + 1 for d in (2, 1, 0))
+
+    def coarse_layout(self):
+        """Per-block point offsets of the coarse buffers and points per field."""
+        offs = (C.c_longlong * len(self.blocks))()
+        total = C.c_longlong()
+        check(self.L.z4c_coarse_layout(self._ctx, offs, C.byref(total)), "coarse_layout")
+        return [int(o) for o in offs], int(total.value)
+
+    def coarse_full(self, nvar=Z4C_NVAR, value=0.0):
+        _, total = self.coarse_layout()
+        return torch.full((nvar * total,), value, dtype=torch.float64, device=self.device)
+
+    def coarse_view(self, buf, b, nvar):
+        offs, _ = self.coarse_layout()
+        shp = self.coarse_shape(b)
+        n = shp[0] * shp[1] * shp[2]
+        return buf[offs[b] * nvar:(offs[b] + n) * nvar].view((nvar,) + shp)
+
+    def RestrictInterior(self, u, coarse, nvar=Z4C_NVAR, stream=None):
+        """Injection of every block's fine vertices onto its coarse interior
+        (src/comm/refinement_ops.cpp:178-205)."""
+        check(self.L.z4c_restrict(self._ctx, nvar, _ptr(u), _ptr(coarse), _stream(stream)),
+              "restrict")
+
+    def ProlongationRange(self, block, ox, lx):
+        """Coarse index box (block, si, ei, sj, ej, sk, ek) prolongated for a
+        coarser neighbour at offset ``ox`` of a block at logical location
+        ``lx`` (src/comm/refinement_ops.hpp:48-72)."""
+        o = (C.c_int * 3)(*[int(v) for v in ox])
+        l = (C.c_longlong * 3)(*[int(v) for v in lx])
+        box = _capi.ProlongBox()
+        check(self.L.z4c_prolongation_range(self._ctx, int(block), o, l, C.byref(box)),
+              "prolongation_range")
+        return tuple(getattr(box, n) for n, _ in box._fields_)
+
+    def set_prolongation(self, boxes):
+        """Boxes (block, si, ei, sj, ej, sk, ek) that ``prolongate`` applies."""
+        arr = (_capi.ProlongBox * max(len(boxes), 1))()
+        for q, b in enumerate(boxes):
+            arr[q] = _capi.ProlongBox(*[int(v) for v in b])
+        check(self.L.z4c_set_prolongation(self._ctx, arr, len(boxes)), "set_prolongation")
+
+    def prolongate(self, coarse, u, nvar=Z4C_NVAR, stream=None):
+        """ProlongateVertexCenteredValues over every box set by set_prolongation."""
+        check(self.L.z4c_prolongate(self._ctx, nvar, _ptr(coarse), _ptr(u), _stream(stream)),
+              "prolongate")
+
     def rk4_stage(self, stage, dt, u, un, k, mat=None, periodic=False, exchange=False,
                   physical=False, stream=None):
         """One GR_Z4c stage; ghosts after SumBT: the single-block periodic

@@ -202,6 +202,50 @@ int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
    and x2 ghosts.  Faces in order x1, x2, x3. */
 int z4c_physical_bc(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
+/* Vertex-centred restriction and prolongation between refinement levels
+   (ProlongOp::LagrangeUniform / RestrictOp::Injection of the Z4c fields,
+   src/z4c/z4c.cpp:224-233; operators src/mesh/mesh_refinement_vc.cpp).
+   Every block needs an even number of cells per axis.  A "coarse buffer"
+   holds each block's coarse representation (MeshRefinement's coarse array,
+   src/mesh/meshblock.cpp:611-640, 851-895): nx_d/2 + 9 vertices per axis,
+   coarse interior 4 .. 4 + nx_d/2 at spacing 2 dx, coarse vertex 4 at the
+   fine vertex il; the blocks sit back to back like a state buffer (block b
+   at element offsets[b]*nvar, field n at + n*cn1*cn2*cn3).
+
+   z4c_coarse_layout returns the per-block offsets (array of nblocks, may be
+   NULL) and the elements one field needs (multiply by nvar). */
+int z4c_coarse_layout(z4c_ctx* ctx, long long* offsets, long long* elems_per_var);
+
+/* RestrictInterior (src/comm/refinement_ops.cpp:178-205) of every block:
+   RestrictVertexCenteredValues (mesh_refinement_vc.cpp:56-100) injects the
+   fine vertex at each coarse interior vertex.  Other coarse vertices are not
+   written. */
+int z4c_restrict(z4c_ctx* ctx, int nvar, const double* u, double* coarse, void* stream);
+
+/* One call of ProlongateVertexCenteredValues (mesh_refinement_vc.cpp:283-772)
+   on block `block` over the coarse index box [si, ei] x [sj, ej] x [sk, ek]. */
+typedef struct {
+  int block;
+  int si, ei, sj, ej, sk, ek;
+} z4c_prolong_box;
+
+/* ProlongationRange (src/comm/refinement_ops.hpp:48-72, VC: pcng = 2) for a
+   coarser neighbour of `block` at offset ox[3] (each -1, 0, 1) and the
+   block's logical location lx[3] (only the parity matters): the box of
+   ProlongateNeighbor (src/comm/refinement_ops.cpp:89-107).  Host only. */
+int z4c_prolongation_range(z4c_ctx* ctx, int block, const int* ox, const long long* lx,
+                           z4c_prolong_box* box);
+
+/* z4c_set_prolongation keeps a list of boxes (host array; neighbour ghost
+   boxes from z4c_prolongation_range, or a block's whole coarse interior as
+   AMR refinement prolongates, src/comm/amr_registry.cpp:760-797); every box
+   must keep its reads inside the coarse buffer and its writes inside the
+   block (Z4C_EINVAL otherwise).  z4c_prolongate then applies all of them in
+   one launch, fine vertices from the coarse buffer, in the reference's
+   summation order; boxes that overlap write the same values there. */
+int z4c_set_prolongation(z4c_ctx* ctx, const z4c_prolong_box* boxes, int nbox);
+int z4c_prolongate(z4c_ctx* ctx, int nvar, const double* coarse, double* u, void* stream);
+
 /* One bt_rk4 stage of the GR_Z4c task list
    (src/task_list/gr/task_list_gr_z4c.cpp:45-89, stage s = 1..4) on all
    blocks: k[s-1] = RHS(u_s) (+ Sommerfeld), u_{s+1} = u_n + sum a dt k
