@@ -168,6 +168,29 @@ __device__ __forceinline__ void bstore(const Buf& b, double v, unsigned vbyte, i
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vbyte, sbyte, 0);
 }
 
+// Cache policy of the derivative-slot stores (stencil pass) and loads
+// (assembly).  The slots are written once and read once per stage, far more
+// than the L2 holds: the stores are non-temporal (NT bit, 2), which measured
+// 5% faster on the stencil pass; NT loads measured slower, so loads keep the
+// default policy.
+#ifndef Z4C_DV_STORE_CPOL
+#define Z4C_DV_STORE_CPOL 2
+#endif
+#ifndef Z4C_DV_LOAD_CPOL
+#define Z4C_DV_LOAD_CPOL 0
+#endif
+
+__device__ __forceinline__ double bload_dv(const Buf& b, unsigned vbyte, int sbyte) {
+  return __builtin_bit_cast(
+      double, __builtin_amdgcn_raw_buffer_load_b64(b.r, vbyte, sbyte, Z4C_DV_LOAD_CPOL));
+}
+
+__device__ __forceinline__ void bstore_dv(const Buf& b, double v, unsigned vbyte, int sbyte) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, vbyte, sbyte,
+                                        Z4C_DV_STORE_CPOL);
+}
+
 // Per-thread addressing of one point.
 struct Pt {
   unsigned v0;     // p*8

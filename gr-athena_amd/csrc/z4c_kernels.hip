@@ -399,6 +399,10 @@ struct StageArgs {
 
 // The curvature part (Ricci, D_aD_b alpha) is formed in the same kernel and
 // staged in LDS (a per-thread column) instead of a separate pass through HBM.
+// The stage kernel's state stores are streaming (non-temporal): the next
+// pass reads them back from HBM in any case.
+__device__ __forceinline__ void st_state(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 template <bool WITH_SOMMERFELD>
 __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   __shared__ double ko[NVAR * TILE];  // this point's k_s, staged (see k_rhs_alg)
@@ -436,10 +440,10 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
     const int e = n * N + p;
     const double kv = ko[n * TILE + t];
     if (stage == 1) {
-      acc[e] = un[e] + wb * kv;
+      st_state(acc + e, un[e] + wb * kv);
       s[n] = un[e] + wa * kv;
     } else if (stage < 4) {
-      acc[e] = acc[e] + wb * kv;
+      st_state(acc + e, acc[e] + wb * kv);
       s[n] = un[e] + wa * kv;
     } else {
       s[n] = acc[e] + wb * kv;
@@ -448,7 +452,7 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
                  a.force_reg);
 #pragma unroll
-  for (int n = 0; n < NVAR; ++n) out[n * N + p] = s[n];
+  for (int n = 0; n < NVAR; ++n) st_state(out + n * N + p, s[n]);
 }
 
 // Ghost vertices of a fused step with ghosts held at u_n: what the unfused
@@ -473,13 +477,27 @@ __global__ void __launch_bounds__(TILE) k_rk4_ghosts(GhostArgs a) {
   const int p = flat(B, i, j, k);
   const long long base = B.off * NVAR;
   double* u = a.u + base;
-  algconstr_point(u, N, p, a.eps_floor, a.tf_force, a.force_reg);
+  // every field is loaded once; AlgConstr works on the registers
+  double s[NVAR];
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) s[n] = u[n * N + p];
+  algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
+                 a.force_reg);
+#pragma unroll
+  for (int ab = 0; ab < 6; ++ab) {
+    u[(V_GXX + ab) * N + p] = s[V_GXX + ab];
+    u[(V_AXX + ab) * N + p] = s[V_AXX + ab];
+  }
+  if (a.force_reg) {
+    u[V_CHI * N + p] = s[V_CHI];
+    u[V_ALPHA * N + p] = s[V_ALPHA];
+  }
   double* w0 = a.w0 + base;
   double* w1 = a.w1 + base;
+#pragma unroll
   for (int n = 0; n < NVAR; ++n) {
-    const double v = u[n * N + p];
-    w0[n * N + p] = v;
-    w1[n * N + p] = v;
+    w0[n * N + p] = s[n];
+    w1[n * N + p] = s[n];
   }
 }
 

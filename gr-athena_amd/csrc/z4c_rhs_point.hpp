@@ -34,7 +34,7 @@ struct PointIO {
   int fd;       // NI*8
   bool has_mat;
   __device__ __forceinline__ double u(int n) const { return bload(U, vu, n * fu); }
-  __device__ __forceinline__ double d(int slot) const { return bload(DV, vd, slot * fd); }
+  __device__ __forceinline__ double d(int slot) const { return bload_dv(DV, vd, slot * fd); }
   __device__ __forceinline__ double m(int n) const { return bload(MT, vu, n * fu); }
 };
 

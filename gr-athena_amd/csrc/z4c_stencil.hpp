@@ -228,7 +228,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       double xv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) xv[m] = cur[o + (m - 4)];
-      if (grad) bstore(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
+      if (grad) bstore_dv(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
       if (full) dd[0] = st_dxx(xv, B.cidx2[0]);
       L += st_lx(xv, B.lidx[0], bx);
       D = st_diss(xv, B.cidxd[0], a.diss);
@@ -237,17 +237,17 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       double yv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) yv[m] = cur[o + (m - 4) * PX];
-      if (grad) bstore(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
+      if (grad) bstore_dv(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
       if (full) dd[3] = st_dxx(yv, B.cidx2[1]);
       L += st_lx(yv, B.lidx[1], by);
       D += st_diss(yv, B.cidxd[1], a.diss);
     }
-    if (grad) bstore(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
+    if (grad) bstore_dv(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
     if (full) dd[5] = st_dxx(zr, B.cidx2[2]);
     // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
     L += st_lx(zr, B.lidx[2], bz);
     D += st_diss(zr, B.cidxd[2], a.diss);
-    bstore(DVB, lc * L + D, q8, sld * NS8);
+    bstore_dv(DVB, lc * L + D, q8, sld * NS8);
     if (full) {
       double v[9];
       v[0] = v[8] = 0.0;
@@ -268,10 +268,10 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
 #pragma unroll
           for (int d = c; d < 3; ++d)
             P += ((c == d) ? 0.5 : 1.0) * guu[sym(c, d)] * dd[sym(c, d)];
-        bstore(DVB, P, q8, (db + 3) * NS8);
+        bstore_dv(DVB, P, q8, (db + 3) * NS8);
       } else {
 #pragma unroll
-        for (int ab = 0; ab < 6; ++ab) bstore(DVB, dd[ab], q8, (db + 3 + ab) * NS8);
+        for (int ab = 0; ab < 6; ++ab) bstore_dv(DVB, dd[ab], q8, (db + 3 + ab) * NS8);
       }
     }
     q8 += qstep8;
