@@ -44,7 +44,7 @@ __host__ __device__ constexpr int sym(int a, int b) {
 // Device-side description of one MeshBlock.
 // Derivative slots of a block: NDV rows of nis = nint + DV_PAD doubles; the
 // DV_PAD entries past the interior take the stores of stencil threads that
-// lie past their tile (never read).
+// lie past their tile (thread t at entry t mod DV_PAD; never read).
 constexpr int DV_PAD = 384;
 
 struct DevBlock {

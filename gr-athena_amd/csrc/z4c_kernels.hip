@@ -2056,14 +2056,17 @@ int launch_prep(z4c_ctx* c, const z4c_params* p, const double* u, double* aux,
 // Stencil-pass plan
 // ---------------------------------------------------------------------------
 constexpr int ST_THREADS_OF[NSTCFG] = {StCfg256::THREADS, StCfg320::THREADS,
-                                       StCfg192::THREADS, StCfg384::THREADS};
+                                       StCfg192::THREADS, StCfg384::THREADS,
+                                       StCfg640::THREADS, StCfg576::THREADS};
 
 bool cfg_fits(int cfg, int ni, int nj) {
   switch (cfg) {
     case 0: return StCfg256::fits(ni, nj);
     case 1: return StCfg320::fits(ni, nj);
     case 2: return StCfg192::fits(ni, nj);
-    default: return StCfg384::fits(ni, nj);
+    case 3: return StCfg384::fits(ni, nj);
+    case 4: return StCfg640::fits(ni, nj);
+    default: return StCfg576::fits(ni, nj);
   }
 }
 
@@ -2127,7 +2130,9 @@ void launch_stencil_cfg(int cfg, const StencilArgs& sa, hipStream_t s) {
     case 0: hipLaunchKernelGGL(k_stencil<StCfg256>, g, dim3(StCfg256::THREADS), 0, s, sa); break;
     case 1: hipLaunchKernelGGL(k_stencil<StCfg320>, g, dim3(StCfg320::THREADS), 0, s, sa); break;
     case 2: hipLaunchKernelGGL(k_stencil<StCfg192>, g, dim3(StCfg192::THREADS), 0, s, sa); break;
-    default: hipLaunchKernelGGL(k_stencil<StCfg384>, g, dim3(StCfg384::THREADS), 0, s, sa); break;
+    case 3: hipLaunchKernelGGL(k_stencil<StCfg384>, g, dim3(StCfg384::THREADS), 0, s, sa); break;
+    case 4: hipLaunchKernelGGL(k_stencil<StCfg640>, g, dim3(StCfg640::THREADS), 0, s, sa); break;
+    default: hipLaunchKernelGGL(k_stencil<StCfg576>, g, dim3(StCfg576::THREADS), 0, s, sa); break;
   }
 }
 
@@ -3149,7 +3154,7 @@ int z4c_set_stencil_threads(z4c_ctx* c, int block, int threads) {
   int q = -1;
   for (int k = 0; k < NSTCFG; ++k)
     if (ST_THREADS_OF[k] == threads) q = k;
-  if (q < 0) return fail(Z4C_EINVAL, "set_stencil_threads: threads must be 192, 256, 320 or 384");
+  if (q < 0) return fail(Z4C_EINVAL, "set_stencil_threads: threads must be 192, 256, 320, 384, 576 or 640");
   const int b0 = block < 0 ? 0 : block, b1 = block < 0 ? c->nblocks : block + 1;
   for (int b = b0; b < b1; ++b)
     if (!plane_tiles(q, b, c->hdev[b].ni, c->hdev[b].nj, nullptr))

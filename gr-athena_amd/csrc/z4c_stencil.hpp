@@ -55,9 +55,10 @@ using StCfg256 = StencilCfg<256, 3, 2>;
 using StCfg320 = StencilCfg<320, 3, 2>;
 using StCfg192 = StencilCfg<192, 4, 3>;
 using StCfg384 = StencilCfg<384, 2, 2>;
-constexpr int NSTCFG = 4;
-constexpr int ST_MAX_THREADS = 384;
-static_assert(DV_PAD >= ST_MAX_THREADS, "sink slots must cover a workgroup");
+// one tile per 25 x 25 plane (24^3-cell blocks), two per 33 x 33 plane
+using StCfg640 = StencilCfg<640, 2, 2>;
+using StCfg576 = StencilCfg<576, 2, 2>;
+constexpr int NSTCFG = 6;
 
 struct StencilArgs {
   const DevBlock* blocks;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   const int o = (ti + 4) + PX * (tj + 4);  // in the LDS plane
   const int w = (ti + 3) + EX * (tj + 3);  // in the derived planes
   const unsigned pxy8 = 8u * ((gi0 + ti + 4) + B.nn1 * (gj0 + tj + 4));
-  unsigned q8 = 8u * (active ? (T.i0 + ti) + B.ni * (T.j0 + tj) : B.nint + tid);
+  unsigned q8 = 8u * (active ? (T.i0 + ti) + B.ni * (T.j0 + tj) : B.nint + tid % DV_PAD);
   const unsigned qstep8 = active ? 8u * B.ni * B.nj : 0u;
 
   // this thread's LDS-plane elements and Dx_z columns (byte offsets, clamped)

@@ -297,7 +297,7 @@ int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
 /* Tile autotuner of the stencil pass (no counterpart in the reference,
    whose pencil loops have no tiling).  For every block size among the
    registered blocks, times the stencil pass of those blocks under each
-   workgroup configuration (192, 256, 320 or 384 threads, each with the
+   workgroup configuration (192, 256, 320, 384, 576 or 640 threads, each with the
    fewest x-y tiles that fit) on state `u`, `reps` launches each, and keeps
    the fastest.  Until it is called a block uses the 256-thread
    configuration, or one that needs at least a quarter fewer

@@ -34,19 +34,19 @@ def test_stencil_configurations_agree_and_autotune():
     """Every stencil workgroup configuration gives bit-identical right-hand
     sides; the autotuner only picks among them."""
     gs = [MeshBlockGeom.cube(16, 0.0625, 0.0), MeshBlockGeom.cube(24, 0.05, 2.0),
-          MeshBlockGeom.cube(12, 0.1, -2.0)]
+          MeshBlockGeom.cube(12, 0.1, -2.0), MeshBlockGeom.cube(32, 0.03, 5.0)]
     p = Params(diss=0.02, damp_kappa1=0.02)
     z = Z4c(gs, p)
     u = z.upload([perturbed_flat(g, seed=3 + b) for b, g in enumerate(gs)])
     outs = []
-    for th in (192, 256, 320, 384):
+    for th in (192, 256, 320, 384, 576, 640):
         z.set_stencil_threads(th)
         assert set(z.stencil_plan().values()) == {th}
         r = z.zeros()
         z.Z4cRHS(u, None, r)
         outs.append(r)
     plan = z.autotune(u, reps=2)
-    assert set(plan.values()) <= {192, 256, 320, 384}
+    assert set(plan.values()) <= {192, 256, 320, 384, 576, 640}
     r = z.zeros()
     z.Z4cRHS(u, None, r)
     outs.append(r)
