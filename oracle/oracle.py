@@ -156,6 +156,12 @@ def lib():
         L.oracle_history.argtypes = [pp, pg, pd, pd, pd]
         L.oracle_physical_bc.argtypes = [pg, C.c_int, pd, P(C.c_int)]
         L.oracle_exchange.argtypes = [C.c_int, pg, P(pd), C.c_int, pd, pd, P(C.c_int)]
+        ub = P(C.c_ubyte)
+        L.oracle_rst_header.restype = C.c_ulong
+        L.oracle_rst_header.argtypes = [ub, C.c_int, C.c_int, pd, P(C.c_int), d, d, C.c_int,
+                                        C.c_ulonglong]
+        L.oracle_rst_idrec.restype = C.c_ulong
+        L.oracle_rst_idrec.argtypes = [ub, P(C.c_longlong), C.c_int, d]
     return _lib
 
 
@@ -351,3 +357,18 @@ def prolongation_range(g, ox, parity):
                                         C.byref(e_))
         out += [s_.value, e_.value]
     return tuple(out)
+
+
+def rst_header(nbtotal, root_level, size9, nx, time, dt, ncycle, datasize):
+    """Bytes of the restart mesh header as the reference copies its structs."""
+    buf = (C.c_ubyte * 256)()
+    n = lib().oracle_rst_header(buf, nbtotal, root_level, (C.c_double * 9)(*size9),
+                                (C.c_int * 3)(*nx), time, dt, ncycle, datasize)
+    return bytes(buf[:n])
+
+
+def rst_idrec(lx, level, cost):
+    """Bytes of one restart ID-list record (LogicalLocation, cost)."""
+    buf = (C.c_ubyte * 64)()
+    n = lib().oracle_rst_idrec(buf, (C.c_longlong * 3)(*lx), level, cost)
+    return bytes(buf[:n])

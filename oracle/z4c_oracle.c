@@ -1775,3 +1775,50 @@ void oracle_prolongation_range(int parity, int ox, int cvs, int cve, int* s, int
       *s -= pcng;
   }
 }
+
+
+/* ------------------------------------------------------------------------ */
+/* Restart records (src/outputs/restart.cpp:141-147, 206-214).              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  double x1min, x2min, x3min;
+  double x1max, x2max, x3max;
+  double x1rat, x2rat, x3rat;
+  int nx1, nx2, nx3;
+} or_region_size; /* RegionSize, src/athena.hpp:88-96 */
+
+typedef struct {
+  long long lx1, lx2, lx3;
+  int level;
+} or_logical_location; /* LogicalLocation, src/athena.hpp:58-66 */
+
+unsigned long oracle_rst_header(unsigned char* out, int nbtotal, int root_level,
+                                const double* size9, const int* nx, double time, double dt,
+                                int ncycle, unsigned long long datasize) {
+  or_region_size rs;
+  memset(&rs, 0, sizeof rs);
+  rs.x1min = size9[0]; rs.x2min = size9[1]; rs.x3min = size9[2];
+  rs.x1max = size9[3]; rs.x2max = size9[4]; rs.x3max = size9[5];
+  rs.x1rat = size9[6]; rs.x2rat = size9[7]; rs.x3rat = size9[8];
+  rs.nx1 = nx[0]; rs.nx2 = nx[1]; rs.nx3 = nx[2];
+  unsigned long o = 0;
+  memcpy(out + o, &nbtotal, sizeof(int)); o += sizeof(int);
+  memcpy(out + o, &root_level, sizeof(int)); o += sizeof(int);
+  memcpy(out + o, &rs, sizeof rs); o += sizeof rs;
+  memcpy(out + o, &time, sizeof(double)); o += sizeof(double);
+  memcpy(out + o, &dt, sizeof(double)); o += sizeof(double);
+  memcpy(out + o, &ncycle, sizeof(int)); o += sizeof(int);
+  memcpy(out + o, &datasize, sizeof datasize); o += sizeof datasize;
+  return o;
+}
+
+unsigned long oracle_rst_idrec(unsigned char* out, const long long* lx, int level,
+                               double cost) {
+  or_logical_location ll;
+  memset(&ll, 0, sizeof ll);
+  ll.lx1 = lx[0]; ll.lx2 = lx[1]; ll.lx3 = lx[2];
+  ll.level = level;
+  memcpy(out, &ll, sizeof ll);
+  memcpy(out + sizeof ll, &cost, sizeof(double));
+  return sizeof ll + sizeof(double);
+}

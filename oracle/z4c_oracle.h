@@ -178,6 +178,17 @@ void oracle_prolongate_vc(const oracle_geom* g, const oracle_geom* cg, int nvar,
                           int ej, int sk, int ek);
 void oracle_prolongation_range(int parity, int ox, int cvs, int cve, int* s, int* e);
 
+/* Restart-file records as RestartOutput::WriteOutputFile copies them
+   (src/outputs/restart.cpp:141-147, 206-214): the mesh header (nbtotal,
+   root_level, RegionSize, time, dt, ncycle, datasize) and one ID-list record
+   (LogicalLocation, cost), the structs declared as src/athena.hpp:58-96
+   declares them, padding zeroed.  Return the bytes written. */
+unsigned long oracle_rst_header(unsigned char* out, int nbtotal, int root_level,
+                                const double* size9, const int* nx, double time, double dt,
+                                int ncycle, unsigned long long datasize);
+unsigned long oracle_rst_idrec(unsigned char* out, const long long* lx, int level,
+                               double cost);
+
 #ifdef __cplusplus
 }
 #endif
