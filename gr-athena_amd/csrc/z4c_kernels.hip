@@ -390,6 +390,7 @@ struct StageArgs {
   const double* un;  // state at the start of the step (unused at the last stage)
   double* acc;       // running sum of the final combination
   double* out;       // stage output state (the caller's state at the last stage)
+  double* keep;      // stage 1 of the in-place step: receives u_n (else null)
   double wa, wb;     // A[s+1][s] * dt and b_s * dt
   int stage;         // 1..4
   int cowling;
@@ -432,6 +433,7 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   const double* un = a.un + base;
   double* acc = a.acc + base;
   double* out = a.out + base;
+  double* keep = a.keep ? a.keep + base : nullptr;
   const int stage = a.stage;
   const double wa = a.wa, wb = a.wb;
   double s[NVAR];
@@ -440,8 +442,10 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
     const int e = n * N + p;
     const double kv = ko[n * TILE + t];
     if (stage == 1) {
-      st_state(acc + e, un[e] + wb * kv);
-      s[n] = un[e] + wa * kv;
+      const double u0 = un[e];
+      if (keep) st_state(keep + e, u0);
+      st_state(acc + e, u0 + wb * kv);
+      s[n] = u0 + wa * kv;
     } else if (stage < 4) {
       st_state(acc + e, acc[e] + wb * kv);
       s[n] = un[e] + wa * kv;
@@ -458,7 +462,10 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
 // Ghost vertices of a fused step with ghosts held at u_n: what the unfused
 // sequence leaves there after every stage is AlgConstr(u_n + 0), so the step
 // applies AlgConstr to the state's ghosts once (after the first stage has
-// read them) and copies the result into the two stage registers.
+// read them).  The in-place step keeps every stage state in the caller's
+// state, whose ghosts are then the only copy; the step with separate stage
+// registers (Sommerfeld faces read neighbouring stage values, so the stage
+// cannot overwrite its input) also copies them into the two registers.
 struct GhostArgs {
   const DevBlock* blocks;
   Launch L;
@@ -468,6 +475,14 @@ struct GhostArgs {
   double eps_floor;
   int tf_force, force_reg;
 };
+
+__global__ void __launch_bounds__(TILE) k_rk4_ghosts_inplace(GhostArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  algconstr_point(a.u + B.off * NVAR, B.npts, flat(B, i, j, k), a.eps_floor, a.tf_force,
+                  a.force_reg);
+}
 
 __global__ void __launch_bounds__(TILE) k_rk4_ghosts(GhostArgs a) {
   int b, i, j, k;
@@ -3215,16 +3230,26 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   // bt_rk4 (task_list_integrators_butcher.cpp:37-49): subdiagonal of A and b
   static const double Asub[4] = {0.5, 0.5, 1.0, 0.0};
   static const double Bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  // In place (no Sommerfeld face): every stage state overwrites the
+  // interior of u (the stage kernel reads u only at its own point), u_n is
+  // kept in W0 by stage 1, and u's ghosts, AlgConstr(u_n) after stage 1, are
+  // the ghosts of every later stage.  With Sommerfeld faces the stage reads
+  // neighbouring values of its input, so the stage states alternate between
+  // W0 and W1 and the ghosts are copied into both.
+  bool faces = false;
+  for (const DevBlock& B : c->hdev) faces = faces || B.faces != 0;
+  const bool inplace = !faces;
   double* const ins[4] = {u, W0, W1, W0};
   double* const outs[4] = {W0, W1, W0, u};
   for (int st = 1; st <= 4; ++st) {
-    const double* in = ins[st - 1];
+    const double* in = inplace ? u : ins[st - 1];
     StageArgs a{};
     a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, nullptr, nullptr, c->cv,
                   rhs_params(p)};
-    a.un = u;
+    a.un = (inplace && st > 1) ? W0 : u;
     a.acc = ACC;
-    a.out = outs[st - 1];
+    a.out = inplace ? u : outs[st - 1];
+    a.keep = (inplace && st == 1) ? W0 : nullptr;
     a.wa = Asub[st - 1] * dt;
     a.wb = Bw[st - 1] * dt;
     a.stage = st;
@@ -3247,7 +3272,11 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
                    p->tf_force_regularization, p->force_regularization};
       {
         PassTimer t(c, Z4C_PASS_GHOSTS, s);
-        hipLaunchKernelGGL(k_rk4_ghosts, dim3(grid_for(ga.L.ntiles)), dim3(TILE), 0, s, ga);
+        if (inplace)
+          hipLaunchKernelGGL(k_rk4_ghosts_inplace, dim3(grid_for(ga.L.ntiles)), dim3(TILE), 0, s,
+                             ga);
+        else
+          hipLaunchKernelGGL(k_rk4_ghosts, dim3(grid_for(ga.L.ntiles)), dim3(TILE), 0, s, ga);
       }
       HIPCALL(hipGetLastError());
     }

@@ -55,10 +55,12 @@ def test_stencil_configurations_agree_and_autotune():
         assert torch.equal(o, outs[0])
 
 
-def test_graph_step_replays_equal_eager_steps():
+@pytest.mark.parametrize("faces", [63, 0])
+def test_graph_step_replays_equal_eager_steps(faces):
     """The captured fused step, replayed several times with eager work and
-    synchronisations in between, gives the eager step's state every time."""
-    gs = [MeshBlockGeom.cube(8, 0.125, 0.3, faces=63), MeshBlockGeom.cube(12, 0.1, 3.0)]
+    synchronisations in between, gives the eager step's state every time
+    (with a Sommerfeld block: stage registers; without: in place)."""
+    gs = [MeshBlockGeom.cube(8, 0.125, 0.3, faces=faces), MeshBlockGeom.cube(12, 0.1, 3.0)]
     p = Params(diss=0.02, damp_kappa1=0.02, shift_eta=2.0)
     z = Z4c(gs, p)
     host = [perturbed_flat(g, seed=40 + b) for b, g in enumerate(gs)]
