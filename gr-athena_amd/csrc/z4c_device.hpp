@@ -42,16 +42,20 @@ __host__ __device__ constexpr int sym(int a, int b) {
 }
 
 // Device-side description of one MeshBlock.
-// Derivative slots of a block: NDV rows of nis = nint + DV_PAD doubles; the
-// DV_PAD entries past the interior take the stores of stencil threads that
-// lie past their tile (thread t at entry t mod DV_PAD; never read).
-constexpr int DV_PAD = 384;
+// Derivative slots of a block: NDV rows of nis doubles, nint + DV_PAD rounded
+// up to 16 (each row starts on a 128-byte line); the entries past the
+// interior are never read or written.
+constexpr int DV_PAD = 16;
+// Byte offset of a store that the buffer unit drops: past every slot
+// buffer's range (< 2^31 bytes, checked at registration), and with any slot
+// offset (< 2^31) added it stays below 2^32.
+constexpr unsigned DV_DROP = 0x80000000u;
 
 struct DevBlock {
   long long off;     // point offset in state buffers (block data at off*nvar)
   long long ioff;    // interior-point offset in interior-indexed buffers
   long long doff;    // offset of the block's derivative slots (NDV * nis each)
-  int nis;           // derivative-slot row length, nint + DV_PAD
+  int nis;           // derivative-slot row length (nint + DV_PAD, multiple of 16)
   int nn1, nn2, nn3;
   int il, iu, jl, ju, kl, ku;
   int npts;          // nn1*nn2*nn3

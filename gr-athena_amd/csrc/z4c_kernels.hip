@@ -2517,27 +2517,28 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     B.nj = in.ju - in.jl + 1;
     B.nk = in.ku - in.kl + 1;
     B.nint = B.ni * B.nj * B.nk;
-    B.nis = B.nint + DV_PAD;
-    if ((long long)B.nis * NDV > std::numeric_limits<int>::max())
+    B.nis = (B.nint + DV_PAD + 15) / 16 * 16;
+    if ((long long)B.nis * NDV * 8 >= (long long)DV_DROP)
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
     B.pfaces = (in.sommerfeld_faces | in.outflow_faces) & 63;
-    // stencil configuration until tuned: 256 threads, unless another size
-    // needs at least a quarter fewer thread slots (idle lanes)
+    // stencil configuration until tuned: the fewest x-y tiles (halo rows
+    // read once), then the fewest thread slots (idle lanes)
     {
       int best = -1;
-      long long best_slots = 0;
+      long long best_tiles = 0, best_slots = 0;
       for (int q = 0; q < NSTCFG; ++q) {
         const long long sl = cfg_slots(q, B.ni, B.nj);
-        if (sl > 0 && (best < 0 || sl < best_slots)) {
+        if (sl <= 0) continue;
+        const long long nt = sl / ST_THREADS_OF[q];
+        if (best < 0 || nt < best_tiles || (nt == best_tiles && sl < best_slots)) {
           best = q;
+          best_tiles = nt;
           best_slots = sl;
         }
       }
       if (best < 0)
         return fail(Z4C_EINVAL, "set_blocks: no stencil tile fits block " + std::to_string(b));
-      const long long s256 = cfg_slots(0, B.ni, B.nj);
-      if (s256 > 0 && 4 * best_slots > 3 * s256) best = 0;
       blk_cfg.push_back(best);
     }
     for (int d = 0; d < 3; ++d) {

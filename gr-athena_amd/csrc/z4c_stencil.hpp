@@ -16,9 +16,9 @@
 //
 // The loop body has no data-dependent branches: every thread issues the
 // same loads and stores every plane (threads past the tile read clamped,
-// in-range addresses and store into the sink slots past each block's
-// interior, see DevBlock::nis), so the memory counter waits only for the
-// next plane's loads, not for the stores just issued.
+// in-range addresses; their stores carry a byte offset past the slot
+// buffer's range, which the buffer unit drops), so the memory counter waits
+// only for the next plane's loads, not for the stores just issued.
 //
 // The mixed second derivatives are the reference's nested Dx(a, Dx(b, u)),
 // a < b (src/z4c/calculate_z4c_rhs.cpp:132-175, src/z4c/adm_z4c.cpp:977-1037),
@@ -99,14 +99,14 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   const int gi0 = B.il + T.i0 - 4, gj0 = B.jl + T.j0 - 4;  // padded coords of plane (0,0)
 
   // this thread's output point; threads past the tile mirror point 0 and
-  // store into the sink slots
+  // their stores are dropped
   const int npt = T.ni * T.nj;
   const bool active = tid < npt;
   const int tj = active ? tid / T.ni : 0, ti = active ? tid - tj * T.ni : 0;
   const int o = (ti + 4) + PX * (tj + 4);  // in the LDS plane
   const int w = (ti + 3) + EX * (tj + 3);  // in the derived planes
   const unsigned pxy8 = 8u * ((gi0 + ti + 4) + B.nn1 * (gj0 + tj + 4));
-  unsigned q8 = 8u * (active ? (T.i0 + ti) + B.ni * (T.j0 + tj) : B.nint + tid % DV_PAD);
+  unsigned q8 = active ? 8u * ((T.i0 + ti) + B.ni * (T.j0 + tj)) : DV_DROP;
   const unsigned qstep8 = active ? 8u * B.ni * B.nj : 0u;
 
   // this thread's LDS-plane elements and Dx_z columns (byte offsets, clamped)
@@ -221,6 +221,9 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       }
       __syncthreads();
     }
+    // slot stores of this plane (dropped for lanes past the tile, whose
+    // offset lies beyond the buffer's range)
+    auto put = [&](double v, int slot) { bstore_dv(DVB, v, q8, slot * NS8); };
     // one direction at a time (x, y, then z from the ring), so that a
     // single 9-point array is live; L and D accumulate in x, y, z order
     double L = 0.0, D;
@@ -229,7 +232,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       double xv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) xv[m] = cur[o + (m - 4)];
-      if (grad) bstore_dv(DVB, st_dx(xv, B.cidx1[0]), q8, (db + 0) * NS8);
+      if (grad) put(st_dx(xv, B.cidx1[0]), db + 0);
       if (full) dd[0] = st_dxx(xv, B.cidx2[0]);
       L += st_lx(xv, B.lidx[0], bx);
       D = st_diss(xv, B.cidxd[0], a.diss);
@@ -238,17 +241,17 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       double yv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) yv[m] = cur[o + (m - 4) * PX];
-      if (grad) bstore_dv(DVB, st_dx(yv, B.cidx1[1]), q8, (db + 1) * NS8);
+      if (grad) put(st_dx(yv, B.cidx1[1]), db + 1);
       if (full) dd[3] = st_dxx(yv, B.cidx2[1]);
       L += st_lx(yv, B.lidx[1], by);
       D += st_diss(yv, B.cidxd[1], a.diss);
     }
-    if (grad) bstore_dv(DVB, st_dx(zr, B.cidx1[2]), q8, (db + 2) * NS8);
+    if (grad) put(st_dx(zr, B.cidx1[2]), db + 2);
     if (full) dd[5] = st_dxx(zr, B.cidx2[2]);
     // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
     L += st_lx(zr, B.lidx[2], bz);
     D += st_diss(zr, B.cidxd[2], a.diss);
-    bstore_dv(DVB, lc * L + D, q8, sld * NS8);
+    put(lc * L + D, sld);
     if (full) {
       double v[9];
       v[0] = v[8] = 0.0;
@@ -269,10 +272,10 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
 #pragma unroll
           for (int d = c; d < 3; ++d)
             P += ((c == d) ? 0.5 : 1.0) * guu[sym(c, d)] * dd[sym(c, d)];
-        bstore_dv(DVB, P, q8, (db + 3) * NS8);
+        put(P, db + 3);
       } else {
 #pragma unroll
-        for (int ab = 0; ab < 6; ++ab) bstore_dv(DVB, dd[ab], q8, (db + 3 + ab) * NS8);
+        for (int ab = 0; ab < 6; ++ab) put(dd[ab], db + 3 + ab);
       }
     }
     q8 += qstep8;
