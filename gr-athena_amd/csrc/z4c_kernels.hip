@@ -388,9 +388,11 @@ __global__ void __launch_bounds__(TILE) k_algconstr(AlgArgs a) {
 struct StageArgs {
   RhsArgs r;         // r.u: stage input state (read only); r.rhs unused
   const double* un;  // state at the start of the step (unused at the last stage)
-  double* acc;       // running sum of the final combination
+  int un_int;        // un is interior-indexed (the kept copy) instead of a state
+  double* acc;       // running sum of the final combination (interior-indexed)
   double* out;       // stage output state (the caller's state at the last stage)
-  double* keep;      // stage 1 of the in-place step: receives u_n (else null)
+  double* keep;      // stage 1 of the in-place step: receives u_n, interior-
+                     // indexed (else null)
   double wa, wb;     // A[s+1][s] * dt and b_s * dt
   int stage;         // 1..4
   int cowling;
@@ -415,10 +417,10 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   const int N = B.npts;
   const int p = flat(B, i, j, k);
   const long long base = B.off * NVAR;
+  const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
   if (a.cowling) {
     for (int n = 0; n < NVAR; ++n) ko[n * TILE + t] = 0.0;
   } else {
-    const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
     const PointIO X = point_io(a.r, B, p, qi);
     rhs_curvature_point(a.r.o, X, [&](int n, double v) { cvs[n * TILE + t] = v; });
     rhs_assembly_point(a.r.o, X, [&](int n) { return cvs[n * TILE + t]; },
@@ -429,28 +431,34 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
                        [&](int n, double v) { ko[n * TILE + t] = v; });
     }
   }
-  // stage update of every variable, then AlgConstr of the point
-  const double* un = a.un + base;
-  double* acc = a.acc + base;
+  // stage update of every variable, then AlgConstr of the point.  The
+  // running sum and the kept u_n are interior-indexed (whole 128-byte lines
+  // of interior points, no ghost columns in the lines read).
+  const long long ibase = B.ioff * NVAR;
+  const double* un = a.un_int ? a.un + ibase : a.un + base;
+  double* acc = a.acc + ibase;
   double* out = a.out + base;
-  double* keep = a.keep ? a.keep + base : nullptr;
+  double* keep = a.keep ? a.keep + ibase : nullptr;
   const int stage = a.stage;
   const double wa = a.wa, wb = a.wb;
+  const int NI = B.nint;
   double s[NVAR];
 #pragma unroll
   for (int n = 0; n < NVAR; ++n) {
     const int e = n * N + p;
+    const int ei = n * NI + qi;
+    const int eu = a.un_int ? ei : e;
     const double kv = ko[n * TILE + t];
     if (stage == 1) {
-      const double u0 = un[e];
-      if (keep) st_state(keep + e, u0);
-      st_state(acc + e, u0 + wb * kv);
+      const double u0 = un[eu];
+      if (keep) st_state(keep + ei, u0);
+      st_state(acc + ei, u0 + wb * kv);
       s[n] = u0 + wa * kv;
     } else if (stage < 4) {
-      st_state(acc + e, acc[e] + wb * kv);
-      s[n] = un[e] + wa * kv;
+      st_state(acc + ei, acc[ei] + wb * kv);
+      s[n] = un[eu] + wa * kv;
     } else {
-      s[n] = acc[e] + wb * kv;
+      s[n] = acc[ei] + wb * kv;
     }
   }
   algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
@@ -3248,6 +3256,7 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
     a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, nullptr, nullptr, c->cv,
                   rhs_params(p)};
     a.un = (inplace && st > 1) ? W0 : u;
+    a.un_int = (inplace && st > 1) ? 1 : 0;
     a.acc = ACC;
     a.out = inplace ? u : outs[st - 1];
     a.keep = (inplace && st == 1) ? W0 : nullptr;
