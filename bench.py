@@ -36,7 +36,7 @@ PEAK_HBM = 8.0e12
 # measured HBM bytes per launch of each kernel (FETCH_SIZE / WRITE_SIZE
 # passes, calibrated against a copy kernel of known size)
 PASS_KERNEL = {"stencil": "k_stencil", "curvature": "k_rhs_curv", "assembly": "k_rk4_stage",
-               "ghosts": "k_rk4_ghosts", "prepare": "k_prep"}
+               "ghosts": "k_rk4_ghosts_inplace", "prepare": "k_prep"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 
 # Algorithmic HBM bytes of each pass per launch, from what the pass must read
@@ -44,15 +44,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 #  stencil:   reads the 22 fields over the padded block, writes 106 slots
 #  curvature: (separate pass of z4c_rhs) reads 7 fields + 51 slots, writes
 #             12 curvature values
-#  assembly:  (fused step: curvature, assembly, stage update, AlgConstr)
-#             reads the 106 slots + 22 fields, and per stage on average
-#             16.5 (u_n) + 16.5 (running sum) doubles, writes 16.5 (running
-#             sum) + 22 (stage state)
+#  assembly:  (fused step in place: curvature, assembly, stage update,
+#             AlgConstr) reads the 106 slots + 22 fields, and per stage on
+#             average 11 (kept u_n, stages 2-3) + 16.5 (running sum, stages
+#             2-4) doubles; writes 22 (stage state) + 16.5 (running sum,
+#             stages 1-3) + 5.5 (u_n kept by stage 1)
+#  ghosts:    AlgConstr of the ghost vertices in place: reads g~, A~, chi,
+#             alpha (14), writes g~, A~ (12)
 PASS_BYTES = {
     "stencil": lambda I, P: 8.0 * (106 * I + 22 * P),
     "curvature": lambda I, P: 8.0 * (7 + 51 + 12) * I,
-    "assembly": lambda I, P: 8.0 * (106 + 22 + 16.5 + 16.5 + 16.5 + 22) * I,
-    "ghosts": lambda I, P: 8.0 * (22 + 14 + 44) * (P - I),
+    "assembly": lambda I, P: 8.0 * (106 + 22 + 11 + 16.5 + 22 + 16.5 + 5.5) * I,
+    "ghosts": lambda I, P: 8.0 * (14 + 12) * (P - I),
     "prepare": lambda I, P: 8.0 * (11 + 33) * P,
 }
 

@@ -2530,22 +2530,12 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
       return fail(Z4C_EINVAL, "set_blocks: block too large");
     B.faces = in.sommerfeld_faces & 63;
     B.pfaces = (in.sommerfeld_faces | in.outflow_faces) & 63;
-    // stencil configuration until tuned: the fewest x-y tiles (halo rows
-    // read once), then the fewest thread slots (idle lanes)
+    // stencil configuration until tuned: 256 threads (the fastest on 17^2,
+    // 25^2 and 33^2 planes when measured), or 192 when the whole plane fits
+    // one 192-thread tile (fewer idle lanes)
     {
-      int best = -1;
-      long long best_tiles = 0, best_slots = 0;
-      for (int q = 0; q < NSTCFG; ++q) {
-        const long long sl = cfg_slots(q, B.ni, B.nj);
-        if (sl <= 0) continue;
-        const long long nt = sl / ST_THREADS_OF[q];
-        if (best < 0 || nt < best_tiles || (nt == best_tiles && sl < best_slots)) {
-          best = q;
-          best_tiles = nt;
-          best_slots = sl;
-        }
-      }
-      if (best < 0)
+      int best = StCfg192::fits(B.ni, B.nj) ? 2 : 0;
+      if (cfg_slots(best, B.ni, B.nj) <= 0)
         return fail(Z4C_EINVAL, "set_blocks: no stencil tile fits block " + std::to_string(b));
       blk_cfg.push_back(best);
     }

@@ -299,10 +299,9 @@ int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
    registered blocks, times the stencil pass of those blocks under each
    workgroup configuration (192, 256, 320, 384, 576 or 640 threads, each with the
    fewest x-y tiles that fit) on state `u`, `reps` launches each, and keeps
-   the fastest.  Until it is called a block uses the 256-thread
-   configuration, or one that needs at least a quarter fewer
-   thread slots (tiles x threads).  Results do not depend on the
-   configuration.
+   the fastest.  Until it is called a block uses 256 threads, or 192 when
+   its whole x-y plane fits one 192-thread tile.  Results do not depend on
+   the configuration.
    Waits for the stream. */
 int z4c_autotune(z4c_ctx* ctx, const z4c_params* p, const double* u, int reps, void* stream);
 /* Workgroup size the stencil pass uses for `block`; setting it (block -1:
