@@ -264,15 +264,18 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
    (src/task_list/gr/task_list_gr_z4c.cpp:45-89 over stages 1..4, tableau
    src/task_list/task_list_integrators_butcher.cpp:37-49) on all blocks, with
    ghost vertices held at u_n (no exchange) and Sommerfeld faces honoured.
-   Per stage: the stencil and curvature passes of Z4cRHS, then one kernel
-   that assembles the RHS and applies SumBT and AlgConstr at each interior
-   vertex; the ghosts get AlgConstr(u_n) once.  The derivative arrays of
-   PrepareZ4cDerivatives are not formed (nothing in the step reads them;
-   z4c_adm_constraints prepares its own).  The state equals z4c_rk4_stage
-   over stages 1..4 (same sums in the same order) without k registers:
-   `work` is caller-owned device memory
-   of 3 * 22 * elems_per_var doubles (two stage states and the running sum).
-   `u` holds u_n on entry and u_{n+1} on return. */
+   Per stage: the stencil pass of Z4cRHS, then one kernel that forms the
+   curvature part, assembles the RHS and applies SumBT and AlgConstr at each
+   interior vertex; the ghosts get AlgConstr(u_n) once.  The derivative
+   arrays of PrepareZ4cDerivatives are not formed (nothing in the step reads
+   them; z4c_adm_constraints prepares its own).  The state equals
+   z4c_rk4_stage over stages 1..4 (same sums in the same order) without k
+   registers: `work` is caller-owned device memory of 3 * 22 * elems_per_var
+   doubles.  Without Sommerfeld faces every stage state is written over the
+   interior of `u`, as the reference's stage update overwrites its state
+   (work then holds u_n and the running sum); with them the stage states
+   alternate between two work registers.  `u` holds u_n on entry and
+   u_{n+1} on return. */
 int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                  double* work, const double* mat, void* stream);
 
