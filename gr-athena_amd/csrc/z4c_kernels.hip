@@ -2635,7 +2635,6 @@ int z4c_rhs(z4c_ctx* c, const z4c_params* p, const double* u, const double* mat,
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !rhs) return fail(Z4C_EINVAL, "rhs: null argument");
-  if (mat) return fail(Z4C_EINVAL, "rhs: matter sources are not enabled in this build (vacuum only)");
   return launch_rhs(c, p, u, mat, rhs, false, (hipStream_t)stream);
 }
 
@@ -3220,7 +3219,7 @@ int z4c_get_timing(z4c_ctx* c, double* ms_total, long long* launches, int npass)
 
 namespace {
 int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
-                    hipStream_t s) {
+                    const double* mat, hipStream_t s) {
   int rc = Z4C_OK;
   const long long n = c->total_pts * NVAR;
   double* W0 = work;
@@ -3243,7 +3242,7 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   for (int st = 1; st <= 4; ++st) {
     const double* in = inplace ? u : ins[st - 1];
     StageArgs a{};
-    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, nullptr, nullptr, c->cv,
+    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, mat, nullptr, c->cv,
                   rhs_params(p)};
     a.un = (inplace && st > 1) ? W0 : u;
     a.un_int = (inplace && st > 1) ? 1 : 0;
@@ -3293,8 +3292,7 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step: bad argument");
-  if (mat) return fail(Z4C_EINVAL, "rk4_step: matter sources are not enabled in this build (vacuum only)");
-  return launch_rk4_step(c, p, dt, u, work, (hipStream_t)stream);
+  return launch_rk4_step(c, p, dt, u, work, mat, (hipStream_t)stream);
 }
 
 int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
@@ -3310,7 +3308,7 @@ int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, do
     c->invalidate_graph();
     if (!c->cap_stream) HIPCALL(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
     HIPCALL(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_rk4_step(c, p, dt, u, work, c->cap_stream);
+    rc = launch_rk4_step(c, p, dt, u, work, nullptr, c->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
     if (rc) {
@@ -3340,7 +3338,6 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   if (rc) return rc;
   if (!p || !u || !un || !k || stage < 1 || stage > 4)
     return fail(Z4C_EINVAL, "rk4_stage: bad argument");
-  if (mat) return fail(Z4C_EINVAL, "rk4_stage: matter sources are not enabled in this build (vacuum only)");
   for (int s = 0; s < stage; ++s)
     if (!k[s]) return fail(Z4C_EINVAL, "rk4_stage: null k register");
   hipStream_t s = (hipStream_t)stream;

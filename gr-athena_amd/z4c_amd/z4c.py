@@ -187,10 +187,9 @@ class Z4c:
 
     def Z4cWeyl(self, u_adm, u_mat, u_weyl, u, stream=None):
         """Z4c::Z4cWeyl(u_adm, u_mat, u_weyl) (calculate_weyl_scalars.cpp:44)
-        with the state ``u`` explicit (the reference reads storage.u);
-        ``u_mat`` must be None (vacuum)."""
-        if u_mat is not None:
-            raise _capi.Z4cError("weyl: matter is not enabled in this build (vacuum only)")
+        with the state ``u`` explicit (the reference reads storage.u).
+        ``u_mat`` is not read: the reference sets its aliases
+        (calculate_weyl_scalars.cpp:57-58) but no term uses them."""
         self.weyl(u, u_adm, u_weyl, stream)
 
     def weyl(self, u, adm, out=None, stream=None):
@@ -322,7 +321,8 @@ class Z4c:
         change)."""
         if graph:
             if mat is not None:
-                raise _capi.Z4cError("rk4_step_graph: matter sources are not enabled (vacuum only)")
+                raise _capi.Z4cError("rk4_step_graph takes no matter sources: replay a vacuum "
+                                     "step, or call the eager step with mat")
             check(self.L.z4c_rk4_step_graph(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                             _ptr(work), _stream(stream)), "rk4_step_graph")
             return

@@ -113,8 +113,8 @@ int z4c_prepare_derivatives(z4c_ctx* ctx, const z4c_params* p, const double* u,
 
 /* Z4c::Z4cRHS (src/z4c/calculate_z4c_rhs.cpp:29-916) incl. KO dissipation,
    interior points only; all derivatives are formed from `u` (ghosts
-   included).  `mat` must be NULL (vacuum; matter sources are rejected in
-   this build).  Ghosts of rhs are not written. */
+   included).  `mat` (10 fields rho, S_a, S_ab, src/z4c/z4c.hpp:163-184)
+   may be NULL (vacuum).  Ghosts of rhs are not written. */
 int z4c_rhs(z4c_ctx* ctx, const z4c_params* p, const double* u,
             const double* mat, double* rhs, void* stream);
 

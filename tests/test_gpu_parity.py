@@ -85,13 +85,72 @@ def test_rhs_single_block_32():
     assert np.all(gh == 0.0)
 
 
-def test_matter_sources_rejected():
-    """Matter sources are not enabled in this build: the call fails loudly."""
-    g = MeshBlockGeom.cube(4, 0.1, 0.0)
-    z = Z4c([g], Params())
-    u = z.upload([constrained_state(g, Params(), seed=1)])
+def _matter(g, seed, amp=1e-2):
+    """Random matter fields (rho, S_a, S_ab: src/z4c/z4c.hpp:163-184)."""
+    return np.ascontiguousarray(amp * np.random.default_rng(seed).standard_normal((10,) + g.shape))
+
+
+def test_rhs_with_matter_sources():
+    """Z4cRHS with the matter terms (calculate_z4c_rhs.cpp:466-476 and the
+    rho / S_a / S_ab terms of K^, Theta, Gam~, A~) against the oracle on two
+    uneven blocks, and bit-identical over repeated calls."""
+    gs = [MeshBlockGeom.cube(8, 0.1, (0.1, -0.2, 0.3)), MeshBlockGeom.cube(12, 0.07, (1.0, 0.0, 0.0))]
+    p = Params(**MP_GAUGE)
+    us = [constrained_state(g, p, seed=60 + b) for b, g in enumerate(gs)]
+    ms = [_matter(g, 70 + b) for b, g in enumerate(gs)]
+    z = Z4c(gs, p)
+    u, m = z.upload(us), z.upload(ms, 10)
+    outs = []
+    for _ in range(3):
+        r = z.zeros()
+        z.Z4cRHS(u, m, r)
+        outs.append(r)
+    torch.cuda.synchronize()
+    assert all(torch.equal(outs[0], o) for o in outs[1:])
+    got = z.download(outs[0], 22)
+    for b, g in enumerate(gs):
+        exp = orc.rhs(oparams(p), ogeom(g), us[b], ms[b])
+        vac = orc.rhs(oparams(p), ogeom(g), us[b])
+        assert np.all(rel(got[b][interior(g)], exp[interior(g)]) <= TOL), b
+        assert np.abs(exp - vac)[interior(g)].max() > 1e-3  # the sources matter
+
+
+@pytest.mark.parametrize("faces", [True, False])
+def test_fused_step_with_matter_equals_stage_sequence(faces):
+    """The fused step (in place without Sommerfeld faces, through the stage
+    registers with them) with matter sources against the stage sequence, and
+    the graph-replayed step refuses matter."""
+    gs = _uneven_batch(faces)
+    p = Params(**MP_GAUGE)
+    host = [constrained_state(g, p, seed=80 + b) for b, g in enumerate(gs)]
+    z = Z4c(gs, p)
+    m = z.upload([_matter(g, 90 + b) for b, g in enumerate(gs)], 10)
+    dt = 0.25 * min(min(g.dx) for g in gs)
+    ua, ub = z.upload(host), z.upload(host)
+    un, k = z.make_rk_registers()
+    work = z.make_step_work()
+    z.rk4_step(dt, ua, un, k, mat=m)
+    z.rk4_step_fused(dt, ub, work, mat=m)
+    torch.cuda.synchronize()
+    A, Bf = z.download(ua, 22), z.download(ub, 22)
+    for b in range(len(gs)):
+        assert np.all(rel(Bf[b], A[b]) <= 1e-14), b
     with pytest.raises(Z4cError):
-        z.Z4cRHS(u, z.zeros(10), z.zeros())
+        z.rk4_step_fused(dt, ub, work, mat=m, graph=True)
+
+
+def test_fused_step_with_matter_against_oracle():
+    g = MeshBlockGeom.cube(12, 1.0 / 12, 0.0)
+    p = Params(**MP_GAUGE)
+    u0 = constrained_state(g, p, seed=5)
+    m0 = _matter(g, 6)
+    dt = 0.25 / 12
+    exp, _ = orc.rk4_step(oparams(p), ogeom(g), u0.copy(), dt, mat=m0)
+    z = Z4c([g], p)
+    u = z.upload([u0])
+    z.rk4_step_fused(dt, u, z.make_step_work(), mat=z.upload([m0], 10))
+    torch.cuda.synchronize()
+    assert np.all(rel(z.download(u, 22)[0], exp) <= TOL)
 
 
 def test_rhs_all_terms_general_power():
