@@ -2823,8 +2823,9 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
     const int il[3] = {B.il, B.jl, B.kl};
     for (int d = 0; d < 3; ++d) {
       if (B.dx[d] != h[d])
-        return fail(Z4C_EINVAL, "set_exchange: blocks of different spacing need prolongation "
-                                "and restriction, which this build does not provide");
+        return fail(Z4C_EINVAL, "set_exchange: blocks of different spacing: the same-level map "
+                                "needs one spacing (levels go through z4c_restrict / "
+                                "z4c_prolongate)");
       const double L = (B.x0[d] + il[d] * h[d] - lo[d]) / h[d];
       Lo[b][d] = std::llround(L);
       if (std::fabs(L - (double)Lo[b][d]) > 1e-6)

@@ -182,8 +182,9 @@ int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
    src/comm/index_utilities.hpp:116-139, division by the node multiplicity
    src/comm/node_multiplicity.cpp:349-420).  z4c_set_exchange builds the map
    for the domain [lo, hi] (periodic[d] != 0: axis d wraps); all blocks must
-   share one spacing and sit on its lattice (Z4C_EINVAL otherwise: levels
-   need prolongation/restriction, not provided).  z4c_exchange then sets
+   share one spacing and sit on its lattice (Z4C_EINVAL otherwise; across
+   levels the caller combines z4c_restrict / z4c_prolongate below with its
+   own copies into the coarse buffers).  z4c_exchange then sets
    every vertex that some block owns as an interior vertex (ghosts, and
    shared boundary vertices) to the mean of its owners' values, summed in
    block order and k, j, i order, times 1/multiplicity. */
