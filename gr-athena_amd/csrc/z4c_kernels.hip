@@ -406,7 +406,10 @@ struct StageArgs {
 // pass reads them back from HBM in any case.
 __device__ __forceinline__ void st_state(double* p, double v) { __builtin_nontemporal_store(v, p); }
 
-template <bool WITH_SOMMERFELD>
+// ALG: AlgConstr at the end (the step with ghosts held at u_n); without it
+// the stage leaves the sums for the ghost update and the whole-block
+// AlgConstr that follow it, in the task list's order.
+template <bool WITH_SOMMERFELD, bool ALG>
 __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   __shared__ double ko[NVAR * TILE];  // this point's k_s, staged (see k_rhs_alg)
   __shared__ double cvs[12 * TILE];   // this point's curvature part
@@ -461,8 +464,9 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
       s[n] = acc[ei] + wb * kv;
     }
   }
-  algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
-                 a.force_reg);
+  if (ALG)
+    algconstr_vals(s + V_GXX, s + V_AXX, s[V_CHI], s[V_ALPHA], a.eps_floor, a.tf_force,
+                   a.force_reg);
 #pragma unroll
   for (int n = 0; n < NVAR; ++n) st_state(out + n * N + p, s[n]);
 }
@@ -483,6 +487,17 @@ struct GhostArgs {
   double eps_floor;
   int tf_force, force_reg;
 };
+
+// Copies the ghost vertices of all fields from a.u into a.w0.
+__global__ void __launch_bounds__(TILE) k_ghost_copy(GhostArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const int N = B.npts;
+  const long long e = B.off * NVAR + flat(B, i, j, k);
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) a.w0[e + (long long)n * N] = a.u[e + (long long)n * N];
+}
 
 __global__ void __launch_bounds__(TILE) k_rk4_ghosts_inplace(GhostArgs a) {
   int b, i, j, k;
@@ -1199,14 +1214,14 @@ struct ExArgs {
   int ndst;
   int nvar;
   double* u;
-  const double* snap;
+  double* tmp;         // nvar * ndst: the new values, field-major
 };
 
-__global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
+// Phase 1: every destination's new value from the state as it stands (no
+// destination is written yet, so shared vertices read their old values).
+__global__ void __launch_bounds__(256) k_exchange_gather(ExArgs a) {
   const int d = blockIdx.x * 256 + threadIdx.x;
   if (d >= a.ndst) return;
-  const DevBlock B = a.blocks[a.dst[2 * d]];
-  const int p = a.dst[2 * d + 1];
   const int s0 = a.src_off[d], s1 = a.src_off[d + 1];
   const int cnt = s1 - s0;
   const double inv = 1.0 / (double)cnt;
@@ -1214,10 +1229,20 @@ __global__ void __launch_bounds__(256) k_exchange(ExArgs a) {
     double acc = 0.0;
     for (int e = s0; e < s1; ++e) {
       const DevBlock& S = a.blocks[a.src[2 * e]];
-      acc += a.snap[S.off * a.nvar + (long long)n * S.npts + a.src[2 * e + 1]];
+      acc += a.u[S.off * a.nvar + (long long)n * S.npts + a.src[2 * e + 1]];
     }
-    a.u[B.off * a.nvar + (long long)n * B.npts + p] = cnt > 1 ? acc * inv : acc;
+    a.tmp[(long long)n * a.ndst + d] = cnt > 1 ? acc * inv : acc;
   }
+}
+
+// Phase 2: the new values into the state.
+__global__ void __launch_bounds__(256) k_exchange_scatter(ExArgs a) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= a.ndst) return;
+  const DevBlock B = a.blocks[a.dst[2 * d]];
+  const int p = a.dst[2 * d + 1];
+  for (int n = 0; n < a.nvar; ++n)
+    a.u[B.off * a.nvar + (long long)n * B.npts + p] = a.tmp[(long long)n * a.ndst + d];
 }
 
 // ---------------------------------------------------------------------------
@@ -1868,7 +1893,7 @@ struct z4c_ctx {
   PlaneTile* st_tiles[NSTCFG] = {};
   int st_ntiles[NSTCFG] = {};
   long long total_int = 0;  // interior points
-  double* snap = nullptr;  // periodic-fill snapshot (lazy)
+  double* snap = nullptr;  // periodic-fill snapshot / exchange scratch (lazy)
   long long snap_elems = 0;
   double* hst = nullptr;   // history partial sums (lazy)
   PhysLine* ph_lines[3] = {};  // physical-fill line tables per axis
@@ -2311,13 +2336,15 @@ int launch_physical(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
 
 int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   if (!c->ex_ndst) return Z4C_OK;
-  const long long n = c->total_pts * nvar;
-  int rc = ensure_snapshot(c, n);
+  // the destinations' new values go through a scratch array of nvar * ndst
+  // doubles (gather, then scatter), not through a copy of the whole state
+  int rc = ensure_snapshot(c, (long long)c->ex_ndst * nvar);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_copy, dim3(elementwise_grid(n)), dim3(256), 0, s, c->snap, (const double*)u, n);
-  HIPCALL(hipGetLastError());
   ExArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ex_src, c->ex_ndst, nvar, u, c->snap};
-  hipLaunchKernelGGL(k_exchange, dim3((c->ex_ndst + 255) / 256), dim3(256), 0, s, a);
+  const dim3 g((c->ex_ndst + 255) / 256);
+  hipLaunchKernelGGL(k_exchange_gather, g, dim3(256), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_exchange_scatter, g, dim3(256), 0, s, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
@@ -3219,8 +3246,12 @@ int z4c_get_timing(z4c_ctx* c, double* ms_total, long long* launches, int npass)
 }  // extern "C"
 
 namespace {
+// ghosts: 0 holds the ghosts at u_n (AlgConstr(u_n), formed once); else the
+// ghost update of z4c_rk4_stage after every stage (1 single-block periodic
+// fill or 2 the exchange, plus 4 the physical fill), on the stage state whose
+// ghosts SumBT resets to u_n, then AlgConstr of the whole block.
 int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
-                    const double* mat, hipStream_t s) {
+                    const double* mat, int ghosts, hipStream_t s) {
   int rc = Z4C_OK;
   const long long n = c->total_pts * NVAR;
   double* W0 = work;
@@ -3240,6 +3271,17 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   const bool inplace = !faces;
   double* const ins[4] = {u, W0, W1, W0};
   double* const outs[4] = {W0, W1, W0, u};
+  // with a ghost update, u_n's ghosts (what SumBT leaves at the ghosts) are
+  // kept in W1 when the stages run in place; otherwise u keeps them
+  const GhostArgs keep_ghosts{c->d_blocks, launch_of(c, BOX_GHOST), u, W1, nullptr,
+                              p->eps_floor, p->tf_force_regularization,
+                              p->force_regularization};
+  if (ghosts && inplace) {
+    PassTimer t(c, Z4C_PASS_GHOSTS, s);
+    hipLaunchKernelGGL(k_ghost_copy, dim3(grid_for(keep_ghosts.L.ntiles)), dim3(TILE), 0, s,
+                       keep_ghosts);
+    HIPCALL(hipGetLastError());
+  }
   for (int st = 1; st <= 4; ++st) {
     const double* in = inplace ? u : ins[st - 1];
     StageArgs a{};
@@ -3264,9 +3306,38 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
     }
     {
       PassTimer t(c, Z4C_PASS_ASSEMBLY, s);
-      hipLaunchKernelGGL(k_rk4_stage<true>, dim3(g), dim3(TILE), 0, s, a);
+      if (ghosts)
+        hipLaunchKernelGGL((k_rk4_stage<true, false>), dim3(g), dim3(TILE), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_rk4_stage<true, true>), dim3(g), dim3(TILE), 0, s, a);
     }
     HIPCALL(hipGetLastError());
+    if (ghosts) {
+      // SumBT's ghost values (u_n) into the stage state, then the ghost
+      // update and AlgConstr of the whole block (z4c_rk4_stage's order)
+      double* cur = a.out;
+      if (cur != u || (inplace && st > 1)) {
+        GhostArgs gc{c->d_blocks, launch_of(c, BOX_GHOST), inplace ? W1 : u, cur, nullptr,
+                     p->eps_floor, p->tf_force_regularization, p->force_regularization};
+        PassTimer t(c, Z4C_PASS_GHOSTS, s);
+        hipLaunchKernelGGL(k_ghost_copy, dim3(grid_for(gc.L.ntiles)), dim3(TILE), 0, s, gc);
+        HIPCALL(hipGetLastError());
+      }
+      if (ghosts & 1) {
+        rc = launch_periodic(c, NVAR, cur, s);
+        if (rc) return rc;
+      } else if (ghosts & 2) {
+        rc = launch_exchange(c, NVAR, cur, s);
+        if (rc) return rc;
+      }
+      if (ghosts & 4) {
+        rc = launch_physical(c, NVAR, cur, s);
+        if (rc) return rc;
+      }
+      rc = launch_alg(c, p, cur, nullptr, 0, s);
+      if (rc) return rc;
+      continue;
+    }
     if (st == 1) {
       GhostArgs ga{c->d_blocks, launch_of(c, BOX_GHOST), u, W0, W1, p->eps_floor,
                    p->tf_force_regularization, p->force_regularization};
@@ -3293,7 +3364,16 @@ int z4c_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* 
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step: bad argument");
-  return launch_rk4_step(c, p, dt, u, work, mat, (hipStream_t)stream);
+  return launch_rk4_step(c, p, dt, u, work, mat, 0, (hipStream_t)stream);
+}
+
+int z4c_rk4_step_ghosts(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                        const double* mat, int ghosts, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work || ghosts < 0 || ghosts > 7 || (ghosts & 3) == 3)
+    return fail(Z4C_EINVAL, "rk4_step_ghosts: bad argument");
+  return launch_rk4_step(c, p, dt, u, work, mat, ghosts, (hipStream_t)stream);
 }
 
 int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
@@ -3309,7 +3389,7 @@ int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, do
     c->invalidate_graph();
     if (!c->cap_stream) HIPCALL(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
     HIPCALL(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_rk4_step(c, p, dt, u, work, nullptr, c->cap_stream);
+    rc = launch_rk4_step(c, p, dt, u, work, nullptr, 0, c->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
     if (rc) {

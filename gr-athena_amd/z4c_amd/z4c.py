@@ -311,14 +311,26 @@ class Z4c:
         for s in range(1, 5):
             self.rk4_stage(s, dt, u, un, k, mat, periodic, exchange, physical, stream)
 
-    def rk4_step_fused(self, dt, u, work, mat=None, stream=None, graph=False):
+    def rk4_step_fused(self, dt, u, work, mat=None, stream=None, graph=False, periodic=False,
+                       exchange=False, physical=False):
         """One bt_rk4 step, fused per stage (RHS assembly + SumBT + AlgConstr
-        in one kernel), ghosts held at u_n.  The state equals ``rk4_step``
-        with no ghost update; the derivative arrays are not prepared
+        in one kernel).  Without a ghost update the ghosts are held at u_n
+        and the state equals ``rk4_step`` with no ghost update; with
+        ``periodic`` / ``exchange`` / ``physical`` every stage updates the
+        ghosts as ``rk4_step`` with the same flags does, then applies
+        AlgConstr to the whole block.  The derivative arrays are not prepared
         (ADMConstraints prepares its own).  ``work`` comes from
-        ``make_step_work``.  ``graph``: replay the step from a HIP graph
-        (captured on first use and whenever u, work, dt or the parameters
-        change)."""
+        ``make_step_work``.  ``graph``: replay the step (ghosts held at u_n)
+        from a HIP graph (captured on first use and whenever u, work, dt or
+        the parameters change)."""
+        mode = (2 if exchange else (1 if periodic else 0)) + (4 if physical else 0)
+        if mode:
+            if graph:
+                raise _capi.Z4cError("the graph-replayed step holds the ghosts at u_n")
+            check(self.L.z4c_rk4_step_ghosts(self._ctx, C.byref(self.params), float(dt), _ptr(u),
+                                             _ptr(work), _ptr(mat), mode, _stream(stream)),
+                  "rk4_step_ghosts")
+            return
         if graph:
             if mat is not None:
                 raise _capi.Z4cError("rk4_step_graph takes no matter sources: replay a vacuum "

@@ -280,6 +280,17 @@ int z4c_rk4_stage(z4c_ctx* ctx, const z4c_params* p, int stage, double dt,
 int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                  double* work, const double* mat, void* stream);
 
+/* z4c_rk4_step with the task list's ghost update after every stage instead
+   of ghosts held at u_n: SumBT leaves u_n at the ghosts, then `ghosts` = 1
+   (single-block periodic fill) or 2 (the exchange set by z4c_set_exchange),
+   plus 4 (physical fill), then AlgConstr of the whole block -- the order and
+   sums of z4c_rk4_stage over stages 1..4 (src/task_list/gr/
+   task_list_gr_z4c.cpp:45-89) without its derivative preparation.  `work`
+   as for z4c_rk4_step.  Not capturable (the exchange's scratch is allocated
+   on first use). */
+int z4c_rk4_step_ghosts(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
+                        double* work, const double* mat, int ghosts, void* stream);
+
 /* z4c_rk4_step replayed from a HIP graph (no counterpart in the reference:
    its task list launches nothing).  The first call, and any call whose u,
    work, dt or parameters differ from the captured ones, captures the step's

@@ -13,7 +13,7 @@ torch = pytest.importorskip("torch")
 
 from z4c_amd import Params, Z4cError  # noqa: E402
 from z4c_amd import awa  # noqa: E402
-from z4c_amd.initial_data import gauge_wave_adm  # noqa: E402
+from z4c_amd.initial_data import gauge_wave_adm, perturbed_flat  # noqa: E402
 from z4c_amd.z4c import MeshBlockGeom, Z4c  # noqa: E402
 
 
@@ -184,3 +184,65 @@ def test_sommerfeld_block_evolution_with_physical_fill():
     assert np.all(num / den <= 1e-8), num / den
     # and the interior stays at the gauge wave's amplitude
     assert np.abs(got[sl][8:14]).max() < 0.1
+
+
+def _fused_vs_stages(gs, params, host, dt, nsteps, setup=None, **flags):
+    """The fused step with a ghost update against rk4_step (four
+    z4c_rk4_stage calls) with the same update."""
+    z = Z4c(gs, params)
+    if setup:
+        setup(z)
+    ua, ub = z.upload(host), z.upload(host)
+    un, k = z.make_rk_registers()
+    work = z.make_step_work()
+    z.PrepareZ4cDerivatives(ua)
+    for _ in range(nsteps):
+        z.rk4_step(dt, ua, un, k, **flags)
+        z.rk4_step_fused(dt, ub, work, **flags)
+    torch.cuda.synchronize()
+    A, Bf = z.download(ua, 22), z.download(ub, 22)
+    for b in range(len(gs)):
+        num = np.abs(Bf[b] - A[b]).reshape(22, -1).max(axis=1)
+        den = np.maximum(np.abs(A[b]).reshape(22, -1).max(axis=1), 1e-300)
+        assert np.all(num / den <= 1e-14), (b, flags, num / den)
+    return A
+
+
+def test_fused_step_with_exchange_equals_stage_sequence():
+    """Split periodic gauge wave (2 x 1 x 2 blocks): the fused step with the
+    exchange after every stage (stages in place) against the stage
+    sequence, two steps."""
+    n = 16
+    gs = _grid((2, 1, 2), n, 1.0 / n, -0.5)
+    params = Params(lapse_oplog=2.0, shift_Gamma=1.0, shift_eta=2.0, diss=0.02,
+                    damp_kappa1=0.02)
+    op = orc.Params(**params.as_dict())
+    host = [orc.adm_to_z4c(op, orc.Geom.block(g.nx, g.dx, g.xmin),
+                           np.ascontiguousarray(gauge_wave_adm(g, A=0.05, d=1.0))) for g in gs]
+    _fused_vs_stages(gs, params, host, 0.25 / n, 2,
+                     setup=lambda z: z.set_exchange([-0.5] * 3, [0.5] * 3, (1, 1, 1)),
+                     exchange=True)
+
+
+def test_fused_step_with_periodic_fill_equals_stage_sequence():
+    n = 12
+    g = MeshBlockGeom.cube(n, 1.0 / n, -0.5)
+    params = Params(lapse_oplog=2.0, shift_Gamma=1.0, shift_eta=2.0, diss=0.02)
+    op = orc.Params(**params.as_dict())
+    u0 = orc.adm_to_z4c(op, orc.Geom.block(g.nx, g.dx, g.xmin),
+                        np.ascontiguousarray(gauge_wave_adm(g, A=0.05, d=1.0)))
+    orc.periodic_fill(orc.Geom.block(g.nx, g.dx, g.xmin), u0)
+    _fused_vs_stages([g], params, [u0], 0.25 / n, 2, periodic=True)
+
+
+@pytest.mark.parametrize("faces,outflow", [(63, 0), (0, 63)])
+def test_fused_step_with_physical_fill_equals_stage_sequence(faces, outflow):
+    """Physical ghost fill after every stage: with Sommerfeld faces (stage
+    states in the work registers) and with outflow faces only (in place)."""
+    g = MeshBlockGeom.cube(12, 1.0 / 12, (0.3, -0.45, 0.2), faces=faces, outflow=outflow)
+    p = Params(lapse_oplog=2.0, shift_Gamma=1.0, shift_eta=2.0, diss=0.02, damp_kappa1=0.02,
+               chi_div_floor=1e-5)
+    op = orc.Params(**p.as_dict())
+    u0 = orc.algconstr(op, orc.Geom.block(g.nx, g.dx, g.xmin),
+                       np.ascontiguousarray(perturbed_flat(g, seed=12, amp=(1e-3, 1e-2))))
+    _fused_vs_stages([g], p, [u0], 0.25 / 12, 2, physical=True)
