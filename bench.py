@@ -96,6 +96,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     ap.add_argument("--no-sensitivity", action="store_true",
                     help="skip the 100-step evolution + 10-step adjoint sensitivity leg")
+    ap.add_argument("--no-exchange", action="store_true",
+                    help="skip the step-with-exchange leg on a uniform periodic block grid")
     ap.add_argument("--sensitivity-only", action="store_true",
                     help="run only the sensitivity leg and print its object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -279,6 +281,8 @@ def main():
         del u, work
         torch.cuda.empty_cache()
         out["sensitivity"] = sensitivity_run(adm_fn, params)
+    if not args.no_exchange:
+        out["exchange_step"] = exchange_step_cost(params)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
@@ -311,6 +315,52 @@ def diagnostics_cost(z, u, reps=3):
         raise RuntimeError("non-finite psi4 on the workload")
     log("diagnostics: " + ", ".join(f"{k} {v:.2f}" for k, v in res.items()))
     del adm, con, w
+    return res
+
+
+def exchange_step_cost(params, nb=8, n=16, steps=3):
+    """A step with the ghost update after every stage, on a uniform periodic
+    domain of nb^3 blocks of n^3 cells (the gauge wave, A = 0.01): the fused
+    step (z4c_rk4_step_ghosts, exchange) against the GR_Z4c stage sequence
+    (four z4c_rk4_stage calls, exchange, with the derivative preparation of
+    the task list), both timed over ``steps`` steps after one warmup."""
+    import torch
+    from z4c_amd.initial_data import gauge_wave_adm
+    from z4c_amd.z4c import MeshBlockGeom, Z4c
+    h = 1.0 / (nb * n)
+    gs = [MeshBlockGeom.cube(n, h, (-0.5 + a * n * h, -0.5 + b * n * h, -0.5 + c * n * h))
+          for c in range(nb) for b in range(nb) for a in range(nb)]
+    z = Z4c(gs, params)
+    z.set_exchange([-0.5] * 3, [0.5] * 3, (1, 1, 1))
+    adm = z.zeros(17)
+    for q, g in enumerate(gs):
+        z.block_view(adm, q, 17).copy_(torch.from_numpy(gauge_wave_adm(g, A=0.01, d=1.0)))
+    u = z.zeros()
+    z.ADMToZ4c(adm, u)
+    del adm
+    z.exchange(u)
+    dt = 0.25 * h
+    work = z.make_step_work()
+    un, k = z.make_rk_registers()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    t_fused = timed(lambda: z.rk4_step_fused(dt, u, work, exchange=True))
+    t_stages = timed(lambda: z.rk4_step(dt, u, un, k, exchange=True))
+    owned = len(gs) * n ** 3
+    res = {"workload": f"gauge wave on a periodic [-0.5, 0.5]^3 split into {nb}^3 blocks of "
+                       f"{n}^3 cells, exchange after every stage",
+           "fused_ms_per_step": t_fused, "stage_sequence_ms_per_step": t_stages,
+           "fused_point_updates_per_s": owned / t_fused * 1e3}
+    log(f"exchange step: fused {t_fused:.2f} ms, stage sequence {t_stages:.2f} ms per step")
+    z.close()
     return res
 
 
