@@ -85,10 +85,13 @@ def run_gauge_wave(n=64, t_end=10.0, cfl=0.25, amplitude=0.01, d=1.0, gauge="har
     if not math.isclose(nsteps * dt, t_end, rel_tol=1e-12):
         raise ValueError("t_end must be a whole number of steps of cfl*dx")
     hst_every = hst_every or max(1, int(round(0.1 / dt)))
-    un, k = z.make_rk_registers()
+    work = z.make_step_work()
     times, rows = [0.0], [constraint_history(z, u)]
     for s in range(1, nsteps + 1):
-        z.rk4_step(dt, u, un, k, periodic=True)
+        # the fused step with the periodic fill after every stage (the
+        # stage sequence's order and sums; ADMConstraints prepares its own
+        # derivatives)
+        z.rk4_step_fused(dt, u, work, periodic=True)
         if s % hst_every == 0 or s == nsteps:
             times.append(s * dt)
             rows.append(constraint_history(z, u))

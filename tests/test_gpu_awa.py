@@ -33,6 +33,24 @@ def _oracle_run(geom, params, nsteps, dt):
     return u, np.array(rows)
 
 
+def test_run_gauge_wave_driver_matches_oracle():
+    """The driver (fused steps with the periodic fill, .hst rows at its
+    cadence) over 4 steps of a 12^3 block against the oracle's run."""
+    n, nsteps = 12, 4
+    geom, params, z, _ = awa.gauge_wave_setup(n=n, gauge="harmonic")
+    dt = z.NewBlockTimeStep(0.25)
+    times, rows, got, _ = awa.run_gauge_wave(n=n, t_end=nsteps * dt, hst_every=2)
+    assert len(times) == 3 and np.isclose(times[-1], nsteps * dt)
+    exp_u, exp_rows = _oracle_run(geom, params, nsteps, dt)
+    num = np.abs(got - exp_u).reshape(22, -1).max(axis=1)
+    den = np.maximum(np.abs(exp_u).reshape(22, -1).max(axis=1), 0.01)
+    assert np.all(num / den <= 1e-11), num / den
+    eps, V = 1e-12, 1.0
+    exp_sel = exp_rows[[0, 2, 4]]
+    bound = 2 * eps * np.sqrt(np.maximum(rows, exp_sel) * V) + eps * eps * V
+    assert np.all(np.abs(rows - exp_sel) <= bound)
+
+
 @pytest.mark.parametrize("gauge", ["harmonic", "puncture"])
 def test_gauge_wave_steps_and_history(gauge):
     n, nsteps = 16, 3
