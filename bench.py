@@ -35,15 +35,13 @@ PEAK_HBM = 8.0e12
 # kernel of each timed pass, and the committed counter summary holding the
 # measured HBM bytes per launch of each kernel (FETCH_SIZE / WRITE_SIZE
 # passes, calibrated against a copy kernel of known size)
-PASS_KERNEL = {"stencil": "k_stencil", "curvature": "k_rhs_curv", "assembly": "k_rk4_stage",
+PASS_KERNEL = {"stencil": "k_stencil", "assembly": "k_rk4_stage",
                "ghosts": "k_rk4_ghosts_inplace", "prepare": "k_prep"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 
 # Algorithmic HBM bytes of each pass per launch, from what the pass must read
 # and write once (doubles): per interior vertex (I) and per padded vertex (P).
 #  stencil:   reads the 22 fields over the padded block, writes 106 slots
-#  curvature: (separate pass of z4c_rhs) reads 7 fields + 51 slots, writes
-#             12 curvature values
 #  assembly:  (fused step in place: curvature, assembly, stage update,
 #             AlgConstr) reads the 106 slots + 22 fields, and per stage on
 #             average 11 (kept u_n, stages 2-3) + 16.5 (running sum, stages
@@ -53,7 +51,6 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 #             alpha (14), writes g~, A~ (12)
 PASS_BYTES = {
     "stencil": lambda I, P: 8.0 * (106 * I + 22 * P),
-    "curvature": lambda I, P: 8.0 * (7 + 51 + 12) * I,
     "assembly": lambda I, P: 8.0 * (106 + 22 + 11 + 16.5 + 22 + 16.5 + 5.5) * I,
     "ghosts": lambda I, P: 8.0 * (14 + 12) * (P - I),
     "prepare": lambda I, P: 8.0 * (11 + 33) * P,

@@ -152,7 +152,6 @@ struct RhsArgs {
   const double* dv;   // derivative slots from the stencil pass
   const double* mat;  // may be null
   double* rhs;
-  double* cv;         // curvature part, 12 fields per interior point
   RhsParams o;
 };
 
@@ -238,30 +237,15 @@ __device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B,
   return X;
 }
 
-// Algebra pass, curvature part: one thread per interior point.
-// Outputs staged in LDS and written at the end (see k_rhs_alg).
-__global__ void __launch_bounds__(TILE) k_rhs_curv(RhsArgs a) {
-  __shared__ double ko[12 * TILE];
-  int b, i, j, k;
-  if (!tile_point(a.L, b, i, j, k)) return;
-  const int t = threadIdx.x;
-  const DevBlock B = a.blocks[b];
-  const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
-  const PointIO X = point_io(a, B, flat(B, i, j, k), qi);
-  rhs_curvature_point(a.o, X, [&](int n, double v) { ko[n * TILE + t] = v; });
-  double* cv = a.cv + B.ioff * 12;
-  const int NI = B.nint;
-#pragma unroll
-  for (int n = 0; n < 12; ++n) cv[n * NI + qi] = ko[n * TILE + t];
-}
-
-// Algebra pass, assembly part.
-// The outputs are staged in LDS (each thread its own column) and written at
-// the end: with no global store among the loads, the compiler is free to
-// issue the derivative loads early.
+// Algebra pass of Z4cRHS: one thread per interior vertex.  The curvature
+// part (Ricci, D_aD_b alpha; 12 values) is formed first and kept in a
+// per-thread LDS column, then the assembly reads it from there; the 22
+// right-hand-side values are staged in LDS and stored at the end, so that no
+// global store sits among the derivative loads.
 template <bool WITH_SOMMERFELD>
 __global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
   __shared__ double ko[NVAR * TILE];
+  __shared__ double cvs[12 * TILE];
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const int t = threadIdx.x;
@@ -270,9 +254,8 @@ __global__ void __launch_bounds__(TILE) k_rhs_alg(RhsArgs a) {
   const int p = flat(B, i, j, k);
   const int qi = (i - B.il) + B.ni * ((j - B.jl) + B.nj * (k - B.kl));
   const PointIO X = point_io(a, B, p, qi);
-  const Buf CV = make_buf(a.cv + B.ioff * 12, (long long)12 * B.nint * 8);
-  const int fc = B.nint * 8;
-  rhs_assembly_point(a.o, X, [&](int n) { return bload(CV, X.vd, n * fc); },
+  rhs_curvature_point(a.o, X, [&](int n, double v) { cvs[n * TILE + t] = v; });
+  rhs_assembly_point(a.o, X, [&](int n) { return cvs[n * TILE + t]; },
                      [&](int n, double v) { ko[n * TILE + t] = v; });
   if (WITH_SOMMERFELD && B.faces && on_sommerfeld_face(B, i, j, k)) {
     const Pt P = make_pt(p, B.nn1, B.nn2, N);
@@ -1885,7 +1868,6 @@ struct z4c_ctx {
   int ntiles[NBOX] = {};
   double* d1 = nullptr;     // prepared first derivatives (PrepareZ4cDerivatives)
   double* dv = nullptr;     // derivative slots of the RHS stencil pass
-  double* cv = nullptr;     // curvature part of the RHS algebra
   // stencil plan: the workgroup configuration of every block, and per
   // configuration the plane tiles of its blocks
   std::vector<int> blk_cfg;
@@ -1986,7 +1968,6 @@ struct z4c_ctx {
     }
     hipFree(d1);
     hipFree(dv);
-    hipFree(cv);
     for (int q = 0; q < NSTCFG; ++q) {
       hipFree(st_tiles[q]);
       st_tiles[q] = nullptr;
@@ -2017,7 +1998,6 @@ struct z4c_ctx {
     d_blocks = nullptr;
     d1 = nullptr;
     dv = nullptr;
-    cv = nullptr;
     blk_cfg.clear();
     snap = nullptr;
     snap_elems = 0;
@@ -2202,13 +2182,8 @@ int launch_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
   if (p->cowling) return fill(c, rhs, 0.0, c->total_pts * NVAR, s);
   int rc = launch_stencil(c, u, p, s);
   if (rc) return rc;
-  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, c->cv, rhs_params(p)};
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, c->dv, mat, rhs, rhs_params(p)};
   const int g = grid_for(a.L.ntiles);
-  {
-    PassTimer t(c, Z4C_PASS_CURVATURE, s);
-    hipLaunchKernelGGL(k_rhs_curv, dim3(g), dim3(TILE), 0, s, a);
-  }
-  HIPCALL(hipGetLastError());
   {
     PassTimer t(c, Z4C_PASS_ASSEMBLY, s);
     if (sommerfeld)
@@ -2630,8 +2605,7 @@ int z4c_set_blocks(z4c_ctx* c, const z4c_block* blocks, int nblocks,
     return fail(Z4C_ENOMEM, "set_blocks: derivative storage allocation failed");
   // prepared derivatives start at zero (the reference zero-initialises them)
   HIPCALL(hipMemset(c->d1, 0, sizeof(double) * off * ND1));
-  if (hipMalloc(&c->dv, sizeof(double) * doff) != hipSuccess ||
-      hipMalloc(&c->cv, sizeof(double) * ioff * 12) != hipSuccess)
+  if (hipMalloc(&c->dv, sizeof(double) * doff) != hipSuccess)
     return fail(Z4C_ENOMEM, "set_blocks: derivative-slot allocation failed");
   c->blk_cfg = blk_cfg;
   c->hdev = hb;
@@ -2670,7 +2644,7 @@ int z4c_boundary_rhs(z4c_ctx* c, const z4c_params* p, const double* u,
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!p || !u || !rhs) return fail(Z4C_EINVAL, "boundary_rhs: null argument");
-  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, nullptr, nullptr, rhs, nullptr, rhs_params(p)};
+  RhsArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), u, nullptr, nullptr, rhs, rhs_params(p)};
   hipLaunchKernelGGL(k_sommerfeld, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0,
                      (hipStream_t)stream, a);
   HIPCALL(hipGetLastError());
@@ -3285,8 +3259,7 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   for (int st = 1; st <= 4; ++st) {
     const double* in = inplace ? u : ins[st - 1];
     StageArgs a{};
-    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, mat, nullptr, c->cv,
-                  rhs_params(p)};
+    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), in, c->dv, mat, nullptr, rhs_params(p)};
     a.un = (inplace && st > 1) ? W0 : u;
     a.un_int = (inplace && st > 1) ? 1 : 0;
     a.acc = ACC;

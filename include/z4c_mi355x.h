@@ -337,8 +337,9 @@ int z4c_weyl(z4c_ctx* ctx, const z4c_params* p, const double* u, const double* a
 /* Passes of the kernel timer. */
 enum {
   Z4C_PASS_STENCIL = 0,   /* derivative slots of Z4cRHS */
-  Z4C_PASS_CURVATURE = 1, /* Ricci and D_i D_j alpha (z4c_rhs; inside the assembly in z4c_rk4_step) */
-  Z4C_PASS_ASSEMBLY = 2,  /* RHS assembly (in z4c_rk4_step fused with the curvature part, SumBT and AlgConstr) */
+  Z4C_PASS_CURVATURE = 1, /* a separate Ricci / D_i D_j alpha pass: none in this build (both
+                             z4c_rhs and z4c_rk4_step form it inside the assembly) */
+  Z4C_PASS_ASSEMBLY = 2,  /* curvature part + RHS assembly (in z4c_rk4_step also SumBT and AlgConstr) */
   Z4C_PASS_GHOSTS = 3,    /* ghost pass of z4c_rk4_step */
   Z4C_PASS_PREPARE = 4,   /* PrepareZ4cDerivatives */
   Z4C_PASS_BT_SUM = 5,    /* SumBT */
