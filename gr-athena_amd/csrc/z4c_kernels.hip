@@ -1765,9 +1765,12 @@ struct AdjStepArgs {
   int tf_force, force_reg;
 };
 
-__device__ __forceinline__ void adj_alg_point(const AdjStepArgs& a, const double* un, long long p,
+// un and the stage registers a.k[] share the state layout: both are read at
+// the block's offset boff = B.off * NVAR
+__device__ __forceinline__ void adj_alg_point(const AdjStepArgs& a, long long boff, long long p,
                                               long long N, bool interior, double wk, int kreg,
                                               double* bar) {
+  const double* un = a.un + boff;
   double g[6], A[6], bg[6], bA[6];
   double chi = un[V_CHI * N + p], alpha = un[V_ALPHA * N + p];
 #pragma unroll
@@ -1779,7 +1782,7 @@ __device__ __forceinline__ void adj_alg_point(const AdjStepArgs& a, const double
     // the forward state entering AlgConstr: u_n + w k (one register) or the
     // final sum ((u_n + b1 dt k1) + b2 dt k2 + b3 dt k3) + b4 dt k4
     if (kreg >= 0) {
-      const double* kk = a.k[kreg];
+      const double* kk = a.k[kreg] + boff;
       chi = chi + wk * kk[V_CHI * N + p];
       alpha = alpha + wk * kk[V_ALPHA * N + p];
 #pragma unroll
@@ -1792,7 +1795,7 @@ __device__ __forceinline__ void adj_alg_point(const AdjStepArgs& a, const double
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const double ws = bw[s] * a.dt;
-        const double* kk = a.k[s];
+        const double* kk = a.k[s] + boff;
         chi = chi + ws * kk[V_CHI * N + p];
         alpha = alpha + ws * kk[V_ALPHA * N + p];
 #pragma unroll
@@ -1824,15 +1827,15 @@ __global__ void __launch_bounds__(TILE) k_adj_step(AdjStepArgs a) {
   const DevBlock B = a.blocks[b];
   const long long N = B.npts, p = flat(B, i, j, k);
   const bool interior = is_interior(B, i, j, k);
-  const double* un = a.un + B.off * NVAR;
-  const long long cb = c * a.reg + B.off * NVAR;
+  const long long boff = B.off * NVAR;
+  const long long cb = c * a.reg + boff;
   double bar[NVAR];
 #pragma unroll
   for (int n = 0; n < NVAR; ++n) bar[n] = a.bar[cb + n * N + p];
   const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
   const double aw[5] = {0., 0., 0.5, 0.5, 1.0};  // a_s of stage s
   if (a.stage == 5) {
-    adj_alg_point(a, un, p, N, interior, 0.0, -1, bar);
+    adj_alg_point(a, boff, p, N, interior, 0.0, -1, bar);
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
       a.ubar_in[cb + n * N + p] = bar[n];
@@ -1844,7 +1847,7 @@ __global__ void __launch_bounds__(TILE) k_adj_step(AdjStepArgs a) {
     return;
   }
   const int s = a.stage;
-  if (s >= 2) adj_alg_point(a, un, p, N, interior, aw[s] * a.dt, s - 2, bar);
+  if (s >= 2) adj_alg_point(a, boff, p, N, interior, aw[s] * a.dt, s - 2, bar);
 #pragma unroll
   for (int n = 0; n < NVAR; ++n) {
     a.ubar_in[cb + n * N + p] += bar[n];
