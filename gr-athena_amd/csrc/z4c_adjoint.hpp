@@ -933,8 +933,8 @@ struct SlotOut {
 // Linearisation at interior vertex (i, j, k): derivative values the
 // right-hand side reads, with the reference's stencils (the mixed second
 // derivatives as the nested Dx_a(Dx_b u), a < b).
-__host__ __device__ inline void adj_lin_at(const RhsParams& o, const DevBlock& B,
-                                           const double* u, double* lin, int i, int j, int k) {
+__host__ __device__ inline void adj_lin_deriv_at(const DevBlock& B, const double* u, double* lin,
+                                                 int i, int j, int k) {
   const int N = B.npts, nint = B.nint;
   const int st[3] = {1, B.nn1, B.nn1 * B.nn2};
   const int p = i + B.nn1 * (j + B.nn2 * k);
@@ -977,10 +977,26 @@ __host__ __device__ inline void adj_lin_at(const RhsParams& o, const DevBlock& B
       lin[l_ls(n, a) * nint + q] = t * B.lidx[a];
     }
   }
-  // forward intermediates of the right-hand side, from the values above
+}
+
+// Forward intermediates of the right-hand side at interior vertex (i, j, k),
+// from u and the derivatives adj_lin_deriv_at stored (a separate pass: the
+// derivative gathers then run at full occupancy, the register-heavy algebra
+// at its own).
+__host__ __device__ inline void adj_lin_fwd_at(const RhsParams& o, const DevBlock& B,
+                                               const double* u, double* lin, int i, int j, int k) {
+  const int N = B.npts, nint = B.nint;
+  const int p = i + B.nn1 * (j + B.nn2 * k);
+  const int q = iidx(B, i, j, k);
   LinIn X{LinPoint{u, lin, N, p, nint, q}};
   double* fw = lin + (long long)NLIN * nint + q;
   rhs_fwd_point(o, X, [&](int s, double v) { fw[(long long)s * nint] = v; });
+}
+
+__host__ __device__ inline void adj_lin_at(const RhsParams& o, const DevBlock& B,
+                                           const double* u, double* lin, int i, int j, int k) {
+  adj_lin_deriv_at(B, u, lin, i, j, k);
+  adj_lin_fwd_at(o, B, u, lin, i, j, k);
 }
 
 // Pointwise adjoint at interior vertex (i, j, k): writes the point part of

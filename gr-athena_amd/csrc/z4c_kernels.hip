@@ -1598,7 +1598,14 @@ __global__ void __launch_bounds__(TILE) k_adj_lin(AdjArgs a) {
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
-  adj_lin_at(a.o, B, a.u + B.off * NVAR, a.lin + B.ioff * NLINF, i, j, k);
+  adj_lin_deriv_at(B, a.u + B.off * NVAR, a.lin + B.ioff * NLINF, i, j, k);
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_fwd(AdjArgs a) {
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  adj_lin_fwd_at(a.o, B, a.u + B.off * NVAR, a.lin + B.ioff * NLINF, i, j, k);
 }
 
 // Buffer accessors of the pointwise adjoint: one 32-bit lane offset per
@@ -2363,6 +2370,8 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
             c->adj_sb, c->adj_part, reg, sreg, w ? 0 : 1};
   hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   a.L = launch_of(c, BOX_WHOLE);
@@ -2383,6 +2392,8 @@ int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hi
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, nullptr,
             nullptr, nullptr, c->adj_part, 0, 0, 1};
   hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_hnorm, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
