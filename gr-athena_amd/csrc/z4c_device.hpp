@@ -326,7 +326,7 @@ __device__ __forceinline__ double st_diss(const double* v, double cidxd, double 
   out += 70.0 * v[4];
   return out * cidxd * diss;
 }
-__device__ __forceinline__ double st_lx_left(const double* v) {
+__device__ __forceinline__ double st_lx(const double* v, double lidx, double vx) {
   double dl = 0.0;
   dl += 1.0 * v[0];
   dl += -8.0 * v[1];
@@ -335,9 +335,6 @@ __device__ __forceinline__ double st_lx_left(const double* v) {
   dl += 35.0 * v[4];
   dl += 24.0 * v[5];
   dl += -2.0 * v[6];
-  return dl;
-}
-__device__ __forceinline__ double st_lx_right(const double* v) {
   double dr = 0.0;
   dr += -1.0 * v[8];
   dr += 8.0 * v[7];
@@ -346,18 +343,7 @@ __device__ __forceinline__ double st_lx_right(const double* v) {
   dr += -35.0 * v[4];
   dr += -24.0 * v[3];
   dr += 2.0 * v[2];
-  return dr;
-}
-// A vertex uses one of the two one-sided sums.  When every lane of the wave
-// takes the same side (the usual case, the shift being smooth), only that
-// sum is formed; each sum has the same operations in the same order on
-// every path.
-__device__ __forceinline__ double st_lx(const double* v, double lidx, double vx) {
-  const bool neg = vx < 0;
-  if (__all(neg)) return (vx * st_lx_left(v)) * lidx;
-  if (!__any(neg)) return (vx * st_lx_right(v)) * lidx;
-  const double dl = st_lx_left(v), dr = st_lx_right(v);
-  return (neg ? (vx * dl) : (vx * dr)) * lidx;
+  return ((vx < 0) ? (vx * dl) : (vx * dr)) * lidx;
 }
 
 // ---------------------------------------------------------------------------
