@@ -2413,7 +2413,7 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
                         const double* ubar_out, double* ubar_in, double* pbar, hipStream_t s) {
   const long long reg = c->total_pts * NVAR;
   int rc = ensure_buf(c->adj_k, c->adj_k_n, 4 * reg, "rk4_step_vjp");
-  if (!rc) rc = ensure_buf(c->adj_stage, c->adj_stage_n, reg, "rk4_step_vjp");
+  if (!rc) rc = ensure_buf(c->adj_stage, c->adj_stage_n, 3 * reg, "rk4_step_vjp");
   if (!rc) rc = ensure_buf(c->adj_reg, c->adj_reg_n, 3LL * nw * reg, "rk4_step_vjp");
   if (rc) return rc;
   double* K[4] = {c->adj_k, c->adj_k + reg, c->adj_k + 2 * reg, c->adj_k + 3 * reg};
@@ -2424,13 +2424,16 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
   // the right-hand side leaves ghosts untouched: they must read zero
   rc = fill(c, c->adj_k, 0.0, 4 * reg, s);
   if (rc) return rc;
+  // stage states u_2..u_4, formed once in the forward sweep and kept for the
+  // reverse sweep
+  auto stage_buf = [&](int st) -> double* { return c->adj_stage + (long long)(st - 2) * reg; };
   auto stage_state = [&](int st) -> const double* {  // u_s
     if (st == 1) return u;
     const double* kk[1] = {K[st - 2]};
     const double wa[1] = {aw[st]};
-    if (launch_bt(c, NVAR, c->adj_stage, u, 1, kk, wa, dt, s)) return nullptr;
-    if (launch_alg(c, p, c->adj_stage, nullptr, 0, s)) return nullptr;
-    return c->adj_stage;
+    if (launch_bt(c, NVAR, stage_buf(st), u, 1, kk, wa, dt, s)) return nullptr;
+    if (launch_alg(c, p, stage_buf(st), nullptr, 0, s)) return nullptr;
+    return stage_buf(st);
   };
   for (int st = 1; st <= 4; ++st) {
     const double* in = stage_state(st);
@@ -2457,8 +2460,7 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
   hipLaunchKernelGGL(k_adj_step, g, dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   for (int st = 4; st >= 1; --st) {
-    const double* in = stage_state(st);
-    if (!in) return Z4C_EHIP;
+    const double* in = st == 1 ? u : stage_buf(st);
     rc = launch_rhs_vjp(c, p, in, nw, kbar, usbar, pbar, st == 4 ? 0 : 1, s);
     if (rc) return rc;
     a.bar = usbar;
