@@ -163,8 +163,8 @@ def cpu_baseline(blocks, params, adm_fn, budget_s):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "the step runs on one GPU; --gpus must be 1 (the "
+                                   "block list would shard by blocks across GPUs; not run)"}))
         sys.exit(2)
     import torch
     from z4c_amd.z4c import Z4c

@@ -34,7 +34,7 @@ def _oracle_run(geom, params, nsteps, dt):
 
 
 def test_run_gauge_wave_driver_matches_oracle():
-    """The driver (fused steps with the periodic fill, .hst rows at its
+    """run_gauge_wave (fused steps with the periodic fill, .hst rows at its
     cadence) over 4 steps of a 12^3 block against the oracle's run."""
     n, nsteps = 12, 4
     geom, params, z, _ = awa.gauge_wave_setup(n=n, gauge="harmonic")
