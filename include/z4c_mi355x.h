@@ -376,11 +376,12 @@ int z4c_rhs_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, int nw, cons
 /* VJP of one z4c_rk4_step from u_n = u (unchanged): ubar_out holds nw
    cotangents of u_{n+1}, ubar_in receives those of u_n (every vertex) and
    pbar (nw * Z4C_NPBAR, device) the parameter gradients of <ubar_out, u_{n+1}>.
-   The forward step is recomputed with its four stage right-hand sides kept
-   (stage checkpointing); the reverse sweep rebuilds each stage state from
-   u_n and the previous right-hand side and applies the AlgConstr adjoint at
-   every stage.  Z4C_EINVAL with Sommerfeld faces or cowling.  Scratch:
-   5 + 3 nw registers of 22 * elems_per_var doubles on top of z4c_rhs_vjp's. */
+   The forward step is recomputed with its four stage right-hand sides and
+   the stage states u_2..u_4 kept (stage checkpointing); the reverse sweep
+   re-forms AlgConstr's input from u_n and the previous right-hand side and
+   applies the AlgConstr adjoint at every stage.  Z4C_EINVAL with Sommerfeld
+   faces or cowling.  Scratch: 7 + 3 nw registers of 22 * elems_per_var
+   doubles on top of z4c_rhs_vjp's. */
 int z4c_rk4_step_vjp(z4c_ctx* ctx, const z4c_params* p, double dt, const double* u, int nw,
                      const double* ubar_out, double* ubar_in, double* pbar, void* stream);
 
