@@ -1930,6 +1930,7 @@ struct z4c_ctx {
   double* adj_k = nullptr;
   double* adj_stage = nullptr;
   double* adj_reg = nullptr;
+  bool adj_k_zero = false;  // adj_k's ghosts hold zeros (set by the first step VJP)
   long long adj_lin_n = 0, adj_sb_n = 0, adj_part_n = 0, adj_k_n = 0, adj_stage_n = 0,
             adj_reg_n = 0;
 
@@ -2005,6 +2006,7 @@ struct z4c_ctx {
       *q = nullptr;
     }
     adj_lin_n = adj_sb_n = adj_part_n = adj_k_n = adj_stage_n = adj_reg_n = 0;
+    adj_k_zero = false;
     d_blocks = nullptr;
     d1 = nullptr;
     dv = nullptr;
@@ -2412,7 +2414,9 @@ int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hi
 int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,
                         const double* ubar_out, double* ubar_in, double* pbar, hipStream_t s) {
   const long long reg = c->total_pts * NVAR;
+  const long long k_n_before = c->adj_k_n;
   int rc = ensure_buf(c->adj_k, c->adj_k_n, 4 * reg, "rk4_step_vjp");
+  if (c->adj_k_n != k_n_before) c->adj_k_zero = false;  // new allocation
   if (!rc) rc = ensure_buf(c->adj_stage, c->adj_stage_n, 3 * reg, "rk4_step_vjp");
   if (!rc) rc = ensure_buf(c->adj_reg, c->adj_reg_n, 3LL * nw * reg, "rk4_step_vjp");
   if (rc) return rc;
@@ -2421,9 +2425,14 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
   double* kbar = c->adj_reg + (long long)nw * reg;
   double* usbar = c->adj_reg + 2LL * nw * reg;
   const double aw[5] = {0., 0., 0.5, 0.5, 1.0};
-  // the right-hand side leaves ghosts untouched: they must read zero
-  rc = fill(c, c->adj_k, 0.0, 4 * reg, s);
-  if (rc) return rc;
+  // the right-hand side leaves ghosts untouched: they must read zero.  Only
+  // the right-hand side writes these registers, so one fill per allocation
+  // keeps their ghosts zero.
+  if (!c->adj_k_zero) {
+    rc = fill(c, c->adj_k, 0.0, c->adj_k_n, s);
+    if (rc) return rc;
+    c->adj_k_zero = true;
+  }
   // stage states u_2..u_4, formed once in the forward sweep and kept for the
   // reverse sweep
   auto stage_buf = [&](int st) -> double* { return c->adj_stage + (long long)(st - 2) * reg; };

@@ -214,6 +214,26 @@ def test_step_vjp_several_blocks():
                            atol=1e-12 * np.abs(p_ref).max())
 
 
+def test_step_vjp_repeated_calls_on_one_context():
+    """A second step VJP on the same context (its scratch reused) equals the
+    same call on a fresh context, bit for bit."""
+    g = MeshBlockGeom.cube(8, 1.0 / 8, (0.1, -0.2, 0.05))
+    p = Params(**PUNCTURE)
+    _, u1 = _state(g, p, seed=12, amp=(1e-3, 3e-2))
+    _, u2 = _state(g, p, seed=13, amp=(1e-3, 3e-2))
+    rng = np.random.default_rng(6)
+    w1, w2 = rng.standard_normal(u1.shape), rng.standard_normal(u2.shape)
+    dt = 0.25 / 8
+    z = Z4c([g], p)
+    z.rk4_step_vjp(dt, z.upload([u1]), z.upload([w1]), 1)
+    got, pgot = z.rk4_step_vjp(dt, z.upload([u2]), z.upload([w2]), 1)
+    z2 = Z4c([g], p)
+    exp, pexp = z2.rk4_step_vjp(dt, z2.upload([u2]), z2.upload([w2]), 1)
+    torch.cuda.synchronize()
+    assert torch.equal(got, exp)
+    assert torch.equal(pgot, pexp)
+
+
 def test_step_vjp_finite_differences():
     """Central differences of the GPU fused step (e = 1e-6).  eps_floor is set
     far below zero so that AlgConstr's regularize_nth_root takes its pow
