@@ -30,7 +30,8 @@ sys.path.insert(0, os.path.join(ROOT, "gr-athena_amd"))
 
 import numpy as np  # noqa: E402
 
-PEAK_FP64 = 78.6e12
+PEAK_FP64 = 78.6e12   # FP64 vector, FMA counted as two operations
+PEAK_FP64_NOFMA = 39.3e12  # the same issue rate when every operation is a separate add or mul
 PEAK_HBM = 8.0e12
 # kernel of each timed pass, and the committed counter summary holding the
 # measured HBM bytes per launch of each kernel (FETCH_SIZE / WRITE_SIZE
@@ -39,22 +40,43 @@ PASS_KERNEL = {"stencil": "k_stencil", "assembly": "k_rk4_stage",
                "ghosts": "k_rk4_ghosts_inplace", "prepare": "k_prep"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "z4c_rk4_step_2048blocks_pmc.md")
 
-# Algorithmic HBM bytes of each pass per launch, from what the pass must read
-# and write once (doubles): per interior vertex (I) and per padded vertex (P).
-#  stencil:   reads the 22 fields over the padded block, writes 106 slots
-#  assembly:  (fused step in place: curvature, assembly, stage update,
-#             AlgConstr) reads the 106 slots + 22 fields, and per stage on
-#             average 11 (kept u_n, stages 2-3) + 16.5 (running sum, stages
-#             2-4) doubles; writes 22 (stage state) + 16.5 (running sum,
-#             stages 1-3) + 5.5 (u_n kept by stage 1)
-#  ghosts:    AlgConstr of the ghost vertices in place: reads g~, A~, chi,
-#             alpha (14), writes g~, A~ (12)
-PASS_BYTES = {
-    "stencil": lambda I, P: 8.0 * (106 * I + 22 * P),
-    "assembly": lambda I, P: 8.0 * (106 + 22 + 11 + 16.5 + 22 + 16.5 + 5.5) * I,
-    "ghosts": lambda I, P: 8.0 * (14 + 12) * (P - I),
-    "prepare": lambda I, P: 8.0 * (11 + 33) * P,
+# FLOPs of one block of n^3 cells, counted exactly by the FLOP-counting build
+# of the oracle (oracle/flops/fdouble.hpp: additions, subtractions,
+# multiplications, divisions and libm calls of the restated algorithm; the
+# perturbed flat state, moving-puncture gauge; tests/test_oracle_flops.py
+# recounts them and pins the stencil share against the operator tally):
+#   (stencil share of one RHS: every Dx, Dxx, Lx, Diss of PrepareZ4cDerivatives
+#    and Z4cRHS; the rest of one RHS; one whole bt_rk4 step: 4 RHS, SumBT,
+#    AlgConstr over the padded block)
+FLOPS_BLOCK = {
+    16: (22187346, 14724327, 157209199),
+    24: (69053250, 46828191, 485519215),
+    32: (157149234, 107703255, 1101589615),
 }
+# Algorithmic HBM bytes per owned point-step (classical RK4 with the minimal
+# fused 4-register scheme: 17 register passes of 22 FP64 values) and per RHS
+# (read 22, write 22).
+B_STEP = 17 * 22 * 8
+B_RHS = 2 * 22 * 8
+
+
+def pass_flops(name, blocks):
+    """Algorithmic FLOPs of one launch of a pass over ``blocks``: the
+    stencil pass carries the stencil share of one RHS; the stage kernel
+    (curvature, assembly, stage update, AlgConstr) a quarter of the rest of
+    the step; the other passes carry data movement only."""
+    tot = 0
+    for g in blocks:
+        st, _, step = FLOPS_BLOCK[g.nx[0]]
+        if name == "stencil":
+            tot += st
+        elif name == "assembly":
+            tot += (step - 4 * st) / 4.0
+    return float(tot)
+
+
+def step_flops(blocks):
+    return float(sum(FLOPS_BLOCK[g.nx[0]][2] for g in blocks))
 
 
 def measured_traffic(kernel):
@@ -222,19 +244,33 @@ def main():
     passes = z.get_timing()
     log(f"{args.steps} eager steps with pass timers in {T_eager:.3f} s")
     pinfo = {}
+    step_f = step_flops(blocks)
     for name, (ms, n) in passes.items():
         if n == 0:
             continue
         avg = ms / n * 1e-3
         e = {"launches": n, "avg_ms": avg * 1e3}
-        if name in PASS_BYTES:
-            bts = PASS_BYTES[name](interior_pts, z.total_points)
+        fl = pass_flops(name, blocks)
+        # algorithmic bytes of the pass: the stencil pass reads the 22
+        # fields once per owned point; the stage kernel moves the rest of a
+        # stage's share of the 2992 B per owned point-step
+        bts = {"stencil": 22 * 8.0 * owned, "assembly": (B_STEP / 4.0 - 22 * 8.0) * owned}.get(name)
+        if fl > 0:
+            e["alg_flops"] = fl
+            e["tflops"] = fl / avg / 1e12
+            e["frac_fp64"] = fl / avg / PEAK_FP64
+        if bts:
             e["alg_bytes"] = bts
             e["gbs"] = bts / avg / 1e9
             e["frac_hbm"] = bts / avg / PEAK_HBM
+        tr = measured_traffic(PASS_KERNEL.get(name, ""))
+        if tr:
+            e["measured_traffic_bytes"] = tr
+            e["measured_gbs"] = tr / avg / 1e9
         pinfo[name] = e
         log(f"  {name:10s} {n:4d} launches, avg {avg*1e3:8.3f} ms"
-            + (f", {e['gbs']:7.1f} GB/s ({e['frac_hbm']*100:5.1f}% of HBM)" if "gbs" in e else ""))
+            + (f", {e['tflops']:6.2f} TF/s ({e['frac_fp64']*100:5.1f}% of FP64)" if "tflops" in e else "")
+            + (f", measured {e['measured_gbs']:7.1f} GB/s" if "measured_gbs" in e else ""))
 
     out = {
         "metric": "FP64 Z4c point-updates/s, batched vs per-block launch; adjoint/forward cost",
@@ -251,13 +287,33 @@ def main():
                    "parallelism": "single GPU",
                    "stencil_threads": {"x".join(map(str, k)): v for k, v in sorted(plan.items())}},
     }
-    dom = max((k for k in pinfo if "gbs" in pinfo[k]), key=lambda k: pinfo[k]["avg_ms"] * pinfo[k]["launches"])
+    # roofline of the dominant kernel (largest total time): algorithmic FLOPs
+    # (counted, FLOPS_BLOCK) and bytes per launch over its event-timed
+    # average duration; the binding one is the larger fraction
+    dom = max((k for k in pinfo if "tflops" in pinfo[k]),
+              key=lambda k: pinfo[k]["avg_ms"] * pinfo[k]["launches"])
     d = pinfo[dom]
-    out["roofline"] = {"bound": "hbm", "achieved": d["gbs"], "peak": PEAK_HBM / 1e9,
-                       "unit": "GB/s", "frac": d["frac_hbm"],
-                       "traffic": measured_traffic(PASS_KERNEL[dom]),
-                       "kernel": dom, "avg_ms": d["avg_ms"], "alg_bytes_per_launch": d["alg_bytes"],
-                       "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+    fr_f, fr_b = d["frac_fp64"], d.get("frac_hbm", 0.0)
+    ms_step = T / args.steps * 1e3
+    step = {"F_flops_per_owned_point_step": step_f / owned, "B_bytes_per_owned_point_step": B_STEP,
+            "tflops": step_f / (ms_step * 1e-3) / 1e12,
+            "frac_fp64": step_f / (ms_step * 1e-3) / PEAK_FP64,
+            "frac_hbm": B_STEP * rate / PEAK_HBM,
+            "frac_fp64_nofma_ceiling": step_f / (ms_step * 1e-3) / PEAK_FP64_NOFMA}
+    step["achieved_frac"] = max(step["frac_fp64"], step["frac_hbm"])
+    out["roofline"] = {"bound": "fp64" if fr_f >= fr_b else "hbm",
+                       "achieved": d["tflops"] if fr_f >= fr_b else d["gbs"],
+                       "peak": PEAK_FP64 / 1e12 if fr_f >= fr_b else PEAK_HBM / 1e9,
+                       "unit": "TFLOP/s" if fr_f >= fr_b else "GB/s",
+                       "frac": max(fr_f, fr_b),
+                       "traffic": d.get("measured_traffic_bytes"),
+                       "kernel": PASS_KERNEL[dom], "pass": dom, "avg_ms": d["avg_ms"],
+                       "alg_flops_per_launch": d["alg_flops"],
+                       "alg_bytes_per_launch": d.get("alg_bytes"),
+                       "frac_hbm_alg": fr_b,
+                       "frac_fp64_nofma_ceiling": d["tflops"] * 1e12 / PEAK_FP64_NOFMA,
+                       "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+                       "step": step}
     out["passes"] = pinfo
 
     if not args.no_adjoint:

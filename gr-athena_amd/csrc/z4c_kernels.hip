@@ -2347,7 +2347,8 @@ int ensure_buf(double*& p, long long& have, long long need, const char* what) {
   return Z4C_OK;
 }
 
-constexpr int MAX_NW = 65535 / NPBAR;  // grid.y / grid.x limits of the adjoint kernels
+constexpr int MAX_NW = Z4C_MAX_NW;  // grid.y / grid.x limits of the adjoint kernels
+static_assert(MAX_NW * NPBAR <= 65535 && Z4C_NPBAR == NPBAR, "cotangent limit");
 
 int check_vjp_args(const z4c_ctx* c, const z4c_params* p, int nw, const char* what) {
   if (nw < 1 || nw > MAX_NW)
@@ -2408,9 +2409,11 @@ int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hi
   return Z4C_OK;
 }
 
-// Adjoint of the fused bt_rk4 step: forward pass storing k_1..k_4, then the
-// reverse sweep stage by stage, each stage input u_s recomputed from u_n and
-// k_{s-1} (stage checkpointing).
+// Adjoint of the fused bt_rk4 step: a forward sweep forms the stage states
+// u_2..u_4 once (u_s = AlgConstr(u_n + a_s dt k_{s-1})) and keeps them with
+// k_1..k_4 (stage checkpointing: four right-hand sides and three states per
+// step, nothing per cotangent); the reverse sweep then runs stage by stage
+// from those kept states.
 int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,
                         const double* ubar_out, double* ubar_in, double* pbar, hipStream_t s) {
   const long long reg = c->total_pts * NVAR;
@@ -2426,8 +2429,12 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
   double* usbar = c->adj_reg + 2LL * nw * reg;
   const double aw[5] = {0., 0., 0.5, 0.5, 1.0};
   // the right-hand side leaves ghosts untouched: they must read zero.  Only
-  // the right-hand side writes these registers, so one fill per allocation
-  // keeps their ghosts zero.
+  // launch_rhs writes these registers, and it writes interior vertices only
+  // on every path that reaches this point (Cowling, whose right-hand side
+  // fills whole blocks, and Sommerfeld faces, whose boundary update is not
+  // part of this adjoint, are rejected by z4c_rk4_step_vjp before the
+  // sweep), so one zero fill per allocation keeps their ghosts zero.  A
+  // path that lets launch_rhs write ghosts must clear adj_k_zero.
   if (!c->adj_k_zero) {
     rc = fill(c, c->adj_k, 0.0, c->adj_k_n, s);
     if (rc) return rc;

@@ -22,8 +22,6 @@ input.
 """
 from __future__ import annotations
 
-import math
-
 import numpy as np
 
 from ._capi import Params
@@ -73,34 +71,138 @@ def constraint_history(z, u):
     return z.history(u, con)
 
 
+class HistorySchedule:
+    """The main loop's time stepping around the history output, as the
+    reference runs it with task_triggers/adjust_mesh_dt = true (its default,
+    src/main.cpp:114-125):
+
+    * the Z4c_ADM_constraints/hst and hst triggers (dt = the hst output's
+      dt, force_first_iter; PopulateTrigger, src/main_triggers.hpp:565-589)
+      shorten the step that would pass their next time so that it lands on
+      it (Trigger::reduce_mesh_dt, :80-130, via AdjustFromAny_mesh_dt);
+    * after the step: the constraints are evaluated when the trigger is
+      satisfied (Z4c_DerivedQuantities, src/main.hpp:970), the triggers
+      advance (Trigger::update, :59-77), time += dt, and the next dt is the
+      block dt, or min(2 dt, block dt) when no trigger shortened the step
+      (Mesh::NewTimeStepBegin, src/mesh/mesh.cpp:1822-1850);
+    * a history row is written when time >= next_time (next_time starts at
+      the start time and grows by the output dt, src/outputs/outputs.cpp:
+      208, 1405-1412, src/outputs/history.cpp:451), inside the loop only
+      while time < tlim, and once more after it (the final outputs,
+      src/main.cpp:356-361);
+    * dt is clamped to tlim - time (Mesh::NewTimeStepFinish,
+      src/mesh/mesh.cpp:1908-1909).
+
+    ``iterations()`` yields one dict per loop iteration: ``dt`` of the step,
+    ``constraints`` (evaluate them on the new state), ``row`` (write a row
+    after the step) with its ``time`` and ``row_dt`` (the mesh dt when the
+    row is written, the .hst dt column)."""
+
+    def __init__(self, block_dt, dt_hst, tlim, time=0.0):
+        self.block_dt = float(block_dt)
+        self.tlim = float(tlim)
+        self.time = float(time)
+        self.dt = self.block_dt
+        if self.time < self.tlim and self.tlim - self.time < self.dt:
+            self.dt = self.tlim - self.time
+        # trigger state (both triggers share dt and force_first_iter)
+        self.tdt = float(dt_hst)
+        nd = int(self.time / self.tdt) if self.tdt > 0 else 0
+        self.t_last = nd * self.tdt
+        self.t_next = min(self.time + self.dt, (nd + 1) * self.tdt)
+        self.force_first = True
+        self.next_time = self.time  # the hst output's next_time
+
+    def _satisfied(self):
+        if self.tdt <= 0:
+            return False
+        if self.force_first:
+            return True
+        t_end = self.time + self.dt
+        if self.t_next > self.tlim:  # the last point(s): trigger dt -> mesh dt
+            self.tdt = self.dt
+            return True
+        return (t_end >= self.t_next) or (self.time >= self.tlim)
+
+    def _reduce(self):
+        t_end = self.time + self.dt
+        with np.errstate(divide="ignore", invalid="ignore"):
+            diff_tol = abs(1.0 - np.float64(self.t_next) / np.float64(self.time)) > 1e-13
+        if t_end > self.t_next and self.time < self.t_next and diff_tol:
+            self.dt = self.t_next - self.time
+            self.t_next = self.time + self.dt
+            self.t_last = self.t_next - self.tdt
+            return True
+        return False
+
+    def _update(self):
+        self.t_last = self.t_next
+        if self.force_first:
+            self.t_next = (int(self.t_last / self.tdt) + 1) * self.tdt
+        else:
+            self.t_next = self.t_last + self.tdt
+        self.force_first = False
+
+    def initial_row(self):
+        """The row of the initial data (the outputs made before the loop)."""
+        t = self.time
+        self.next_time += self.tdt
+        return t, self.dt
+
+    def iterations(self):
+        while self.time < self.tlim:
+            adjusted = self._satisfied() and self._reduce()
+            dt = self.dt
+            con = self._satisfied()
+            if self._satisfied():
+                self._update()
+            self.time += dt
+            self.dt = self.block_dt if adjusted else min(2.0 * self.dt, self.block_dt)
+            row = False
+            if self.time < self.tlim and self.time >= self.next_time:
+                row = True
+                self.next_time += self.tdt
+            if self.time < self.tlim and self.tlim - self.time < self.dt:
+                self.dt = self.tlim - self.time
+            if self.time >= self.tlim:
+                row = True  # the final outputs
+            yield {"dt": dt, "constraints": con, "row": row, "time": self.time,
+                   "row_dt": self.dt}
+
+
 def run_gauge_wave(n=64, t_end=10.0, cfl=0.25, amplitude=0.01, d=1.0, gauge="harmonic",
-                   hst_every=None, hst_path=None, progress=None):
-    """Evolve to ``t_end``; returns (times, rows, final state on the host).
-    ``hst_every`` (steps) defaults to the step count of t = 0.1; rows hold the
-    eight Z4c history columns.  ``hst_path`` writes the rows as a .hst file."""
+                   dt_hst=0.1, hst_path=None, progress=None):
+    """Evolve to ``t_end`` with the reference's output-driven step control
+    (HistorySchedule); returns (times, dts, rows, final state on the host,
+    geometry, steps taken).  Rows hold the eight Z4c history columns at the
+    initial data and at every output time; ``dts`` is each row's dt column.
+    ``hst_path`` writes them as a .hst file."""
     import torch
     geom, params, z, u = gauge_wave_setup(n, 1.0, amplitude, d, gauge)
-    dt = z.NewBlockTimeStep(cfl)
-    nsteps = int(round(t_end / dt))
-    if not math.isclose(nsteps * dt, t_end, rel_tol=1e-12):
-        raise ValueError("t_end must be a whole number of steps of cfl*dx")
-    hst_every = hst_every or max(1, int(round(0.1 / dt)))
+    sched = HistorySchedule(z.NewBlockTimeStep(cfl), dt_hst, t_end)
     work = z.make_step_work()
-    times, rows = [0.0], [constraint_history(z, u)]
-    for s in range(1, nsteps + 1):
+    t0, dt0 = sched.initial_row()
+    times, dts, rows = [t0], [dt0], [constraint_history(z, u)]
+    nsteps = 0
+    for it in sched.iterations():
         # the fused step with the periodic fill after every stage (the
         # stage sequence's order and sums; ADMConstraints prepares its own
         # derivatives)
-        z.rk4_step_fused(dt, u, work, periodic=True)
-        if s % hst_every == 0 or s == nsteps:
-            times.append(s * dt)
+        z.rk4_step_fused(it["dt"], u, work, periodic=True)
+        nsteps += 1
+        if it["row"]:
+            if not it["constraints"]:
+                raise RuntimeError("history row without a constraint evaluation")
+            times.append(it["time"])
+            dts.append(it["row_dt"])
             rows.append(constraint_history(z, u))
             if progress:
-                progress(s, nsteps, rows[-1])
+                progress(nsteps, it["time"], rows[-1])
     torch.cuda.synchronize()
     if hst_path:
-        write_hst(hst_path, times, rows, dt)
-    return np.array(times), np.array(rows), z.download(u, 22)[0], geom
+        write_hst(hst_path, times, rows, dts, nbtotal=1)
+    return (np.array(times), np.array(dts), np.array(rows), z.download(u, 22)[0], geom,
+            nsteps)
 
 
 def exact_lapse(geom, t, amplitude=0.01, d=1.0):

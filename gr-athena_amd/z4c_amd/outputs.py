@@ -100,13 +100,47 @@ def read_tab(path):
     return info, names, data
 
 
-def write_hst(path, times, rows, dt):
-    """History file in the reference's layout (history.cpp:385-445): header
-    lines, then time, dt and the Z4c columns, %.18g."""
-    cols = ["time", "dt", "H-norm2", "M-norm2", "Mx-norm2", "My-norm2", "Mz-norm2",
-            "Z-norm2", "Theta-norm2", "C-norm2"]
+HST_Z4C_COLUMNS = ["H-norm2", "M-norm2", "Mx-norm2", "My-norm2", "Mz-norm2", "Z-norm2",
+                   "Theta-norm2", "C-norm2"]
+
+
+def write_hst(path, times, rows, dts, nbtotal, data_format="%12.5e", git_hash="unknown"):
+    """History file of a vacuum Z4c run in the reference's layout
+    (HistoryOutput::WriteOutputFile, src/outputs/history.cpp:383-447):
+
+    * first line ``# GR-Athena++ (<GIT_HASH>) history data`` (:389-392);
+    * the column line ``# [1]=time [2]=dt [3]=N_MeshBlock [4]=H-norm2 ...``,
+      each entry followed by one blank (:394-432);
+    * per row: time and dt with ``" " + data_format``, the block count with
+      ``" %d"``, then the eight Z4c columns with ``" " + data_format``
+      (:436-443); ``data_format`` defaults to ``%12.5e`` and gets its
+      leading blank as in src/outputs/outputs.cpp:322-325.
+
+    ``dts`` is one dt per row (the mesh dt when the row is written) or a
+    single value for every row."""
+    fmt = " " + data_format
+    dts = np.broadcast_to(np.asarray(dts, dtype=float), (len(times),))
+    cols = ["time", "dt", "N_MeshBlock"] + HST_Z4C_COLUMNS
     with open(path, "w") as f:
-        f.write("# Athena++ history data\n")
-        f.write("# " + " ".join(f"[{i + 1}]={c}" for i, c in enumerate(cols)) + " \n")
-        for t, r in zip(times, rows):
-            f.write(" ".join("%.18g" % v for v in [t, dt] + list(r)) + "\n")
+        f.write(f"# GR-Athena++ ({git_hash}) history data\n")
+        f.write("# " + "".join(f"[{i + 1}]={c} " for i, c in enumerate(cols)) + "\n")
+        for t, dt, r in zip(times, dts, rows):
+            if len(r) != len(HST_Z4C_COLUMNS):
+                raise ValueError("a history row holds the eight Z4c columns")
+            f.write(fmt % t + fmt % dt + " %d" % int(nbtotal) + "".join(fmt % v for v in r) + "\n")
+
+
+def read_hst(path):
+    """Parse a history file: (column names, data array), names from the
+    ``[n]=name`` header entries."""
+    names, data = [], []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                toks = [t for t in line[1:].split() if t.startswith("[") and "]=" in t]
+                if toks:
+                    names = [t.split("]=", 1)[1] for t in toks]
+                continue
+            if line.strip():
+                data.append([float(v) for v in line.split()])
+    return names, np.array(data)

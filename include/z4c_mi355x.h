@@ -363,7 +363,13 @@ int z4c_get_timing(z4c_ctx* ctx, double* ms_total, long long* launches, int npas
    the gauge parameters.  Parameter gradients per cotangent, in this order: */
 enum { Z4C_PBAR_SHIFT_ETA = 0, Z4C_PBAR_DAMP_KAPPA1 = 1, Z4C_PBAR_DAMP_KAPPA2 = 2, Z4C_NPBAR = 3 };
 
-/* VJP of z4c_rhs at state u for nw cotangents at once (1 <= nw <= 2978):
+/* Largest cotangent count of one adjoint call (the per-cotangent grid
+   dimension of the adjoint kernels and the parameter-gradient reduction,
+   nw * Z4C_NPBAR, stay within 65535). */
+#define Z4C_MAX_NW 21845
+
+/* VJP of z4c_rhs at state u for nw cotangents at once
+   (1 <= nw <= Z4C_MAX_NW, Z4C_EINVAL otherwise):
    w holds nw registers of 22 * elems_per_var doubles (interior values are
    read); ubar receives nw registers, ubar_c = (d rhs/d u)^T w_c at every
    vertex of u, ghosts included; pbar (device, nw * Z4C_NPBAR doubles)

@@ -116,25 +116,67 @@ class Geom(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(FLOPS_PATH):
         subprocess.run(["make", "-C", HERE, "-B" if force else "all"],
                        check=True, capture_output=True)
     return LIB_PATH
 
 
+FLOPS_PATH = os.path.join(HERE, "liboracle_z4c_flops.so")
 _lib = None
+_plain = None
+_counting = None
 
 
 def lib():
-    global _lib
+    """The library every wrapper calls: the plain build, or the counting
+    build inside ``counting()``."""
+    global _lib, _plain
     if _lib is None:
-        _lib = C.CDLL(build())
+        if _plain is None:
+            _plain = _setup(C.CDLL(build()))
+        _lib = _plain
+    return _lib
+
+
+class counting:
+    """Context manager: wrapper calls inside it run the FLOP-counting build
+    (oracle/flops/fdouble.hpp; same results) and ``counts`` then holds the
+    additions, subtractions, multiplications, divisions and libm calls they
+    performed, split into ``stencil`` (inside the finite-difference
+    functions) and ``other``."""
+
+    def __enter__(self):
+        global _lib, _counting
+        lib()
+        if _counting is None:
+            build()
+            _counting = _setup(C.CDLL(FLOPS_PATH))
+            _counting.oracle_flops_get.argtypes = [C.POINTER(C.c_ulonglong)]
+        _counting.oracle_flops_reset()
+        _lib = _counting
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        out = (C.c_ulonglong * 10)()
+        _counting.oracle_flops_get(out)
+        ops = ("add", "sub", "mul", "div", "call")
+        self.other = dict(zip(ops, (int(v) for v in out[:5])))
+        self.stencil = dict(zip(ops, (int(v) for v in out[5:])))
+        self.counts = {k: self.other[k] + self.stencil[k] for k in ops}
+        _lib = _plain
+        return False
+
+
+def _setup(L):
+    """Argument types of every entry point of a loaded oracle library."""
+    if True:
         P = C.POINTER
         d = C.c_double
         pd = P(C.c_double)
         pg = P(Geom)
         pp = P(Params)
-        L = _lib
         L.oracle_fd_point.restype = d
         L.oracle_fd_point.argtypes = [C.c_int, pg, pd, C.c_long, C.c_int]
         L.oracle_lx_point.restype = d
@@ -162,7 +204,7 @@ def lib():
                                         C.c_ulonglong]
         L.oracle_rst_idrec.restype = C.c_ulong
         L.oracle_rst_idrec.argtypes = [ub, P(C.c_longlong), C.c_int, d]
-    return _lib
+    return L
 
 
 def _p(a):

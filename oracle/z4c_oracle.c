@@ -22,6 +22,13 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* Marks the finite-difference functions: the FLOP-counting build
+   (flops/fdouble.hpp) attributes their operations to the stencil share;
+   empty in the plain build. */
+#ifndef ORACLE_FD_SCOPE
+#define ORACLE_FD_SCOPE
+#endif
+
 /* SYM2 component order xx,xy,xz,yy,yz,zz (src/athena_tensor.hpp:57-179). */
 static const int S[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
 
@@ -57,6 +64,7 @@ static void fd_init(const oracle_geom* g, fdop* f) {
 /* FDStencilCenteredDegreeOdd<1,3>: {-1, 9, -45} / 60  (stencils.tpp:264-270);
    Dx symmetrised branch finite_differencing.tpp:99-110. */
 static inline double Dx(const fdop* f, const double* u, int d) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   const double* pu = u - 3 * s;
   double out = 0.0;
@@ -69,6 +77,7 @@ static inline double Dx(const fdop* f, const double* u, int d) {
 /* FDStencilCenteredDegreeEven<2,3>: {2,-27,270,-490} / 180 (stencils.tpp:330-336);
    Dxx symmetrised branch finite_differencing.tpp:225-234. */
 static inline double Dxx(const fdop* f, const double* u, int d) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   const double* pu = u - 3 * s;
   double out = 0.0;
@@ -81,6 +90,7 @@ static inline double Dxx(const fdop* f, const double* u, int d) {
 
 /* FDStencilCenteredDegreeOdd<1,1>: {-1} / 2; Ds symmetrised branch .tpp:134-143. */
 static inline double Ds(const fdop* f, const double* u, int d) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   const double* pu = u - s;
   double out = 0.0;
@@ -91,6 +101,7 @@ static inline double Ds(const fdop* f, const double* u, int d) {
 /* FDStencilCenteredDegreeEven<8,4>: {1,-8,28,-56,70} (stencils.tpp:395-401);
    Diss symmetrised branch finite_differencing.tpp:339-350. */
 static inline double Diss(const fdop* f, const double* u, int d, double diss) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   const double* pu = u - 4 * s;
   double out = 0.0;
@@ -109,18 +120,21 @@ static const double LX_L[7] = {1, -8, 30, -80, 35, 24, -2};
 static const double LX_R[7] = {2, -24, -35, 80, -30, 8, -1};
 
 static inline double Lx_left(const fdop* f, const double* u, int d) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   double dl = 0.0;
   for (int n = 0; n < 7; ++n) dl += LX_L[n] * u[(n - 4) * s];
   return dl;
 }
 static inline double Lx_right(const fdop* f, const double* u, int d) {
+  ORACLE_FD_SCOPE;
   const long s = f->s[d];
   double dr = 0.0;
   for (int n = 6; n >= 0; --n) dr += LX_R[n] * u[(n - 2) * s];
   return dr;
 }
 static inline double Lx(const fdop* f, const double* u, int d, double vx) {
+  ORACLE_FD_SCOPE;
   const double dl = Lx_left(f, u, d);
   const double dr = Lx_right(f, u, d);
   return ((vx < 0) ? (vx * dl) : (vx * dr)) * f->lidx[d];

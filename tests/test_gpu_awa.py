@@ -34,21 +34,40 @@ def _oracle_run(geom, params, nsteps, dt):
 
 
 def test_run_gauge_wave_driver_matches_oracle():
-    """run_gauge_wave (fused steps with the periodic fill, .hst rows at its
-    cadence) over 4 steps of a 12^3 block against the oracle's run."""
-    n, nsteps = 12, 4
+    """run_gauge_wave (fused steps with the periodic fill, the reference's
+    output-driven step control, .hst rows at every output time) on a 12^3
+    block to t = 0.05 with rows every 0.025 against the oracle's run under
+    the same schedule: dt = 1/48 steps shortened to land on each output
+    time, so the run takes 4 steps and writes rows at 0, 0.025 and 0.05."""
+    n, t_end, dt_hst = 12, 0.05, 0.025
     geom, params, z, _ = awa.gauge_wave_setup(n=n, gauge="harmonic")
-    dt = z.NewBlockTimeStep(0.25)
-    times, rows, got, _ = awa.run_gauge_wave(n=n, t_end=nsteps * dt, hst_every=2)
-    assert len(times) == 3 and np.isclose(times[-1], nsteps * dt)
-    exp_u, exp_rows = _oracle_run(geom, params, nsteps, dt)
-    num = np.abs(got - exp_u).reshape(22, -1).max(axis=1)
-    den = np.maximum(np.abs(exp_u).reshape(22, -1).max(axis=1), 0.01)
+    times, dts, rows, got, _, nsteps = awa.run_gauge_wave(n=n, t_end=t_end, dt_hst=dt_hst)
+    np.testing.assert_allclose(times, [0.0, 0.025, 0.05], rtol=0, atol=1e-15)
+    assert nsteps == 4
+    np.testing.assert_allclose(dts[:2], [1.0 / 48] * 2, rtol=1e-15)
+    o = orc.Params(**params.as_dict())
+    g = orc.Geom.block(geom.nx, geom.dx, geom.xmin)
+    u = orc.adm_to_z4c(o, g, np.ascontiguousarray(gauge_wave_adm(geom, A=0.01, d=1.0)))
+    orc.periodic_fill(g, u)
+    orc.algconstr_range(o, g, u, 0, g.nn1 - 1, 0, g.nn2 - 1, 0, g.nn3 - 1, 1)
+
+    def hist(u):
+        adm = orc.z4c_to_adm(o, g, u)
+        return orc.history(o, g, u, orc.adm_constraints(o, g, u, adm))
+    sched = awa.HistorySchedule(orc.new_dt(g, 0.25), dt_hst, t_end)
+    sched.initial_row()
+    exp_rows = [hist(u)]
+    for it in sched.iterations():
+        u, _ = orc.rk4_step(o, g, u, it["dt"], between_stages=lambda c: orc.periodic_fill(g, c))
+        if it["row"]:
+            exp_rows.append(hist(u))
+    exp_rows = np.array(exp_rows)
+    num = np.abs(got - u).reshape(22, -1).max(axis=1)
+    den = np.maximum(np.abs(u).reshape(22, -1).max(axis=1), 0.01)
     assert np.all(num / den <= 1e-11), num / den
     eps, V = 1e-12, 1.0
-    exp_sel = exp_rows[[0, 2, 4]]
-    bound = 2 * eps * np.sqrt(np.maximum(rows, exp_sel) * V) + eps * eps * V
-    assert np.all(np.abs(rows - exp_sel) <= bound)
+    bound = 2 * eps * np.sqrt(np.maximum(rows, exp_rows) * V) + eps * eps * V
+    assert np.all(np.abs(rows - exp_rows) <= bound)
 
 
 @pytest.mark.parametrize("gauge", ["harmonic", "puncture"])

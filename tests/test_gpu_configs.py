@@ -1,0 +1,274 @@
+"""The benchmark configurations at their stated sizes, on the GPU.
+
+* 64^3 periodic gauge wave, harmonic gauge, bt_rk4 at CFL 0.25 to t = 10
+  with the reference's output-driven step control: the .hst Z4c columns
+  every 0.1 and the final fields against the oracle's run
+  (tests/golden/gauge_wave_history.npz, written by
+  tests/golden/make_gauge_wave_history.py), and the final lapse against
+  the exact wave (awa_z4c.cpp:298-351) within the error bound the oracle's
+  32^3 run gives under 4th-order convergence.
+* One 32^3-cell block of perturbed flat space: one bt_rk4 step (the fused
+  step and the GR_Z4c stage sequence) against the oracle.
+* The same block: the RHS VJP with 64 Gaussian cotangents in one launch
+  (each equal to its single-cotangent call bit for bit, two against
+  torch.autograd of the restatement, all 64 through the dot-product
+  identity <w, J v> = <J^T w, v> with J v by central differences,
+  eps = 1e-6), gauge-parameter gradients (eta in [1, 2], kappa1 in
+  [0, 0.05]) against central differences, and the step VJP through the
+  dot-product identity.
+* 2048 blocks of 16^3 cells, Brill-Lindquist binary, moving-puncture gauge:
+  10 steps, then the adjoint through the last 2; dJ/d eta and dJ/d kappa1
+  against central differences of J over the window.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from z4c_amd import Params, awa  # noqa: E402
+from z4c_amd._capi import PBAR  # noqa: E402
+from z4c_amd.initial_data import (brill_lindquist_adm, octree_blocks,  # noqa: E402
+                                  perturbed_flat)
+from z4c_amd.z4c import MeshBlockGeom, Z4c  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MP_GAUGE = dict(lapse_oplog=2.0, lapse_harmonicf=1.0, lapse_harmonic=0.0, lapse_advect=1.0,
+                shift_Gamma=1.0, shift_advect=1.0, shift_eta=2.0, diss=0.02,
+                damp_kappa1=0.02, damp_kappa2=0.0, chi_div_floor=1e-5)
+
+
+def _rel(got, ref, floor=1e-300):
+    num = np.abs(got - ref).reshape(got.shape[0], -1).max(axis=1)
+    den = np.maximum(np.abs(ref).reshape(ref.shape[0], -1).max(axis=1), floor)
+    return num / den
+
+
+# ---------------------------------------------------------------------------
+# 64^3 gauge wave to t = 10
+# ---------------------------------------------------------------------------
+def test_gauge_wave_64_to_t10_history_and_fields():
+    fx = np.load(os.path.join(GOLDEN, "gauge_wave_history.npz"))
+    times, dts, rows, u, geom, steps = awa.run_gauge_wave(n=64, t_end=10.0, dt_hst=0.1)
+    print(f"gauge wave 64^3: {steps} steps, {len(rows)} history rows", flush=True)
+    # the step control is host logic: identical schedule
+    assert steps == int(fx["steps_64"])
+    np.testing.assert_array_equal(times, fx["times_64"])
+    np.testing.assert_array_equal(dts, fx["dts_64"])
+    exp = fx["rows_64"]
+    # H-norm2 etc. integrate squares of constraints that are cancellation-
+    # dominated remainders of O(1) terms: relative 1e-5 (the histories'
+    # tolerance), above an absolute floor for the rows at truncation level
+    # near t = 0 (a pointwise difference eps = 1e-12 in a constraint moves
+    # a column by at most 2 eps sqrt(col V) + eps^2 V, V = 1).
+    eps = 1e-12
+    floor = 2 * eps * np.sqrt(np.maximum(rows, exp)) + eps * eps
+    err = np.abs(rows - exp)
+    worst = np.max(err / (1e-5 * np.abs(exp) + floor))
+    print("history: max |gpu - oracle| / (1e-5 |oracle| + floor) =", worst)
+    assert np.all(err <= 1e-5 * np.abs(exp) + floor)
+    # the H-norm2 history is a real signal: it grows from ~1e-18 to ~1e-10
+    assert exp[-1, 0] > 1e6 * exp[0, 0]
+    # final fields: y/z independent, and the x line against the oracle's
+    line = u[:, 4, 4, :]
+    assert np.all(np.abs(u[:, 4:69, 4:69, :] - line[:, None, None, :]) <= 1e-14 *
+                  np.maximum(np.abs(line).max(axis=1), 1e-2)[:, None, None, None])
+    r = _rel(line, fx["line_64"], floor=1e-2)
+    print("final fields: max per-variable relative diff", r.max())
+    assert np.all(r <= 1e-9), r
+    # exact wave: error within the 32^3 oracle run's error / 2^4 (RK4 at
+    # fixed CFL converges at 4th order), with 25% headroom
+    x, _, _ = geom.coords()
+    s = slice(4, 69)
+    a_err = np.max(np.abs(line[18, s] - np.sqrt(1.0 - 0.01 * np.sin(2 * np.pi * (x[s] - 10.0)))))
+    bound = 1.25 * float(fx["alpha_err_32"]) / 16.0
+    print(f"|alpha - exact| = {a_err:.3e}, oracle 64: {float(fx['alpha_err_64']):.3e}, "
+          f"bound {bound:.3e}")
+    assert a_err <= bound
+
+
+# ---------------------------------------------------------------------------
+# 32^3 block: one RK4 step
+# ---------------------------------------------------------------------------
+def _block32(p, seed=0):
+    g = MeshBlockGeom.cube(32, 1.0 / 32, (-0.5, -0.5, -0.5))
+    og = orc.Geom.block(g.nx, g.dx, g.xmin)
+    u = orc.algconstr(orc.Params(**p.as_dict()), og, np.ascontiguousarray(perturbed_flat(g, seed)))
+    return g, og, u
+
+
+def test_rk4_step_32cube_against_oracle():
+    p = Params(**MP_GAUGE)
+    g, og, u = _block32(p)
+    dt = 0.25 / 32
+    exp, _ = orc.rk4_step(orc.Params(**p.as_dict()), og, u.copy(), dt)
+    z = Z4c([g], p)
+    # the fused step (the benchmarked form)
+    ud = z.upload([u])
+    z.rk4_step_fused(dt, ud, z.make_step_work())
+    # the GR_Z4c stage sequence (RHS, SumBT, AlgConstr, prepare per stage)
+    us = z.upload([u])
+    un, k = z.make_rk_registers()
+    z.rk4_step(dt, us, un, k)
+    torch.cuda.synchronize()
+    for name, buf in (("fused", ud), ("stages", us)):
+        got = z.download(buf, 22)[0]
+        r = _rel(got, exp)
+        print(f"{name}: max per-variable relative diff {r.max():.3e}")
+        assert np.all(r <= 1e-12), (name, r)
+    z.close()
+
+
+# ---------------------------------------------------------------------------
+# 32^3 block: adjoints with 64 cotangents
+# ---------------------------------------------------------------------------
+ADJ = dict(MP_GAUGE, shift_eta=1.5, damp_kappa1=0.03)
+
+
+def test_rhs_vjp_64_cotangents_32cube():
+    import z4c_torch as zt
+    p = Params(**ADJ)
+    g, og, u = _block32(p, seed=2)
+    nw = 64
+    z = Z4c([g], p)
+    ud = z.upload([u])
+    reg = 22 * z.total_points
+    gen = torch.Generator(device=ud.device).manual_seed(7)
+    w = torch.randn(nw * reg, dtype=torch.float64, device=ud.device, generator=gen)
+    ubar, pbar = z.rhs_vjp(ud, w, nw)
+    torch.cuda.synchronize()
+    # each cotangent of the batched launch equals its own single call
+    for c in (0, 1, 31, 63):
+        u1, p1 = z.rhs_vjp(ud, w[c * reg:(c + 1) * reg].clone(), 1)
+        torch.cuda.synchronize()
+        assert torch.equal(u1, ubar[c * reg:(c + 1) * reg]), c
+        assert torch.equal(p1[0], pbar[c]), c
+    # two cotangents against torch.autograd of the restatement
+    d = p.as_dict()
+    for c in (0, 63):
+        wc = w[c * reg:(c + 1) * reg].cpu().numpy().reshape(u.shape)
+        ut = torch.from_numpy(u).requires_grad_(True)
+        pt = {k: torch.tensor(d[k], dtype=torch.float64, requires_grad=True) for k in PBAR}
+        s = (zt.rhs(ut, og, dict(d, **pt)) * torch.from_numpy(wc)).sum()
+        gr = torch.autograd.grad(s, [ut] + [pt[k] for k in PBAR])
+        got = ubar[c * reg:(c + 1) * reg].cpu().numpy().reshape(u.shape)
+        r = _rel(got, gr[0].numpy())
+        print(f"cotangent {c}: vs autograd {r.max():.3e}")
+        assert np.all(r <= 1e-10), r
+        pref = np.array([x.item() for x in gr[1:]])
+        assert np.allclose(pbar[c].cpu().numpy(), pref, rtol=1e-10, atol=1e-12 * np.abs(pref).max())
+    # dot-product identity for all 64: <w_c, J v> = <ubar_c, v>, J v by
+    # central differences of the GPU right-hand side (eps = 1e-6).  Only
+    # interior rhs values exist; w is read at the interior.
+    v = torch.randn(reg, dtype=torch.float64, device=ud.device, generator=gen)
+    eps = 1e-6
+    rp, rm = z.zeros(), z.zeros()
+    z.Z4cRHS(ud + eps * v, None, rp)
+    z.Z4cRHS(ud - eps * v, None, rm)
+    Jv = ((rp - rm) / (2 * eps)).view(22, *g.shape)
+    mask = torch.zeros_like(Jv)
+    mask[:, 4:-4, 4:-4, 4:-4] = 1.0
+    W = w.view(nw, 22, *g.shape)
+    lhs = (W * (Jv * mask)[None]).sum(dim=(1, 2, 3, 4))
+    rhs_ = (ubar.view(nw, reg) * v[None]).sum(dim=1)
+    rel = ((lhs - rhs_).abs() / rhs_.abs()).cpu().numpy()
+    print("dot-product identity, 64 cotangents: max rel", rel.max())
+    assert np.all(rel <= 1e-6), rel
+    # parameter gradients: <w_c, d rhs / d eta> and d kappa1 by central differences
+    for k, name in enumerate(PBAR[:2]):
+        x0 = getattr(p, name)
+        h = 1e-6 * max(abs(x0), 1.0)
+        outs = []
+        for sgn in (1, -1):
+            q = Params(**dict(p.as_dict(), **{name: x0 + sgn * h}))
+            zq = Z4c([g], q)
+            r_ = zq.zeros()
+            zq.Z4cRHS(ud, None, r_)
+            outs.append(r_)
+            zq.close()
+        dr = ((outs[0] - outs[1]) / (2 * h)).view(22, *g.shape) * mask
+        fd = (W * dr[None]).sum(dim=(1, 2, 3, 4)).cpu().numpy()
+        an = pbar[:, k].cpu().numpy()
+        relp = np.abs(an - fd) / np.abs(fd)
+        print(f"d/d{name}: max rel {relp.max():.3e}")
+        assert np.all(relp <= 1e-6), (name, relp)
+    z.close()
+
+
+def test_rk4_step_vjp_32cube_dot_product():
+    """Step VJP with stage checkpointing on the 32^3 block: <w, S'(u) v> by
+    central differences of the fused step equals <S'(u)^T w, v> for two
+    cotangents.  eps_floor is set far below zero: AlgConstr's
+    regularize_nth_root switches to a constant branch at det g~ - 1 <
+    eps_floor, which every projected state sits on, so finite differences
+    would straddle that switch."""
+    p = Params(**dict(ADJ, eps_floor=-1e-3))
+    g, og, u = _block32(p, seed=3)
+    z = Z4c([g], p)
+    ud = z.upload([u])
+    reg = 22 * z.total_points
+    gen = torch.Generator(device=ud.device).manual_seed(11)
+    nw = 2
+    w = torch.randn(nw * reg, dtype=torch.float64, device=ud.device, generator=gen)
+    v = torch.randn(reg, dtype=torch.float64, device=ud.device, generator=gen)
+    dt = 0.25 / 32
+    ubar, _ = z.rk4_step_vjp(dt, ud, w, nw)
+    work = z.make_step_work()
+    eps = 1e-6
+    up, um = ud + eps * v, ud - eps * v
+    z.rk4_step_fused(dt, up, work)
+    z.rk4_step_fused(dt, um, work)
+    Sv = (up - um) / (2 * eps)
+    lhs = (w.view(nw, reg) * Sv[None]).sum(dim=1)
+    rhs_ = (ubar.view(nw, reg) * v[None]).sum(dim=1)
+    rel = ((lhs - rhs_).abs() / rhs_.abs()).cpu().numpy()
+    print("step VJP dot-product identity: rel", rel)
+    assert np.all(rel <= 1e-6), rel
+    z.close()
+
+
+# ---------------------------------------------------------------------------
+# 2048 blocks of 16^3: shortened sensitivity run
+# ---------------------------------------------------------------------------
+def test_sensitivity_2048_blocks_fd():
+    from z4c_amd.sensitivity import hamiltonian_sensitivity
+    params = Params(**MP_GAUGE)
+    blocks = octree_blocks(nblocks=2048, seed=0, sizes=(16,), mix=(1.0,))
+    z = Z4c(blocks, params)
+    adm = z.empty(17)
+    for b, g in enumerate(blocks):
+        z.block_view(adm, b, 17).copy_(torch.from_numpy(brill_lindquist_adm(g)))
+    u0 = z.zeros()
+    z.ADMToZ4c(adm, u0)
+    del adm
+    dt = min(z.NewBlockTimeStep(0.25), 1.0)
+    nsteps, window = 10, 2
+    res = hamiltonian_sensitivity(z, u0, dt, nsteps, window)
+    print(f"J = {res['J']:.6e}, dJ = {res['pbar']}", flush=True)
+    w0 = res["u_window"]
+    work = z.make_step_work()
+
+    def J_of(**kw):
+        saved = z.params
+        z.params = Params(**dict(params.as_dict(), **kw))
+        y = w0.clone()
+        for _ in range(window):
+            z.rk4_step_fused(dt, y, work)
+        J = z.hamiltonian_norm(y)
+        z.params = saved
+        return J
+
+    for name in ("shift_eta", "damp_kappa1"):
+        x0 = getattr(params, name)
+        h = 1e-3 * max(abs(x0), 1e-2)
+        fd = (J_of(**{name: x0 + h}) - J_of(**{name: x0 - h})) / (2 * h)
+        an = res["pbar"][name]
+        rel = abs(an - fd) / abs(fd)
+        print(f"dJ/d{name}: adjoint {an:.9e}, central difference {fd:.9e}, rel {rel:.2e}")
+        assert rel <= 1e-6
+    z.close()

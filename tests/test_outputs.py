@@ -47,9 +47,26 @@ def test_tab_dump_with_ghosts_and_single_field(tmp_path):
 
 
 def test_hst_layout(tmp_path):
+    """The reference's history layout for a vacuum Z4c run
+    (src/outputs/history.cpp:389-443): hash line, column line with
+    [3]=N_MeshBlock before the Z4c columns, " %12.5e" values and " %d" for
+    the block count."""
     p = tmp_path / "a.hst"
-    outputs.write_hst(str(p), [0.0, 0.1], [[1.0] * 8, [2.0] * 8], 0.05)
-    lines = p.read_text().splitlines()
-    assert lines[0] == "# Athena++ history data"
-    assert lines[1].startswith("# [1]=time [2]=dt [3]=H-norm2 [4]=M-norm2")
-    assert lines[3].split() == ["0.100000000000000006", "0.0500000000000000028"] + ["2"] * 8
+    outputs.write_hst(str(p), [0.0, 0.1], [[1.0] * 8, [2.5e-11] * 8], [0.05, 0.00234375], 8,
+                      git_hash="abc123 ")
+    text = p.read_text()
+    lines = text.splitlines()
+    assert lines[0] == "# GR-Athena++ (abc123 ) history data"
+    assert lines[1] == ("# [1]=time [2]=dt [3]=N_MeshBlock [4]=H-norm2 [5]=M-norm2 "
+                        "[6]=Mx-norm2 [7]=My-norm2 [8]=Mz-norm2 [9]=Z-norm2 [10]=Theta-norm2 "
+                        "[11]=C-norm2 ")
+    assert lines[2] == "  0.00000e+00  5.00000e-02 8" + "  1.00000e+00" * 8
+    assert lines[3] == "  1.00000e-01  2.34375e-03 8" + "  2.50000e-11" * 8
+    assert text.endswith("\n")
+    names, data = outputs.read_hst(str(p))
+    assert names[:3] == ["time", "dt", "N_MeshBlock"] and names[3:] == outputs.HST_Z4C_COLUMNS
+    assert data.shape == (2, 11)
+    np.testing.assert_array_equal(data[:, 2], [8, 8])
+    # a custom data_format replaces %12.5e (outputs.cpp:322-325 prepends the blank)
+    outputs.write_hst(str(p), [0.1], [[1.0] * 8], 0.05, 1, data_format="%.3f")
+    assert p.read_text().splitlines()[2] == " 0.100 0.050 1" + " 1.000" * 8
