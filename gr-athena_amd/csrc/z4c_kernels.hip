@@ -381,6 +381,11 @@ struct StageArgs {
   int cowling;
   double eps_floor;
   int tf_force, force_reg;
+  // low-storage stage (k_rk4_stage<..., true>): u1 register (interior-
+  // indexed) and the stage's delta, gamma_1, gamma_2 and beta * dt
+  double* u1;
+  double ls_w[4];
+  int ls_last;       // the last stage: u1 is not stored
 };
 
 // The curvature part (Ricci, D_aD_b alpha) is formed in the same kernel and
@@ -391,9 +396,11 @@ __device__ __forceinline__ void st_state(double* p, double v) { __builtin_nontem
 
 // ALG: AlgConstr at the end (the step with ghosts held at u_n); without it
 // the stage leaves the sums for the ghost update and the whole-block
-// AlgConstr that follow it, in the task list's order.
-template <bool WITH_SOMMERFELD, bool ALG>
-__global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
+// AlgConstr that follow it, in the task list's order.  LS: the update is a
+// low-storage 2S stage instead of bt_rk4's (see below).
+template <bool WITH_SOMMERFELD, bool ALG, bool LS = false>
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(2)))
+k_rk4_stage(StageArgs a) {
   __shared__ double ko[NVAR * TILE];  // this point's k_s, staged (see k_rhs_alg)
   __shared__ double cvs[12 * TILE];   // this point's curvature part
   int b, i, j, k;
@@ -429,6 +436,32 @@ __global__ void __launch_bounds__(TILE) k_rk4_stage(StageArgs a) {
   const double wa = a.wa, wb = a.wb;
   const int NI = B.nint;
   double s[NVAR];
+  if (LS) {
+    // IntegrateZ4c's low-storage branch (task_list_gr_z4c.cpp:176-193) at
+    // this vertex, with WeightedAve_'s case split (weighted_ave.cpp:21-132):
+    //   u1 = WeightedAve(u1, u, -, {1, delta, 0})   (u1 = 0 before stage 1)
+    //   u  = WeightedAve(u, u1, -, {gamma_1, gamma_2, 0})
+    //   u += (beta dt) rhs                          (AddZ4cRHS)
+    const double* uin = a.r.u + base;
+    double* u1 = a.u1 + ibase;
+    const double dl = a.ls_w[0], g1 = a.ls_w[1], g2 = a.ls_w[2], bdt = a.ls_w[3];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+      const int ei = n * NI + qi;
+      const double u0 = uin[n * N + p];
+      double v1 = (stage == 1) ? 0.0 : u1[ei];
+      if (dl != 0.0) v1 += dl * u0;
+      if (!a.ls_last) st_state(u1 + ei, v1);
+      double v;
+      if (g1 == 1.0)
+        v = (g2 != 0.0) ? u0 + g2 * v1 : u0;
+      else if (g1 == 0.0)
+        v = (g2 == 1.0) ? v1 : g2 * v1;
+      else
+        v = (g2 != 0.0) ? g1 * u0 + g2 * v1 : u0 * g1;
+      s[n] = v + bdt * ko[n * TILE + t];
+    }
+  } else
 #pragma unroll
   for (int n = 0; n < NVAR; ++n) {
     const int e = n * N + p;
@@ -3360,6 +3393,111 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   return Z4C_OK;
 }
 
+int launch_wave(z4c_ctx* c, double* out, const double* in1, const double* in2, double w0,
+                double w1, double w2, int mode, hipStream_t s) {
+  WaveArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), NVAR, out, in1, in2, {w0, w1, w2}, mode};
+  hipLaunchKernelGGL(k_wave, dim3(grid_for(a.L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+int ghost_update(z4c_ctx* c, double* u, int ghosts, hipStream_t s) {
+  int rc = Z4C_OK;
+  if (ghosts & 1)
+    rc = launch_periodic(c, NVAR, u, s);
+  else if (ghosts & 2)
+    rc = launch_exchange(c, NVAR, u, s);
+  if (!rc && (ghosts & 4)) rc = launch_physical(c, NVAR, u, s);
+  return rc;
+}
+
+int check_ls(const z4c_ls_tableau* t, bool need_2s, const char* what) {
+  if (!t || t->nstages < 1 || t->nstages > Z4C_LS_MAX_STAGES)
+    return fail(Z4C_EINVAL, std::string(what) + ": bad tableau");
+  if (need_2s)
+    for (int q = 0; q < t->nstages; ++q)
+      if (t->gamma_3[q] != 0.0)
+        return fail(Z4C_EINVAL, std::string(what) + ": 3S* stages (gamma_3 != 0) need u2; "
+                                "use z4c_ls_stage");
+  return Z4C_OK;
+}
+
+// Fused low-storage step.  In place (no Sommerfeld face): each stage reads
+// the state only at its own vertex, so the stage writes over the interior
+// of u; u1 lives interior-indexed in work.  With Sommerfeld faces the stage
+// reads neighbouring values of its input, so the stage states alternate
+// between u and the second work register, each stage's output first
+// receiving its input's ghosts.
+int launch_ls_step(z4c_ctx* c, const z4c_params* p, const z4c_ls_tableau* T, double dt,
+                   double* u, double* work, const double* mat, int ghosts, hipStream_t s) {
+  const long long n = c->total_pts * NVAR;
+  double* U1 = work;
+  double* W = work + n;
+  bool faces = false;
+  for (const DevBlock& B : c->hdev) faces = faces || B.faces != 0;
+  double* cur = u;
+  int rc = Z4C_OK;
+  for (int st = 1; st <= T->nstages; ++st) {
+    double* nxt = faces ? (cur == u ? W : u) : u;
+    if (faces) {
+      GhostArgs gc{c->d_blocks, launch_of(c, BOX_GHOST), cur, nxt, nullptr, p->eps_floor,
+                   p->tf_force_regularization, p->force_regularization};
+      PassTimer t(c, Z4C_PASS_GHOSTS, s);
+      hipLaunchKernelGGL(k_ghost_copy, dim3(grid_for(gc.L.ntiles)), dim3(TILE), 0, s, gc);
+      HIPCALL(hipGetLastError());
+    }
+    StageArgs a{};
+    a.r = RhsArgs{c->d_blocks, launch_of(c, BOX_INTERIOR), cur, c->dv, mat, nullptr, rhs_params(p)};
+    a.un = cur;
+    a.acc = U1;
+    a.out = nxt;
+    a.stage = st;
+    a.cowling = p->cowling;
+    a.eps_floor = p->eps_floor;
+    a.tf_force = p->tf_force_regularization;
+    a.force_reg = p->force_regularization;
+    a.u1 = U1;
+    a.ls_w[0] = T->delta[st - 1];
+    a.ls_w[1] = T->gamma_1[st - 1];
+    a.ls_w[2] = T->gamma_2[st - 1];
+    a.ls_w[3] = T->beta[st - 1] * dt;  // dt_scaled (time_integrators.hpp:37-40)
+    a.ls_last = st == T->nstages;
+    if (!p->cowling) {
+      rc = launch_stencil(c, cur, p, s);
+      if (rc) return rc;
+    }
+    {
+      PassTimer t(c, Z4C_PASS_ASSEMBLY, s);
+      const int g = grid_for(a.r.L.ntiles);
+      if (ghosts)
+        hipLaunchKernelGGL((k_rk4_stage<true, false, true>), dim3(g), dim3(TILE), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_rk4_stage<true, true, true>), dim3(g), dim3(TILE), 0, s, a);
+    }
+    HIPCALL(hipGetLastError());
+    if (ghosts) {
+      rc = ghost_update(c, nxt, ghosts, s);
+      if (!rc) rc = launch_alg(c, p, nxt, nullptr, 0, s);
+      if (rc) return rc;
+    } else {
+      // the update leaves ghosts alone; AlgConstr of the whole block
+      // reaches them after every stage
+      GhostArgs ga{c->d_blocks, launch_of(c, BOX_GHOST), nxt, nullptr, nullptr, p->eps_floor,
+                   p->tf_force_regularization, p->force_regularization};
+      PassTimer t(c, Z4C_PASS_GHOSTS, s);
+      hipLaunchKernelGGL(k_rk4_ghosts_inplace, dim3(grid_for(ga.L.ntiles)), dim3(TILE), 0, s,
+                         ga);
+      HIPCALL(hipGetLastError());
+    }
+    cur = nxt;
+  }
+  if (cur != u) {
+    hipLaunchKernelGGL(k_copy, dim3(elementwise_grid(n)), dim3(256), 0, s, u, (const double*)cur, n);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3452,6 +3590,81 @@ int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double*
   rc = launch_alg(c, p, u, nullptr, 0, s);
   if (rc) return rc;
   return launch_prep(c, p, u, nullptr, s);
+}
+
+int z4c_ls_tableau_get(const char* name, z4c_ls_tableau* out) {
+  if (!name || !out) return fail(Z4C_EINVAL, "ls_tableau_get: null argument");
+  // (delta, gamma_1, gamma_2, gamma_3, beta) per stage,
+  // task_list_integrators_low_storage.cpp:41-195 (ONE_3RD / TWO_3RD of
+  // defs.hpp.in:257-258 round to 1/3 and 2/3)
+  struct W { double d, g1, g2, g3, b; };
+  static const std::map<std::string, std::vector<W>> tab = {
+      {"vl2", {{1.0, 0.0, 1.0, 0.0, 0.5}, {0.0, 0.0, 1.0, 0.0, 1.0}}},
+      {"rk1", {{1.0, 0.0, 1.0, 0.0, 1.0}}},
+      {"rk2", {{1.0, 0.0, 1.0, 0.0, 1.0}, {0.0, 0.5, 0.5, 0.0, 0.5}}},
+      {"rk3", {{1.0, 0.0, 1.0, 0.0, 1.0}, {0.0, 0.25, 0.75, 0.0, 0.25},
+               {0.0, 2.0 / 3.0, 1.0 / 3.0, 0.0, 2.0 / 3.0}}},
+      {"rk4", {{1.0, 0.0, 1.0, 0.0, 1.193743905974738},
+               {0.217683334308543, 0.121098479554482, 0.721781678111411, 0.0, 0.099279895495783},
+               {1.065841341361089, -3.843833699660025, 2.121209265338722, 0.0, 1.131678018054042},
+               {0.0, 0.546370891121863, 0.198653035682705, 0.0, 0.310665766509336}}},
+  };
+  const std::string nm(name);
+  if (nm == "ssprk5_4")
+    return fail(Z4C_EINVAL, "ls_tableau_get: ssprk5_4 is not available with GR (the GR_Z4c "
+                            "task list rejects it)");
+  auto it = tab.find(nm);
+  if (it == tab.end()) return fail(Z4C_EINVAL, "ls_tableau_get: unknown integrator " + nm);
+  std::memset(out, 0, sizeof(*out));
+  out->nstages = (int)it->second.size();
+  for (int q = 0; q < out->nstages; ++q) {
+    const W& w = it->second[q];
+    out->delta[q] = w.d;
+    out->gamma_1[q] = w.g1;
+    out->gamma_2[q] = w.g2;
+    out->gamma_3[q] = w.g3;
+    out->beta[q] = w.b;
+  }
+  return Z4C_OK;
+}
+
+int z4c_ls_stage(z4c_ctx* c, const z4c_params* p, const z4c_ls_tableau* t, int stage, double dt,
+                 double* u, double* u1, const double* u2, double* rhs, const double* mat,
+                 int ghosts, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  rc = check_ls(t, false, "ls_stage");
+  if (rc) return rc;
+  if (!p || !u || !u1 || !rhs || stage < 1 || stage > t->nstages || ghosts < 0 || ghosts > 7 ||
+      (ghosts & 3) == 3)
+    return fail(Z4C_EINVAL, "ls_stage: bad argument");
+  const int q = stage - 1;
+  if (t->gamma_3[q] != 0.0 && !u2) return fail(Z4C_EINVAL, "ls_stage: gamma_3 != 0 needs u2");
+  hipStream_t s = (hipStream_t)stream;
+  // StartupTaskList: u1 = 0 at the beginning of each cycle
+  if (stage == 1) {
+    rc = fill(c, u1, 0.0, c->total_pts * NVAR, s);
+    if (rc) return rc;
+  }
+  rc = launch_rhs(c, p, u, mat, rhs, true, s);
+  if (!rc) rc = launch_wave(c, u1, u, u2, 1.0, t->delta[q], 0.0, 0, s);
+  if (!rc) rc = launch_wave(c, u, u1, u2, t->gamma_1[q], t->gamma_2[q], t->gamma_3[q], 0, s);
+  if (!rc) rc = launch_wave(c, u, rhs, nullptr, t->beta[q] * dt, 0.0, 0.0, 1, s);
+  if (!rc) rc = ghost_update(c, u, ghosts, s);
+  if (!rc) rc = launch_alg(c, p, u, nullptr, 0, s);
+  if (rc) return rc;
+  return launch_prep(c, p, u, nullptr, s);
+}
+
+int z4c_ls_step(z4c_ctx* c, const z4c_params* p, const z4c_ls_tableau* t, double dt, double* u,
+                double* work, const double* mat, int ghosts, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  rc = check_ls(t, true, "ls_step");
+  if (rc) return rc;
+  if (!p || !u || !work || ghosts < 0 || ghosts > 7 || (ghosts & 3) == 3)
+    return fail(Z4C_EINVAL, "ls_step: bad argument");
+  return launch_ls_step(c, p, t, dt, u, work, mat, ghosts, (hipStream_t)stream);
 }
 
 int z4c_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const double* w,

@@ -28,6 +28,7 @@ EXPORTS = [
     "z4c_physical_bc", "z4c_rhs_vjp", "z4c_rk4_step_vjp", "z4c_hamiltonian_norm",
     "z4c_hamiltonian_norm_vjp", "z4c_weyl", "z4c_coarse_layout", "z4c_restrict",
     "z4c_prolongation_range", "z4c_set_prolongation", "z4c_prolongate", "z4c_rk4_step_ghosts",
+    "z4c_ls_tableau_get", "z4c_ls_stage", "z4c_ls_step",
 ]
 
 # parameter gradients of the adjoints (Z4C_PBAR_* of the header)
@@ -80,6 +81,22 @@ class Block(C.Structure):
 class ProlongBox(C.Structure):
     """z4c_prolong_box: one ProlongateVertexCenteredValues call."""
     _fields_ = [(n, C.c_int) for n in ("block", "si", "ei", "sj", "ej", "sk", "ek")]
+
+
+Z4C_LS_MAX_STAGES = 5
+
+
+class LsTableau(C.Structure):
+    """z4c_ls_tableau: per-stage weights of a low-storage 2S integrator
+    (TaskLists::Integrators::LowStorage)."""
+    _fields_ = [("nstages", C.c_int), ("pad_", C.c_int)] + [
+        (n, C.c_double * Z4C_LS_MAX_STAGES) for n in ("delta", "gamma_1", "gamma_2", "gamma_3",
+                                                       "beta")]
+
+    def stages(self):
+        """[(delta, gamma_1, gamma_2, gamma_3, beta), ...]"""
+        return [tuple(getattr(self, n)[q] for n in ("delta", "gamma_1", "gamma_2", "gamma_3",
+                                                    "beta")) for q in range(self.nstages)]
 
 
 class Z4cError(RuntimeError):
@@ -143,6 +160,9 @@ def load(path: str | None = None):
         "z4c_prolongation_range": [vp, i, C.POINTER(i), C.POINTER(ll), C.POINTER(ProlongBox)],
         "z4c_set_prolongation": [vp, C.POINTER(ProlongBox), i],
         "z4c_prolongate": [vp, i, vp, vp, vp],
+        "z4c_ls_tableau_get": [C.c_char_p, C.POINTER(LsTableau)],
+        "z4c_ls_stage": [vp, pp, C.POINTER(LsTableau), i, d, vp, vp, vp, vp, vp, i, vp],
+        "z4c_ls_step": [vp, pp, C.POINTER(LsTableau), d, vp, vp, vp, i, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -341,6 +341,46 @@ class Z4c:
         check(self.L.z4c_rk4_step(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                   _ptr(work), _ptr(mat), _stream(stream)), "rk4_step")
 
+    # -- low-storage integrators (task_list_integrators_low_storage.cpp) -------
+    def ls_tableau(self, name):
+        """The reference's low-storage tableau of <time>/integrator ``name``."""
+        t = _capi.LsTableau()
+        check(self.L.z4c_ls_tableau_get(name.encode(), C.byref(t)), "ls_tableau_get")
+        return t
+
+    def ls_stage(self, tableau, stage, dt, u, u1, rhs, u2=None, mat=None, periodic=False,
+                 exchange=False, physical=False, stream=None):
+        """One low-storage GR_Z4c stage (u1 zeroed at stage 1; Z4cRHS,
+        IntegrateZ4c's two WeightedAve and AddZ4cRHS, ghost update,
+        AlgConstr, PrepareZ4cDerivatives); ``tableau`` is a name or an
+        LsTableau."""
+        t = self.ls_tableau(tableau) if isinstance(tableau, str) else tableau
+        mode = (2 if exchange else (1 if periodic else 0)) + (4 if physical else 0)
+        check(self.L.z4c_ls_stage(self._ctx, C.byref(self.params), C.byref(t), int(stage),
+                                  float(dt), _ptr(u), _ptr(u1), _ptr(u2), _ptr(rhs), _ptr(mat),
+                                  mode, _stream(stream)), "ls_stage")
+
+    def ls_step_stages(self, tableau, dt, u, u1, rhs, mat=None, periodic=False, exchange=False,
+                       physical=False, stream=None):
+        """One low-storage step as the task list runs it: ``ls_stage`` for
+        every stage."""
+        t = self.ls_tableau(tableau) if isinstance(tableau, str) else tableau
+        for st in range(1, t.nstages + 1):
+            self.ls_stage(t, st, dt, u, u1, rhs, None, mat, periodic, exchange, physical, stream)
+
+    def ls_step(self, tableau, dt, u, work, mat=None, periodic=False, exchange=False,
+                physical=False, stream=None):
+        """One low-storage step fused per stage (z4c_ls_step); ``work`` from
+        ``make_ls_work``."""
+        t = self.ls_tableau(tableau) if isinstance(tableau, str) else tableau
+        mode = (2 if exchange else (1 if periodic else 0)) + (4 if physical else 0)
+        check(self.L.z4c_ls_step(self._ctx, C.byref(self.params), C.byref(t), float(dt), _ptr(u),
+                                 _ptr(work), _ptr(mat), mode, _stream(stream)), "ls_step")
+
+    def make_ls_work(self):
+        """Work registers of ``ls_step``: u1 and a second state buffer."""
+        return self.empty(2 * Z4C_NVAR)
+
     HISTORY_COLUMNS = ["H-norm2", "M-norm2", "Mx-norm2", "My-norm2", "Mz-norm2",
                        "Z-norm2", "Theta-norm2", "C-norm2"]
 

@@ -291,6 +291,57 @@ int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
 int z4c_rk4_step_ghosts(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                         double* work, const double* mat, int ghosts, void* stream);
 
+/* Low-storage (2S) integrator: per stage the weights delta, gamma_1,
+   gamma_2, gamma_3 and beta of TaskLists::Integrators::LowStorage
+   (src/task_list/task_list_integrators_low_storage.cpp:17-204, the
+   IntegratorWeights of src/task_list/time_integrators.hpp). */
+#define Z4C_LS_MAX_STAGES 5
+typedef struct {
+  int nstages;
+  int pad_;
+  double delta[Z4C_LS_MAX_STAGES];
+  double gamma_1[Z4C_LS_MAX_STAGES];
+  double gamma_2[Z4C_LS_MAX_STAGES];
+  double gamma_3[Z4C_LS_MAX_STAGES];
+  double beta[Z4C_LS_MAX_STAGES];
+} z4c_ls_tableau;
+
+/* The reference's tableau of <time>/integrator `name` (LowStorage
+   constructor, task_list_integrators_low_storage.cpp:41-195): "vl2",
+   "rk1", "rk2", "rk3", "rk4" (Ketcheson RK4()4[2S]; not bt_rk4).
+   "ssprk5_4" and unknown names give Z4C_EINVAL, as GR_Z4c::StartupTaskList
+   rejects ssprk5_4 (src/task_list/gr/task_list_gr_z4c.cpp:106-113).
+   Host only. */
+int z4c_ls_tableau_get(const char* name, z4c_ls_tableau* out);
+
+/* One low-storage stage of the GR_Z4c task list (stage = 1..nstages;
+   src/task_list/gr/task_list_gr_z4c.cpp:45-89): at stage 1 u1 is zeroed
+   (StartupTaskList, :115-119); rhs = Z4cRHS(u) + Z4cBoundaryRHS; then
+   IntegrateZ4c (:170-194): u1 = WeightedAve(u1, u, u2, {1, delta, 0}),
+   u = WeightedAve(u, u1, u2, {gamma_1, gamma_2, gamma_3}),
+   u += (beta dt) rhs (AddZ4cRHS); the ghost update selected by `ghosts`
+   (as for z4c_rk4_stage); AlgConstr; PrepareZ4cDerivatives.  u, u1, rhs
+   (and u2 when some gamma_3 != 0) are state buffers of 22 fields. */
+int z4c_ls_stage(z4c_ctx* ctx, const z4c_params* p, const z4c_ls_tableau* t, int stage,
+                 double dt, double* u, double* u1, const double* u2, double* rhs,
+                 const double* mat, int ghosts, void* stream);
+
+/* One whole low-storage step, fused per stage like z4c_rk4_step: the
+   stencil pass, then one kernel forming the right-hand side, the two
+   weighted averages, AddZ4cRHS and AlgConstr at each interior vertex (the
+   sums and their order of z4c_ls_stage).  `ghosts` = 0 holds the ghosts
+   out of the update (WeightedAve and AddZ4cRHS act on interior vertices),
+   with AlgConstr applied to them after every stage; otherwise the ghost
+   update of z4c_ls_stage (1 periodic fill or 2 exchange, plus 4 physical
+   fill) and AlgConstr of the whole block after every stage.  The
+   derivative arrays are not prepared.  `work` is caller-owned device
+   memory of 2 * 22 * elems_per_var doubles (u1, and a second state buffer
+   that Sommerfeld faces need).  2S tableaus only (every gamma_3 == 0, as
+   for every integrator GR_Z4c accepts).  `u` holds u_n on entry and
+   u_{n+1} on return. */
+int z4c_ls_step(z4c_ctx* ctx, const z4c_params* p, const z4c_ls_tableau* t, double dt,
+                double* u, double* work, const double* mat, int ghosts, void* stream);
+
 /* z4c_rk4_step replayed from a HIP graph (no counterpart in the reference:
    its task list launches nothing).  The first call, and any call whose u,
    work, dt or parameters differ from the captured ones, captures the step's

@@ -356,6 +356,56 @@ def rk4_step(o, g, u, dt, mat=None, between_stages=None):
     return cur, ks
 
 
+# Low-storage 2S integrators (TaskLists::Integrators::LowStorage,
+# src/task_list/task_list_integrators_low_storage.cpp:17-204): per stage
+# (delta, gamma_1, gamma_2, gamma_3, beta).  ONE_3RD / TWO_3RD of
+# src/defs.hpp.in:257-258 round to 1/3 and 2/3.  ssprk5_4 is left out: the
+# GR_Z4c task list rejects it (task_list_gr_z4c.cpp:106-113).
+LS_TABLEAUS = {
+    "vl2": [(1.0, 0.0, 1.0, 0.0, 0.5), (0.0, 0.0, 1.0, 0.0, 1.0)],
+    "rk1": [(1.0, 0.0, 1.0, 0.0, 1.0)],
+    "rk2": [(1.0, 0.0, 1.0, 0.0, 1.0), (0.0, 0.5, 0.5, 0.0, 0.5)],
+    "rk3": [(1.0, 0.0, 1.0, 0.0, 1.0), (0.0, 0.25, 0.75, 0.0, 0.25),
+            (0.0, 2.0 / 3.0, 1.0 / 3.0, 0.0, 2.0 / 3.0)],
+    "rk4": [(1.0, 0.0, 1.0, 0.0, 1.193743905974738),
+            (0.217683334308543, 0.121098479554482, 0.721781678111411, 0.0, 0.099279895495783),
+            (1.065841341361089, -3.843833699660025, 2.121209265338722, 0.0, 1.131678018054042),
+            (0.0, 0.546370891121863, 0.198653035682705, 0.0, 0.310665766509336)],
+}
+
+
+def ls_step(o, g, u, dt, tableau, mat=None, between_stages=None, faces=None, rhs_fn=None,
+            constrain=True):
+    """One low-storage step in the GR_Z4c task order: u1 = 0 at the start
+    (StartupTaskList, task_list_gr_z4c.cpp:115-119), then per stage
+    rhs = Z4cRHS(u) (+ Z4cBoundaryRHS on ``faces``), and IntegrateZ4c
+    (:170-194): u1 = WeightedAve(u1, u, -, {1, delta, 0}), u =
+    WeightedAve(u, u1, u2, {gamma_1, gamma_2, gamma_3}), u += (beta dt) rhs
+    (AddZ4cRHS); the optional ghost update hook; AlgConstr.  ``tableau`` is
+    a name of LS_TABLEAUS or its list.  ``rhs_fn(u)`` replaces the
+    right-hand side and ``constrain=False`` skips AlgConstr (for scalar ODE
+    checks of the integrator).  Returns u (updated in place)."""
+    st = LS_TABLEAUS[tableau] if isinstance(tableau, str) else tableau
+    u1 = np.zeros_like(u)
+    for delta, g1, g2, g3, beta in st:
+        if g3 != 0.0:
+            raise ValueError("3S* stages (gamma_3 != 0) need u2, which GR_Z4c never allocates")
+        if rhs_fn is not None:
+            r = rhs_fn(u)
+        else:
+            r = rhs(o, g, u, mat)
+            if faces is not None:
+                boundary_rhs(o, g, u, faces, r)
+        weighted_ave(g, u1, u, None, [1.0, delta, 0.0])
+        weighted_ave(g, u, u1, None, [g1, g2, g3])
+        add_rhs(g, r, beta * dt, u)
+        if between_stages is not None:
+            between_stages(u)
+        if constrain:
+            algconstr(o, g, u)
+    return u
+
+
 def weyl(o, g, u, adm=None):
     """Z4c::Z4cWeyl: (2, nn3, nn2, nn1) array of Re, Im psi4 (interior; NaN
     elsewhere).  The ADM variables default to Z4cToADM of u."""

@@ -61,3 +61,20 @@ def test_params_layout_matches_header():
     # 17 doubles + 4 ints
     assert ctypes.sizeof(_capi.Params) == 17 * 8 + 4 * 4
     assert ctypes.sizeof(_capi.Block) == 8 + 9 * 4 + 4 + 6 * 8 + 4 * 4
+
+
+def test_low_storage_tableaus_equal_the_oracle():
+    """z4c_ls_tableau_get (host only, no device) returns the reference's
+    LowStorage weights, the ones the oracle's integrator tests pin; the
+    integrators GR_Z4c rejects or does not know give Z4C_EINVAL."""
+    import oracle as orc
+    L = _capi.load()
+    for name, stages in orc.LS_TABLEAUS.items():
+        t = _capi.LsTableau()
+        assert L.z4c_ls_tableau_get(name.encode(), ctypes.byref(t)) == 0
+        assert t.nstages == len(stages)
+        assert t.stages() == [tuple(float(v) for v in st) for st in stages], name
+    t = _capi.LsTableau()
+    for bad in (b"ssprk5_4", b"bt_rk4", b""):
+        assert L.z4c_ls_tableau_get(bad, ctypes.byref(t)) == -1
+    assert ctypes.sizeof(_capi.LsTableau) == 8 + 5 * 5 * 8

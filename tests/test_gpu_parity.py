@@ -219,6 +219,33 @@ def test_batched_uneven_blocks():
         assert np.all(r <= TOL), (b, r)
 
 
+@pytest.mark.parametrize("pw", [-4.0, -2.0])
+def test_prepare_derivatives_aux_output(pw):
+    """PrepareZ4cDerivatives' storage.aux (adm_z4c.cpp:1044-1119): the 30
+    physical ADM derivatives d alpha, d beta, d gamma by the chain rule from
+    the prepared conformal derivatives, both chi_psi_power branches (p = -4
+    :1046-1080, general p after it), interior vertices, on two uneven
+    blocks; vertices outside the interior are not written."""
+    gs = [MeshBlockGeom.cube(8, 0.1, (0.1, -0.2, 0.3)),
+          MeshBlockGeom.cube((12, 10, 6), (0.07, 0.05, 0.09), (1.0, 0.0, 0.0))]
+    p = Params(**dict(MP_GAUGE, chi_psi_power=pw))
+    us = [constrained_state(g, p, seed=80 + b) for b, g in enumerate(gs)]
+    z = Z4c(gs, p)
+    aux = torch.full((30 * z.total_points,), 7.0, dtype=torch.float64, device=z.device)
+    z.PrepareZ4cDerivatives(z.upload(us), aux)
+    torch.cuda.synchronize()
+    got = z.download(aux, 30)
+    for b, g in enumerate(gs):
+        _, _, exp = orc.prep_derivs(oparams(p), ogeom(g), us[b], want_aux=True)
+        s = interior(g)
+        r = rel(got[b][s], exp[s])
+        print(f"block {b}, p = {pw}: aux max rel diff {r.max():.3e}")
+        assert np.all(r <= TOL), r
+        outside = got[b].copy()
+        outside[s] = 7.0
+        assert np.all(outside == 7.0)
+
+
 def test_algconstr_parity():
     g = MeshBlockGeom.cube(8, 0.1, 0.0)
     p = Params()

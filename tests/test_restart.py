@@ -97,3 +97,46 @@ def test_bad_files_are_rejected(tmp_path):
     p.write_bytes(b"no parameters here")
     with pytest.raises(ValueError):
         rst.read_restart(p)
+
+
+def test_layout_by_explicit_offsets(tmp_path):
+    """The file parsed with byte offsets derived by hand from the
+    reference's declarations and write order, independent of the ctypes
+    structs and of the oracle's struct copies:
+
+    RestartOutput::WriteOutputFile (restart.cpp:138-147) writes, after the
+    parameter text: int nbtotal (+0), int root_level (+4), RegionSize (+8;
+    athena.hpp:88-96: nine doubles +0..+71, int nx1 +72, nx2 +76, nx3 +80,
+    4 bytes of tail padding to the struct's 8-byte alignment: 88 bytes),
+    Real time (+96), Real dt (+104), int ncycle (+112), uint64 datasize
+    (+116, io_wrapper.hpp:26) -- 124 bytes, which is headeroffset minus the
+    text and the user data (restart.cpp:115-117).  Each ID record
+    (restart.cpp:215-218) is LogicalLocation (athena.hpp:58-66: int64 lx1
+    +0, lx2 +8, lx3 +16, int level +24, tail padding to 32) then double cost
+    (+32): listsize = 40 (restart.cpp:119).  Block b's data follows at
+    headeroffset + 40 nbtotal + b datasize (restart.cpp:336)."""
+    import struct
+    r = _restart(nb=2)
+    p = tmp_path / "z4c.00001.rst"
+    datasize = rst.write_restart(p, r)
+    raw = p.read_bytes()
+    n = raw.index(b"<par_end>\n") + len(b"<par_end>\n")
+    assert raw[:n].decode() == r.params
+    nbtotal, root_level = struct.unpack_from("<ii", raw, n)
+    assert (nbtotal, root_level) == (2, 1)
+    rs = struct.unpack_from("<9d3i", raw, n + 8)
+    assert list(rs[:9]) == SIZE9 and list(rs[9:]) == [32, 32, 32]
+    time_, dt = struct.unpack_from("<dd", raw, n + 96)
+    (ncycle,) = struct.unpack_from("<i", raw, n + 112)
+    (dsize,) = struct.unpack_from("<Q", raw, n + 116)
+    assert (time_, dt, ncycle, dsize) == (12.5, 0.0625, 200, datasize)
+    head = n + 124  # headeroffset, no user mesh data
+    for b in range(2):
+        lx1, lx2, lx3, level = struct.unpack_from("<3qi", raw, head + 40 * b)
+        (cost,) = struct.unpack_from("<d", raw, head + 40 * b + 32)
+        assert (lx1, lx2, lx3, level) == tuple(r.locs[b]) and cost == r.costs[b]
+    for b in range(2):
+        o = head + 40 * 2 + b * datasize
+        blk = np.frombuffer(raw, dtype="<f8", count=datasize // 8, offset=o)
+        np.testing.assert_array_equal(blk, r.blocks[b].reshape(-1))
+    assert len(raw) == head + 40 * 2 + 2 * datasize
