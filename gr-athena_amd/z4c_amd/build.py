@@ -51,6 +51,27 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+HOST = os.path.join(os.path.dirname(PKG), "host")
+HOST_DEMO = os.path.join(HOST, "z4c_host_demo")
+
+
+def build_host(verbose: bool = False) -> str:
+    """Compile the C++ host driver (host/z4c_host_demo.cpp over
+    host/z4c_host.hpp) against include/z4c_mi355x.h, linked to the library
+    with an rpath to its directory; no device code of its own."""
+    tmp = HOST_DEMO + ".tmp"
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I" + INCLUDE, "-I" + HOST, "-o", tmp,
+           os.path.join(HOST, "z4c_host_demo.cpp"), "-L" + PKG, "-lz4c_mi355x",
+           "-Wl,-rpath,$ORIGIN/../z4c_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HOST_DEMO)
+    return HOST_DEMO
+
+
 def is_stale() -> bool:
     if not os.path.exists(LIB):
         return True
@@ -60,4 +81,5 @@ def is_stale() -> bool:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_host(verbose=True))
     sys.exit(0)
