@@ -151,3 +151,38 @@ def test_polynomial_exactness_and_order():
         x = X.reshape(-1)[q]
         errs.append(abs(orc.fd_point(0, gg, u, q, 0) - 2 * np.pi * np.cos(2 * np.pi * x)))
     assert 50 < errs[0] / errs[1] < 80
+
+
+def test_lx_selection_is_upwind_by_stability():
+    """Pins Lx's selection rule by a property, not by restating it: for the
+    advection the right-hand side carries, du/dt = v Lx(v, u), the
+    semi-discrete operator on a periodic line must not amplify any Fourier
+    mode (real part of its symbol <= 0 for every wavenumber, for either
+    sign of v), which holds only if the stencil's extra points lie upwind.
+    The opposite choice is checked to amplify, so a swapped selection fails."""
+    g = orc.Geom.block(12, 1.0, 0.0)
+    p = 10 + g.nn1 * (10 + g.nn2 * 10)
+    lib = orc.lib()
+    offs = np.arange(-4, 5)
+    theta = np.linspace(0.0, np.pi, 721)
+    for d in range(3):
+        stride = [1, g.nn1, g.nn1 * g.nn2][d]
+
+        def symbol_re(v):
+            w = []
+            u = np.zeros(g.shape)
+            flat = u.reshape(-1)
+            for m in offs:
+                flat[:] = 0.0
+                flat[p + m * stride] = 1.0
+                w.append(lib.oracle_lx_point(orc.C.byref(g), orc._p(u), p, d, v))
+            return (np.array(w)[None, :] * np.cos(np.outer(theta, offs))).sum(1)
+
+        for v in (-0.7, 0.3):
+            re = symbol_re(v)
+            assert re.max() <= 1e-13, (d, v, re.max())
+            assert re.min() < -1e-3  # dissipative, not neutral
+            # the other one-sided stencil at the same v (the one the oracle
+            # selects for -v): v D_other = -Lx(-v, u); it amplifies
+            re_dn = -symbol_re(-v)
+            assert re_dn.max() > 1e-3
