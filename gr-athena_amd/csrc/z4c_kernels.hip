@@ -18,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,12 @@ using namespace z4c;
 namespace {
 
 thread_local std::string g_err;
+
+// Device address of `scratch` on each GPU (z4c_history_noalloc), resolved
+// once per device by z4c_ctx_create with that device current.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[MAX_DEVICES] = {};
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -620,6 +627,27 @@ __global__ void __launch_bounds__(TILE) k_history(HstArgs a) {
     for (int w = 0; w < TILE / 64; ++w) x += red[threadIdx.x][w];
     a.partial[(long long)t * NHST + threadIdx.x] = x;
   }
+}
+
+// Partials of z4c_history_noalloc: NHST doubles per tile in a file-scope
+// device array instead of a context allocation, for callers that cannot
+// allocate (HIP graph capture, serving loops).  Each GPU holds one copy until
+// the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (g_scratch_dev).  Every tile of a
+// call writes its partials before k_history_sum reads them, so nothing needs
+// clearing; calls that overlap on one GPU share the array and race.
+constexpr int HIST_NOALLOC_MAX_TILES = 1 << 17;  // 8 MiB: 33.5 M interior vertices
+static __device__ __align__(256) double scratch[(size_t)HIST_NOALLOC_MAX_TILES * NHST];
+
+// out[q] = sum of the tiles' partials in tile order, one lane per column:
+// the order (and so every rounding) of z4c_history's host loop.
+__global__ void __launch_bounds__(64) k_history_sum(const double* partial, int ntiles,
+                                                    double* out) {
+  const int q = threadIdx.x;
+  if (q >= NHST) return;
+  double x = 0.0;
+  for (int t = 0; t < ntiles; ++t) x += partial[(long long)t * NHST + q];
+  out[q] = x;
 }
 
 // ---------------------------------------------------------------------------
@@ -2537,6 +2565,17 @@ int z4c_ctx_create(z4c_ctx** out, int device) {
   HIPCALL(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(Z4C_EINVAL, "device index out of range");
   HIPCALL(hipSetDevice(device));
+  if (device >= MAX_DEVICES) return fail(Z4C_EINVAL, "device index beyond 63");
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr)
+        return fail(Z4C_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      g_scratch_dev[device] = static_cast<double*>(ptr);
+    }
+  }
   z4c_ctx* c = new z4c_ctx();
   c->device = device;
   hipDeviceProp_t prop;
@@ -2865,6 +2904,29 @@ int z4c_history(z4c_ctx* c, const z4c_params* p, const double* u, const double* 
   for (int q = 0; q < NHST; ++q) out[q] = 0.0;
   for (int t = 0; t < L.ntiles; ++t)
     for (int q = 0; q < NHST; ++q) out[q] += c->hst_host[(size_t)t * NHST + q];
+  return Z4C_OK;
+}
+
+int z4c_history_noalloc(z4c_ctx* c, const z4c_params* p, const double* u, const double* con,
+                        double* out, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !con || !out) return fail(Z4C_EINVAL, "history_noalloc: null argument");
+  const Launch L = launch_of(c, BOX_INTERIOR);
+  if (L.ntiles > HIST_NOALLOC_MAX_TILES)
+    return fail(Z4C_EINVAL, "history_noalloc: the blocks need " + std::to_string(L.ntiles) +
+                                " interior tiles, more than the " +
+                                std::to_string(HIST_NOALLOC_MAX_TILES) +
+                                " its fixed scratch holds; use z4c_history");
+  double* part = g_scratch_dev[c->device];
+  if (!part) return fail(Z4C_ESTATE, "history_noalloc: scratch not resolved on this device");
+  hipStream_t s = (hipStream_t)stream;
+  const double r = p->r_max_con;
+  HstArgs a{c->d_blocks, L, u, con, r * r, part};
+  hipLaunchKernelGGL(k_history, dim3(grid_for(L.ntiles)), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_history_sum, dim3(1), dim3(64), 0, s, (const double*)part, L.ntiles, out);
+  HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
 

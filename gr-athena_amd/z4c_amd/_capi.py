@@ -28,7 +28,7 @@ EXPORTS = [
     "z4c_physical_bc", "z4c_rhs_vjp", "z4c_rk4_step_vjp", "z4c_hamiltonian_norm",
     "z4c_hamiltonian_norm_vjp", "z4c_weyl", "z4c_coarse_layout", "z4c_restrict",
     "z4c_prolongation_range", "z4c_set_prolongation", "z4c_prolongate", "z4c_rk4_step_ghosts",
-    "z4c_ls_tableau_get", "z4c_ls_stage", "z4c_ls_step",
+    "z4c_ls_tableau_get", "z4c_ls_stage", "z4c_ls_step", "z4c_history_noalloc",
 ]
 
 # parameter gradients of the adjoints (Z4C_PBAR_* of the header)
@@ -142,6 +142,7 @@ def load(path: str | None = None):
         "z4c_rk4_step_ghosts": [vp, pp, d, vp, vp, vp, i, vp],
         "z4c_set_timing": [vp, i],
         "z4c_history": [vp, pp, vp, vp, C.POINTER(d), vp],
+        "z4c_history_noalloc": [vp, pp, vp, vp, vp, vp],
         "z4c_set_exchange": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(i)],
         "z4c_exchange": [vp, i, vp, vp],
         "z4c_autotune": [vp, pp, vp, i, vp],

@@ -392,6 +392,14 @@ class Z4c:
                                  _stream(stream)), "history")
         return list(out)
 
+    def history_noalloc(self, u, con, out, stream=None):
+        """``history`` into ``out`` (8 doubles of device memory) without any
+        allocation: the per-tile partials live in the library's fixed device
+        scratch.  Stream-ordered (does not wait); equals ``history`` bit for
+        bit.  Calls that overlap on one GPU race."""
+        check(self.L.z4c_history_noalloc(self._ctx, C.byref(self.params), _ptr(u), _ptr(con),
+                                         _ptr(out), _stream(stream)), "history_noalloc")
+
     def autotune(self, u, reps=3, stream=None):
         """Time the stencil-pass workgroup configurations per block size on
         state ``u`` and keep the fastest; returns {block size: threads}."""

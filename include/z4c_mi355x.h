@@ -360,6 +360,20 @@ int z4c_rk4_step_graph(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
 int z4c_history(z4c_ctx* ctx, const z4c_params* p, const double* u,
                 const double* con, double* out, void* stream);
 
+/* z4c_history without any allocation (no counterpart in the reference): for
+   callers that cannot allocate device or host memory during the call (code
+   under HIP graph capture, tight serving loops).  The per-tile partial sums
+   live in a fixed device array of the library (8 MiB per GPU, held until the
+   process ends) instead of a context allocation, and the columns are summed
+   on the device in z4c_history's order, so `out` (8 doubles of DEVICE
+   memory) equals z4c_history's result bit for bit.  Stream-ordered: does
+   not wait.  At most 131072 interior tiles of 256 vertices (about 33.5 M
+   interior vertices); beyond that Z4C_EINVAL.  Every call on one GPU uses
+   the same array, so calls that overlap on one GPU (several streams or
+   threads) race and must be serialised by the caller. */
+int z4c_history_noalloc(z4c_ctx* ctx, const z4c_params* p, const double* u,
+                        const double* con, double* out, void* stream);
+
 /* Tile autotuner of the stencil pass (no counterpart in the reference,
    whose pencil loops have no tiling).  For every block size among the
    registered blocks, times the stencil pass of those blocks under each
