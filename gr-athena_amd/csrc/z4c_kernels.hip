@@ -20,10 +20,12 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "z4c_adjoint.hpp"
 #include "z4c_device.hpp"
+#include "z4c_mesh.hpp"
 #include "z4c_mi355x.h"
 #include "z4c_refine.hpp"
 #include "z4c_rhs_point.hpp"
@@ -1289,6 +1291,7 @@ __global__ void __launch_bounds__(256) k_exchange_scatter(ExArgs a) {
     a.u[B.off * a.nvar + (long long)n * B.npts + p] = a.tmp[(long long)n * a.ndst + d];
 }
 
+
 // ---------------------------------------------------------------------------
 // Vertex-centred restriction and prolongation between levels
 // (src/mesh/mesh_refinement_vc.cpp; the index algebra and the per-vertex
@@ -1372,6 +1375,67 @@ __global__ void __launch_bounds__(256) k_prolong_vc(ProlongArgs a) {
   double* dst = a.u + B.off * a.nvar + p;
   for (int n = 0; n < a.nvar; ++n)
     dst[(long long)n * B.npts] = rf_prolong_point(C + (long long)n * R.cnpts, s2, s3, ci, cj, ck, m);
+}
+
+// ---------------------------------------------------------------------------
+// Ghost update across refinement levels (the plan of z4c_mesh.hpp): every
+// destination vertex (fine array or coarse buffer) is its ordered source
+// list summed from zero, times 1/multiplicity, all fields.  Sources are
+// interior vertices read before any destination is written (gather into
+// tmp, then scatter), as the reference packs every message before the
+// receivers unpack.  A reference 2*b + A addresses block b's fine array
+// (A = 0) or coarse buffer (A = 1).
+// ---------------------------------------------------------------------------
+struct MlxArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const int* dst;
+  const long long* off;
+  const int* src;
+  const double* inv;
+  int ndst;
+  int nvar;
+  double* u;
+  double* coarse;
+  double* tmp;  // nvar * ndst, field-major
+};
+
+__device__ __forceinline__ long long mlx_addr(const MlxArgs& a, int blkarr, int p, int n) {
+  const int b = blkarr >> 1;
+  if (blkarr & 1) {
+    const RfBlock& R = a.cb[b];
+    return -1 - (R.coff * a.nvar + (long long)n * R.cnpts + p);  // coarse: negative
+  }
+  const DevBlock& B = a.blocks[b];
+  return B.off * a.nvar + (long long)n * B.npts + p;
+}
+
+__global__ void __launch_bounds__(256) k_mlx_gather(MlxArgs a) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= a.ndst) return;
+  const long long s0 = a.off[d], s1 = a.off[d + 1];
+  const double inv = a.inv[d];
+  for (int n = 0; n < a.nvar; ++n) {
+    double acc = 0.0;
+    for (long long e = s0; e < s1; ++e) {
+      const long long q = mlx_addr(a, a.src[2 * e], a.src[2 * e + 1], n);
+      acc += q >= 0 ? a.u[q] : a.coarse[-1 - q];
+    }
+    a.tmp[(long long)n * a.ndst + d] = acc * inv;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlx_scatter(MlxArgs a) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= a.ndst) return;
+  for (int n = 0; n < a.nvar; ++n) {
+    const long long q = mlx_addr(a, a.dst[2 * d], a.dst[2 * d + 1], n);
+    const double v = a.tmp[(long long)n * a.ndst + d];
+    if (q >= 0)
+      a.u[q] = v;
+    else
+      a.coarse[-1 - q] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1956,6 +2020,44 @@ struct z4c_ctx {
   int* ex_off = nullptr;
   int* ex_src = nullptr;
   int ex_ndst = 0;
+  // ghost update across refinement levels (z4c_set_exchange on blocks of
+  // several spacings): gather lists, the context's coarse buffers, the
+  // gathered values, and the prolongation entries of the coarser neighbours
+  bool ml = false;
+  int* ml_dst = nullptr;        // 2 ints per destination: 2*block + array, point
+  long long* ml_off = nullptr;  // CSR offsets into ml_src
+  int* ml_src = nullptr;        // 2 ints per source
+  double* ml_inv = nullptr;     // 1/multiplicity per destination
+  int ml_ndst = 0;
+  double* ml_coarse = nullptr;  // NVAR fields of every block's coarse buffer
+  double* ml_tmp = nullptr;     // NVAR * ml_ndst
+  RfEntry* ml_pent = nullptr;
+  RfChunk* ml_pchunks = nullptr;
+  int ml_npchunks = 0;
+  std::vector<std::vector<z4c_mesh::Nbr>> ml_nbrs;  // per block, for inspection
+  std::vector<int> ml_level;
+
+  void drop_ml() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    hipFree(ml_dst);
+    hipFree(ml_off);
+    hipFree(ml_src);
+    hipFree(ml_inv);
+    hipFree(ml_coarse);
+    hipFree(ml_tmp);
+    hipFree(ml_pent);
+    hipFree(ml_pchunks);
+    ml_dst = ml_src = nullptr;
+    ml_off = nullptr;
+    ml_inv = ml_coarse = ml_tmp = nullptr;
+    ml_pent = nullptr;
+    ml_pchunks = nullptr;
+    ml_ndst = ml_npchunks = 0;
+    ml = false;
+    ml_nbrs.clear();
+    ml_level.clear();
+  }
   // refinement (lazy): coarse-buffer table and restriction chunks, and the
   // prolongation entries set by z4c_set_prolongation with their chunks
   std::vector<RfBlock> hrf;
@@ -1974,6 +2076,7 @@ struct z4c_ctx {
   const double* g_u = nullptr;
   const double* g_work = nullptr;
   double g_dt = 0.0;
+  int g_ghosts = 0;
   z4c_params g_params{};
   // per-pass kernel timing (events recorded on the launch stream)
   struct TimedLaunch {
@@ -2059,6 +2162,7 @@ struct z4c_ctx {
     hipFree(ex_src);
     ex_dst = ex_off = ex_src = nullptr;
     ex_ndst = 0;
+    drop_ml();
     drop_refine();
     hst_host.clear();
     double** adj[6] = {&adj_lin, &adj_sb, &adj_part, &adj_k, &adj_stage, &adj_reg};
@@ -2311,6 +2415,7 @@ int launch_bt(z4c_ctx* c, int nvar, double* out, const double* u0, int nk,
 int launch_periodic(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   const long long n = c->total_pts * nvar;
   if (c->snap_elems < n) {
+    c->invalidate_graph();  // a captured step may hold the old snapshot
     hipFree(c->snap);
     c->snap = nullptr;
     c->snap_elems = 0;
@@ -2328,6 +2433,7 @@ int launch_periodic(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
 
 int ensure_snapshot(z4c_ctx* c, long long n) {
   if (c->snap_elems >= n) return Z4C_OK;
+  c->invalidate_graph();  // a captured step may hold the old scratch
   hipFree(c->snap);
   c->snap = nullptr;
   c->snap_elems = 0;
@@ -2382,7 +2488,31 @@ int launch_physical(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   return Z4C_OK;
 }
 
+int launch_ml_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
+  if (nvar > NVAR) return fail(Z4C_EINVAL, "exchange: at most 22 fields across levels");
+  // every block restricts its interior into its coarse buffer
+  RestrictArgs ra{c->d_blocks, c->d_rf, c->d_rchunks, nvar, u, c->ml_coarse};
+  hipLaunchKernelGGL(k_restrict_vc, dim3(c->n_rchunks), dim3(256), 0, s, ra);
+  HIPCALL(hipGetLastError());
+  if (c->ml_ndst) {
+    MlxArgs a{c->d_blocks, c->d_rf, c->ml_dst, c->ml_off, c->ml_src, c->ml_inv,
+              c->ml_ndst, nvar, u, c->ml_coarse, c->ml_tmp};
+    const dim3 g((c->ml_ndst + 255) / 256);
+    hipLaunchKernelGGL(k_mlx_gather, g, dim3(256), 0, s, a);
+    HIPCALL(hipGetLastError());
+    hipLaunchKernelGGL(k_mlx_scatter, g, dim3(256), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  if (c->ml_npchunks) {
+    ProlongArgs pa{c->d_blocks, c->d_rf, c->ml_pent, c->ml_pchunks, nvar, c->ml_coarse, u};
+    hipLaunchKernelGGL(k_prolong_vc, dim3(c->ml_npchunks), dim3(256), 0, s, pa);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
 int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
+  if (c->ml) return launch_ml_exchange(c, nvar, u, s);
   if (!c->ex_ndst) return Z4C_OK;
   // the destinations' new values go through a scratch array of nvar * ndst
   // doubles (gather, then scatter), not through a copy of the whole state
@@ -2930,10 +3060,29 @@ int z4c_history_noalloc(z4c_ctx* c, const z4c_params* p, const double* u, const 
   return Z4C_OK;
 }
 
+namespace {
+int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* periodic);
+}  // namespace
+
 int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* periodic) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!lo || !hi || !periodic) return fail(Z4C_EINVAL, "set_exchange: null argument");
+  for (int d = 0; d < 3; ++d)
+    if (!(hi[d] > lo[d])) return fail(Z4C_EINVAL, "set_exchange: empty domain");
+  c->invalidate_graph();  // a captured step holds the old map
+  bool levels = false;
+  for (const z4c_block& B : c->hblocks)
+    for (int d = 0; d < 3; ++d) levels = levels || B.dx[d] != c->hblocks[0].dx[d];
+  if (levels) {
+    hipFree(c->ex_dst);
+    hipFree(c->ex_off);
+    hipFree(c->ex_src);
+    c->ex_dst = c->ex_off = c->ex_src = nullptr;
+    c->ex_ndst = 0;
+    return build_ml_exchange(c, lo, hi, periodic);
+  }
+  c->drop_ml();
   const double* h = c->hblocks[0].dx;
   long long Nd[3];
   for (int d = 0; d < 3; ++d) {
@@ -3029,7 +3178,9 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
   HIPCALL(hipMemcpy(c->ex_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   HIPCALL(hipMemcpy(c->ex_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
   c->ex_ndst = (int)(dst.size() / 2);
-  return Z4C_OK;
+  // the gather scratch for up to NVAR fields now, so that z4c_exchange
+  // allocates nothing (and a step with the exchange can be captured)
+  return ensure_snapshot(c, (long long)c->ex_ndst * NVAR);
 }
 
 int z4c_exchange(z4c_ctx* c, int nvar, double* u, void* stream) {
@@ -3117,6 +3268,84 @@ int make_entry(const z4c_ctx* c, const z4c_prolong_box& in, int q, RfEntry& E) {
     E.f0[a] = flo;
     E.fn[a] = fhi - flo + 1;
   }
+  return Z4C_OK;
+}
+
+// The ghost update across refinement levels (z4c_mesh.hpp): block tree,
+// neighbour lists, gather lists and prolongation boxes, uploaded with the
+// context's coarse buffers and gather scratch (so that an exchange
+// allocates nothing and can be captured).
+int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* periodic) {
+  c->drop_ml();
+  std::vector<z4c_mesh::Geo> geo(c->nblocks);
+  for (int b = 0; b < c->nblocks; ++b) {
+    const z4c_block& B = c->hblocks[b];
+    const int il[3] = {B.il, B.jl, B.kl}, iu[3] = {B.iu, B.ju, B.ku};
+    for (int d = 0; d < 3; ++d) {
+      geo[b].nx[d] = iu[d] - il[d];
+      geo[b].dx[d] = B.dx[d];
+      geo[b].xmin[d] = B.x0[d] + il[d] * B.dx[d];
+    }
+    if (B.il != z4c_mesh::MNG || B.jl != z4c_mesh::MNG || B.kl != z4c_mesh::MNG)
+      return fail(Z4C_EINVAL, "set_exchange: blocks need 4 ghost layers");
+  }
+  z4c_mesh::Tree t;
+  std::string e = t.build(geo, lo, hi, periodic);
+  if (!e.empty()) return fail(Z4C_EINVAL, "set_exchange: " + e);
+  z4c_mesh::Plan P;
+  e = z4c_mesh::build_plan(t, P);
+  if (!e.empty()) return fail(Z4C_EINVAL, "set_exchange: " + e);
+  // a block that prolongates next to an open face would need the coarse
+  // level's physical fill first (ApplyPhysicalBCsOnCoarseLevel)
+  for (int b = 0; b < t.n; ++b) {
+    if (!P.prolongates[b]) continue;
+    for (int d = 0; d < 3; ++d) {
+      if (t.periodic[d]) continue;
+      const long long m = t.nrb0[d] << t.level[b];
+      if (t.loc[b][d] == 0 || t.loc[b][d] == m - 1)
+        return fail(Z4C_EINVAL, "set_exchange: block " + std::to_string(b) +
+                                    " has a coarser neighbour and an open face; the coarse "
+                                    "level's physical fill is not built");
+    }
+  }
+  int rc = ensure_coarse(c);
+  if (rc) return rc;
+  std::vector<RfEntry> ent(P.boxes.size());
+  std::vector<RfChunk> ch;
+  for (size_t q = 0; q < P.boxes.size(); ++q) {
+    const auto& bx = P.boxes[q];
+    z4c_prolong_box box{bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], bx[6]};
+    rc = make_entry(c, box, (int)q, ent[q]);
+    if (rc) return rc;
+    const long long np = (long long)ent[q].fn[0] * ent[q].fn[1] * ent[q].fn[2];
+    for (long long s0 = 0; s0 < np; s0 += 256) ch.push_back(RfChunk{(int)q, (int)s0});
+  }
+  const size_t nd = P.dst.size(), ns = P.src.size();
+  if (nd > (size_t)std::numeric_limits<int>::max() / NVAR)
+    return fail(Z4C_EINVAL, "set_exchange: plan too large");
+  auto up = [&](auto*& d, const auto* h, size_t n) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(d)>>;
+    if (n == 0) return true;
+    if (hipMalloc(&d, sizeof(T) * n) != hipSuccess) return false;
+    return hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  static_assert(sizeof(z4c_mesh::Ref) == 2 * sizeof(int), "Ref is two ints");
+  bool ok = up(c->ml_dst, reinterpret_cast<const int*>(P.dst.data()), 2 * nd) &&
+            up(c->ml_off, P.off.data(), P.off.size()) &&
+            up(c->ml_src, reinterpret_cast<const int*>(P.src.data()), 2 * ns) &&
+            up(c->ml_inv, P.inv.data(), nd) && up(c->ml_pent, ent.data(), ent.size()) &&
+            up(c->ml_pchunks, ch.data(), ch.size());
+  ok = ok && hipMalloc(&c->ml_coarse, sizeof(double) * NVAR * (size_t)c->rf_total) == hipSuccess;
+  ok = ok && (nd == 0 || hipMalloc(&c->ml_tmp, sizeof(double) * NVAR * nd) == hipSuccess);
+  if (!ok) {
+    c->drop_ml();
+    return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
+  }
+  c->ml_ndst = (int)nd;
+  c->ml_npchunks = (int)ch.size();
+  c->ml_nbrs = t.nbrs;
+  c->ml_level = t.level;
+  c->ml = true;
   return Z4C_OK;
 }
 
@@ -3581,40 +3810,65 @@ int z4c_rk4_step_ghosts(z4c_ctx* c, const z4c_params* p, double dt, double* u, d
   return launch_rk4_step(c, p, dt, u, work, mat, ghosts, (hipStream_t)stream);
 }
 
-int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
-                       void* stream) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step_graph: bad argument");
-  if (c->timing) return fail(Z4C_ESTATE, "rk4_step_graph: the kernel timer cannot be captured");
-  hipStream_t s = (hipStream_t)stream;
+namespace {
+// The fused step (ghost update `ghosts` after every stage, 0 = ghosts held
+// at u_n) replayed from a HIP graph captured once per (u, work, dt,
+// parameters, ghosts).  Scratch the update needs is allocated before the
+// capture, so the graph holds kernel nodes only.
+int step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work, int ghosts,
+               hipStream_t s, const char* who) {
+  if (c->timing) return fail(Z4C_ESTATE, std::string(who) + ": the kernel timer cannot be captured");
   const bool same = c->graph_exec && c->g_u == u && c->g_work == work && c->g_dt == dt &&
+                    c->g_ghosts == ghosts &&
                     std::memcmp(&c->g_params, p, sizeof(z4c_params)) == 0;
   if (!same) {
     c->invalidate_graph();
+    if (ghosts & 1) {
+      const int rc0 = ensure_snapshot(c, c->total_pts * NVAR);
+      if (rc0) return rc0;
+    }
     if (!c->cap_stream) HIPCALL(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
     HIPCALL(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-    rc = launch_rk4_step(c, p, dt, u, work, nullptr, 0, c->cap_stream);
+    int rc = launch_rk4_step(c, p, dt, u, work, nullptr, ghosts, c->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
     if (rc) {
       if (g) hipGraphDestroy(g);
       return rc;
     }
-    if (e != hipSuccess || !g) return fail(Z4C_EHIP, "rk4_step_graph: capture failed");
+    if (e != hipSuccess || !g) return fail(Z4C_EHIP, std::string(who) + ": capture failed");
     const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     if (ei != hipSuccess) {
       c->graph_exec = nullptr;
-      return fail(Z4C_EHIP, "rk4_step_graph: instantiation failed");
+      return fail(Z4C_EHIP, std::string(who) + ": instantiation failed");
     }
     c->g_u = u;
     c->g_work = work;
     c->g_dt = dt;
+    c->g_ghosts = ghosts;
     c->g_params = *p;
   }
   HIPCALL(hipGraphLaunch(c->graph_exec, s));
   return Z4C_OK;
+}
+}  // namespace
+
+int z4c_rk4_step_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u, double* work,
+                       void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work) return fail(Z4C_EINVAL, "rk4_step_graph: bad argument");
+  return step_graph(c, p, dt, u, work, 0, (hipStream_t)stream, "rk4_step_graph");
+}
+
+int z4c_rk4_step_ghosts_graph(z4c_ctx* c, const z4c_params* p, double dt, double* u,
+                              double* work, int ghosts, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !work || ghosts < 0 || ghosts > 6 || (ghosts & 3) == 3)
+    return fail(Z4C_EINVAL, "rk4_step_ghosts_graph: bad argument");
+  return step_graph(c, p, dt, u, work, ghosts, (hipStream_t)stream, "rk4_step_ghosts_graph");
 }
 
 int z4c_rk4_stage(z4c_ctx* c, const z4c_params* p, int stage, double dt, double* u,

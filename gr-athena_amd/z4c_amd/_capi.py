@@ -29,6 +29,7 @@ EXPORTS = [
     "z4c_hamiltonian_norm_vjp", "z4c_weyl", "z4c_coarse_layout", "z4c_restrict",
     "z4c_prolongation_range", "z4c_set_prolongation", "z4c_prolongate", "z4c_rk4_step_ghosts",
     "z4c_ls_tableau_get", "z4c_ls_stage", "z4c_ls_step", "z4c_history_noalloc",
+    "z4c_rk4_step_ghosts_graph",
 ]
 
 # parameter gradients of the adjoints (Z4C_PBAR_* of the header)
@@ -140,6 +141,7 @@ def load(path: str | None = None):
         "z4c_rk4_stage": [vp, pp, i, d, vp, vp, C.POINTER(vp), vp, i, vp],
         "z4c_rk4_step": [vp, pp, d, vp, vp, vp, vp],
         "z4c_rk4_step_ghosts": [vp, pp, d, vp, vp, vp, i, vp],
+        "z4c_rk4_step_ghosts_graph": [vp, pp, d, vp, vp, i, vp],
         "z4c_set_timing": [vp, i],
         "z4c_history": [vp, pp, vp, vp, C.POINTER(d), vp],
         "z4c_history_noalloc": [vp, pp, vp, vp, vp, vp],

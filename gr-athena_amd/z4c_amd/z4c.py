@@ -320,13 +320,18 @@ class Z4c:
         ghosts as ``rk4_step`` with the same flags does, then applies
         AlgConstr to the whole block.  The derivative arrays are not prepared
         (ADMConstraints prepares its own).  ``work`` comes from
-        ``make_step_work``.  ``graph``: replay the step (ghosts held at u_n)
-        from a HIP graph (captured on first use and whenever u, work, dt or
-        the parameters change)."""
+        ``make_step_work``.  ``graph``: replay the step from a HIP graph
+        (captured on first use and whenever u, work, dt, the parameters or
+        the ghost update change; vacuum only)."""
         mode = (2 if exchange else (1 if periodic else 0)) + (4 if physical else 0)
         if mode:
             if graph:
-                raise _capi.Z4cError("the graph-replayed step holds the ghosts at u_n")
+                if mat is not None:
+                    raise _capi.Z4cError("the graph-replayed step takes no matter sources")
+                check(self.L.z4c_rk4_step_ghosts_graph(self._ctx, C.byref(self.params), float(dt),
+                                                       _ptr(u), _ptr(work), mode,
+                                                       _stream(stream)), "rk4_step_ghosts_graph")
+                return
             check(self.L.z4c_rk4_step_ghosts(self._ctx, C.byref(self.params), float(dt), _ptr(u),
                                              _ptr(work), _ptr(mat), mode, _stream(stream)),
                   "rk4_step_ghosts")

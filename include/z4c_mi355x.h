@@ -176,18 +176,40 @@ int z4c_new_dt(z4c_ctx* ctx, double cfl, double* dt_out);
    Applies to every registered block independently. */
 int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
-/* Same-level vertex-centred ghost exchange between the registered blocks,
-   as GR_Z4c's SEND_Z4C / RECV_Z4C / SETB_Z4C do for neighbours on one level
-   (additive unpack src/comm/comm_channel.cpp:1277-1330, ranges
-   src/comm/index_utilities.hpp:116-139, division by the node multiplicity
-   src/comm/node_multiplicity.cpp:349-420).  z4c_set_exchange builds the map
-   for the domain [lo, hi] (periodic[d] != 0: axis d wraps); all blocks must
-   share one spacing and sit on its lattice (Z4C_EINVAL otherwise; across
-   levels the caller combines z4c_restrict / z4c_prolongate below with its
-   own copies into the coarse buffers).  z4c_exchange then sets
-   every vertex that some block owns as an interior vertex (ghosts, and
-   shared boundary vertices) to the mean of its owners' values, summed in
-   block order and k, j, i order, times 1/multiplicity. */
+/* Vertex-centred ghost update between the registered blocks, as GR_Z4c's
+   SEND_Z4C / RECV_Z4C / SETB_Z4C (and PROLONG across levels) do it
+   (src/task_list/gr/task_list_gr_z4c.cpp:47-58).  z4c_set_exchange builds
+   the update for the domain [lo, hi] (periodic[d] != 0: axis d wraps) once
+   per block list, with its scratch for up to 22 fields; z4c_exchange
+   applies it (stream-ordered; with at most 22 fields it allocates nothing
+   and can be captured into a HIP graph).
+   * All blocks on one spacing (sitting on its lattice): every vertex that
+     some block owns as an interior vertex (ghosts, and shared boundary
+     vertices) becomes the mean of its owners' values, summed in block
+     order and k, j, i order, times 1/multiplicity (additive unpack
+     src/comm/comm_channel.cpp:1277-1330, ranges
+     src/comm/index_utilities.hpp:116-139, division
+     src/comm/node_multiplicity.cpp:349-420).
+   * Blocks on several spacings (equal cell counts nx >= 8, even, 4 ghost
+     layers; spacings h0 * 2^-L on a 2:1-balanced block tree of the domain):
+     the reference's multi-level update.  Neighbours as
+     mesh_topology::SearchAndSetNeighbors finds them
+     (src/mesh/mesh_topology.cpp:278-960); every block restricts its
+     interior into a coarse buffer of the context (RestrictInterior,
+     src/comm/refinement_ops.cpp:178-212); fine ghosts and interior
+     boundary vertices take same-level slabs and finer neighbours'
+     restricted slabs, coarse buffers take same-level coarse slabs and
+     coarser neighbours' half-block slabs (CommChannel::PackAndSend /
+     Unpack, src/comm/comm_channel.cpp:616-817, 1277-1472, ranges
+     src/comm/index_utilities.cpp), ghosts start from zero and everything is
+     divided by the node multiplicity (src/comm/node_multiplicity.cpp);
+     then every block prolongates over each coarser neighbour's
+     ProlongationRange (ProlongateBoundaries,
+     src/comm/refinement_ops.cpp:143-176).  Ghosts beyond an open face are
+     zero afterwards (for z4c_physical_bc).  Z4C_EINVAL for a block list off
+     such a tree, and for a block with a coarser neighbour at an open face
+     (the coarse level's physical fill, ApplyPhysicalBCsOnCoarseLevel, is
+     not built).  At most 22 fields per call on several spacings. */
 int z4c_set_exchange(z4c_ctx* ctx, const double* lo, const double* hi, const int* periodic);
 int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
@@ -286,10 +308,21 @@ int z4c_rk4_step(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
    plus 4 (physical fill), then AlgConstr of the whole block -- the order and
    sums of z4c_rk4_stage over stages 1..4 (src/task_list/gr/
    task_list_gr_z4c.cpp:45-89) without its derivative preparation.  `work`
-   as for z4c_rk4_step.  Not capturable (the exchange's scratch is allocated
-   on first use). */
+   as for z4c_rk4_step.  With the exchange on blocks of several spacings
+   (z4c_set_exchange) every stage's update includes the restriction and the
+   prolongation from coarser neighbours (GR_Z4c's PROLONG). */
 int z4c_rk4_step_ghosts(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
                         double* work, const double* mat, int ghosts, void* stream);
+
+/* z4c_rk4_step_ghosts (vacuum) replayed from a HIP graph (no counterpart in
+   the reference, whose task list launches nothing): captured on a stream of
+   the context on the first call and whenever u, work, dt, the parameters,
+   `ghosts` or the exchange map change, then launched on `stream`.  The
+   scratch of the ghost update is allocated before the capture, so the graph
+   holds kernel nodes only.  Z4C_ESTATE while the kernel timer is on.
+   Results equal z4c_rk4_step_ghosts'. */
+int z4c_rk4_step_ghosts_graph(z4c_ctx* ctx, const z4c_params* p, double dt, double* u,
+                              double* work, int ghosts, void* stream);
 
 /* Low-storage (2S) integrator: per stage the weights delta, gamma_1,
    gamma_2, gamma_3 and beta of TaskLists::Integrators::LowStorage

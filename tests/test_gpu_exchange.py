@@ -43,7 +43,9 @@ def test_exchange_matches_oracle(parts, periodic):
         assert np.array_equal(got[b], exp[b]), b
 
 
-def test_exchange_rejects_mixed_spacing():
+def test_exchange_rejects_mixed_spacing_off_the_block_tree():
+    """Blocks of several spacings go through the level-aware plan, which
+    needs equal blocks of at least 8 cells on a 2:1-balanced tree."""
     gs = [MeshBlockGeom.cube(4, 0.25, 0.0), MeshBlockGeom.cube(4, 0.125, 1.0)]
     z = Z4c(gs, Params())
     with pytest.raises(Z4cError):
