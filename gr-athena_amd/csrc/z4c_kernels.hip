@@ -1438,6 +1438,65 @@ __global__ void __launch_bounds__(256) k_mlx_scatter(MlxArgs a) {
   }
 }
 
+// ApplyPhysicalBCsOnCoarseLevel (src/comm/physical_bcs.cpp:687-876) for a
+// block that prolongates: the fine level's face fill (k_physical) on the
+// block's coarse buffer with the vertex-centred coarse ranges civs..cive
+// that Z4c passes (src/z4c/z4c.cpp:48-58), so a face fills the 4 coarse
+// ghosts beyond civs / cive by the 5-point extrapolation and keeps the
+// boundary vertex; one thread per (field, line), faces in order x1, x2, x3.
+struct PhysCArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const PhysLine* lines;
+  int nent;
+  long long nlines1;
+  long long nlines;
+  int axis;
+  int nvar;
+  double* coarse;
+};
+
+__global__ void __launch_bounds__(256) k_physical_coarse(PhysCArgs a) {
+  const long long Lg = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (Lg >= a.nlines) return;
+  const int n = (int)(Lg / a.nlines1);
+  const long long L = Lg - (long long)n * a.nlines1;
+  int lo = 0, hi = a.nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.lines[mid].start <= L) lo = mid; else hi = mid - 1;
+  }
+  const PhysLine E = a.lines[lo];
+  const RfBlock R = a.cb[E.block];
+  const int pf = a.blocks[E.block].pfaces;
+  const int r = (int)(L - E.start);
+  const int t1 = E.lo1 + r / E.n0, t0 = E.lo0 + r % E.n0;
+  const long long str[3] = {1, R.cn1, (long long)R.cn1 * R.cn2};
+  const int ax = a.axis;
+  const int o0 = ax == 0 ? 1 : 0, o1 = ax == 2 ? 1 : 2;
+  double* v = a.coarse + R.coff * a.nvar + (long long)n * R.cnpts + t0 * str[o0] + t1 * str[o1];
+  const long long s = str[ax];
+  const int cni = ax == 0 ? R.cni : (ax == 1 ? R.cnj : R.cnk);
+  const int vs = RF_NCG, ve = RF_NCG + cni - 1;
+  const double c[5] = {5., -10., 10., -5., 1.};
+  if (pf & (1 << (2 * ax))) {
+    for (int i = vs - 1; i >= vs - RF_NCG; --i) {
+      double x = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) x += c[p] * v[(i + p + 1) * s];
+      v[i * s] = x;
+    }
+  }
+  if (pf & (2 << (2 * ax))) {
+    for (int i = ve + 1; i <= ve + RF_NCG; ++i) {
+      double x = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) x += c[p] * v[(i - p - 1) * s];
+      v[i * s] = x;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): Re / Im psi4 at
 // interior vertices, one thread per vertex, in the reference's operation
@@ -2034,6 +2093,9 @@ struct z4c_ctx {
   RfEntry* ml_pent = nullptr;
   RfChunk* ml_pchunks = nullptr;
   int ml_npchunks = 0;
+  PhysLine* mlp_lines[3] = {};  // the coarse level's physical fill, per axis
+  int mlp_nent[3] = {};
+  long long mlp_nlines1[3] = {};
   std::vector<std::vector<z4c_mesh::Nbr>> ml_nbrs;  // per block, for inspection
   std::vector<int> ml_level;
 
@@ -2048,6 +2110,12 @@ struct z4c_ctx {
     hipFree(ml_tmp);
     hipFree(ml_pent);
     hipFree(ml_pchunks);
+    for (int ax = 0; ax < 3; ++ax) {
+      hipFree(mlp_lines[ax]);
+      mlp_lines[ax] = nullptr;
+      mlp_nent[ax] = 0;
+      mlp_nlines1[ax] = 0;
+    }
     ml_dst = ml_src = nullptr;
     ml_off = nullptr;
     ml_inv = ml_coarse = ml_tmp = nullptr;
@@ -2501,6 +2569,14 @@ int launch_ml_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
     hipLaunchKernelGGL(k_mlx_gather, g, dim3(256), 0, s, a);
     HIPCALL(hipGetLastError());
     hipLaunchKernelGGL(k_mlx_scatter, g, dim3(256), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!c->mlp_nent[ax]) continue;
+    PhysCArgs pc{c->d_blocks, c->d_rf, c->mlp_lines[ax], c->mlp_nent[ax], c->mlp_nlines1[ax],
+                 c->mlp_nlines1[ax] * nvar, ax, nvar, c->ml_coarse};
+    hipLaunchKernelGGL(k_physical_coarse, dim3((unsigned)((pc.nlines + 255) / 256)), dim3(256), 0,
+                       s, pc);
     HIPCALL(hipGetLastError());
   }
   if (c->ml_npchunks) {
@@ -3295,17 +3371,20 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
   z4c_mesh::Plan P;
   e = z4c_mesh::build_plan(t, P);
   if (!e.empty()) return fail(Z4C_EINVAL, "set_exchange: " + e);
-  // a block that prolongates next to an open face would need the coarse
-  // level's physical fill first (ApplyPhysicalBCsOnCoarseLevel)
+  // a block that prolongates next to an open face fills its coarse ghosts
+  // there with the physical fill first (ApplyPhysicalBCsOnCoarseLevel): the
+  // face must carry one (z4c_block.sommerfeld_faces | outflow_faces)
   for (int b = 0; b < t.n; ++b) {
     if (!P.prolongates[b]) continue;
+    const int pf = c->hblocks[b].sommerfeld_faces | c->hblocks[b].outflow_faces;
     for (int d = 0; d < 3; ++d) {
       if (t.periodic[d]) continue;
       const long long m = t.nrb0[d] << t.level[b];
-      if (t.loc[b][d] == 0 || t.loc[b][d] == m - 1)
+      if ((t.loc[b][d] == 0 && !(pf & (1 << (2 * d)))) ||
+          (t.loc[b][d] == m - 1 && !(pf & (2 << (2 * d)))))
         return fail(Z4C_EINVAL, "set_exchange: block " + std::to_string(b) +
-                                    " has a coarser neighbour and an open face; the coarse "
-                                    "level's physical fill is not built");
+                                    " has a coarser neighbour and an open face without a "
+                                    "physical boundary condition");
     }
   }
   int rc = ensure_coarse(c);
@@ -3340,6 +3419,35 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
   if (!ok) {
     c->drop_ml();
     return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
+  }
+  // line tables of the coarse level's physical fill (blocks that prolongate)
+  for (int ax = 0; ax < 3; ++ax) {
+    std::vector<PhysLine> lt;
+    long long start = 0;
+    for (int b = 0; b < t.n; ++b) {
+      const int pf = c->hblocks[b].sommerfeld_faces | c->hblocks[b].outflow_faces;
+      if (!P.prolongates[b] || !(pf & (3 << (2 * ax)))) continue;
+      const RfBlock& R = c->hrf[b];
+      const int is = RF_NCG, ie = RF_NCG + R.cni - 1, js = RF_NCG, je = RF_NCG + R.cnj - 1,
+                ks = RF_NCG, ke = RF_NCG + R.cnk - 1;
+      const int bis = is - RF_NCG, bie = ie + RF_NCG;
+      const int bjs = (pf & 4) ? js : js - RF_NCG, bje = (pf & 8) ? je : je + RF_NCG;
+      const int bks = (pf & 16) ? ks : ks - RF_NCG, bke = (pf & 32) ? ke : ke + RF_NCG;
+      PhysLine e{};
+      e.block = b;
+      if (ax == 0) { e.lo0 = bjs; e.n0 = bje - bjs + 1; e.lo1 = bks; e.n1 = bke - bks + 1; }
+      else if (ax == 1) { e.lo0 = bis; e.n0 = bie - bis + 1; e.lo1 = bks; e.n1 = bke - bks + 1; }
+      else { e.lo0 = bis; e.n0 = bie - bis + 1; e.lo1 = js - RF_NCG; e.n1 = je - js + 1 + 2 * RF_NCG; }
+      e.start = start;
+      start += (long long)e.n0 * e.n1;
+      lt.push_back(e);
+    }
+    c->mlp_nent[ax] = (int)lt.size();
+    c->mlp_nlines1[ax] = start;
+    if (!lt.empty() && !up(c->mlp_lines[ax], lt.data(), lt.size())) {
+      c->drop_ml();
+      return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
+    }
   }
   c->ml_ndst = (int)nd;
   c->ml_npchunks = (int)ch.size();

@@ -203,13 +203,16 @@ int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
      Unpack, src/comm/comm_channel.cpp:616-817, 1277-1472, ranges
      src/comm/index_utilities.cpp), ghosts start from zero and everything is
      divided by the node multiplicity (src/comm/node_multiplicity.cpp);
-     then every block prolongates over each coarser neighbour's
-     ProlongationRange (ProlongateBoundaries,
-     src/comm/refinement_ops.cpp:143-176).  Ghosts beyond an open face are
-     zero afterwards (for z4c_physical_bc).  Z4C_EINVAL for a block list off
-     such a tree, and for a block with a coarser neighbour at an open face
-     (the coarse level's physical fill, ApplyPhysicalBCsOnCoarseLevel, is
-     not built).  At most 22 fields per call on several spacings. */
+     then every block with a coarser neighbour fills its coarse ghosts at
+     its physical faces (sommerfeld_faces | outflow_faces: the 5-point
+     extrapolation over the coarse vertex ranges,
+     ApplyPhysicalBCsOnCoarseLevel, src/comm/physical_bcs.cpp:687-876) and
+     prolongates over each coarser neighbour's ProlongationRange
+     (ProlongateBoundaries, src/comm/refinement_ops.cpp:143-176).  Fine
+     ghosts beyond an open face are zero afterwards (for z4c_physical_bc).
+     Z4C_EINVAL for a block list off such a tree, and for a block with a
+     coarser neighbour at an open face that carries no physical boundary
+     condition.  At most 22 fields per call on several spacings. */
 int z4c_set_exchange(z4c_ctx* ctx, const double* lo, const double* hi, const int* periodic);
 int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
 

@@ -23,8 +23,13 @@ beyond an open face: its ghosts stay zero here, for the physical fill):
    coarse buffer (SetFromCoarser); then divides both arrays by the node
    multiplicity (CommRegistry::SetBoundaries, src/comm/comm_registry.cpp:
    336-382; NodeMultiplicity, src/comm/node_multiplicity.cpp:104-440);
-4. every block with coarser neighbours prolongates its coarse buffer over
-   each one's ProlongationRange box (ProlongateBoundaries,
+4. every block with coarser neighbours fills the ghosts of its coarse buffer
+   at physical faces (ApplyPhysicalBCsOnCoarseLevel,
+   src/comm/physical_bcs.cpp:687-876: the fine level's face fill on the
+   coarse buffer with the vertex-centred coarse ranges civs..cive, so the
+   5-point extrapolation fills cive+1..cive+4 and keeps the boundary vertex)
+   and prolongates its coarse buffer over each coarser neighbour's
+   ProlongationRange box (ProlongateBoundaries,
    src/comm/refinement_ops.cpp:143-176).
 
 Neighbours are found and ordered as mesh_topology::SearchAndSetNeighbors
@@ -327,11 +332,30 @@ class Mesh:
     def coarse_shape(self):
         return tuple(self.nx[d] // 2 + 2 * CNG + 1 for d in (2, 1, 0))
 
-    def ghost_fill(self, us, coarse=None):
+    def coarse_geom(self, q):
+        """oracle.Geom of block q's coarse buffer with iu = cive + 1, so that
+        oracle.physical_bc's cell-centred end iu - 1 is the coarse vertex
+        range's end cive, as ApplyPhysicalBCsOnCoarseLevel passes it."""
+        b = self.blocks[q]
+        g = orc.Geom()
+        g.nn1, g.nn2, g.nn3 = (self.nx[d] // 2 + 2 * CNG + 1 for d in range(3))
+        g.ng = CNG
+        g.il, g.iu = CNG, CNG + self.nx[0] // 2 + 1
+        g.jl, g.ju = CNG, CNG + self.nx[1] // 2 + 1
+        g.kl, g.ku = CNG, CNG + self.nx[2] // 2 + 1
+        for d in range(3):
+            g.dx[d] = 2 * b.dx[d]
+            g.x0[d] = b.xmin[d] - CNG * 2 * b.dx[d]
+        return g
+
+    def ghost_fill(self, us, coarse=None, faces=None):
         """One ghost update, in place on the fine arrays ``us`` (list of
         (nvar, nn3, nn2, nn1)); returns the coarse buffers (list of (nvar,
         cn3, cn2, cn1)).  ``coarse`` supplies the coarse buffers' values
-        before the update (their ghosts are zeroed; default NaN)."""
+        before the update (their ghosts are zeroed; default NaN).  ``faces``
+        (per block six flags ix1, ox1, ix2, ox2, ix3, ox3, or None) marks
+        the faces with the physical fill, applied to the coarse buffer of
+        every block that prolongates."""
         nb = len(self.blocks)
         nvar = us[0].shape[0]
         fa = [_axis_fine(self.nx[d]) for d in range(3)]
@@ -401,9 +425,12 @@ class Mesh:
                 mult[cr[2][0]:cr[2][1] + 1, cr[1][0]:cr[1][1] + 1, cr[0][0]:cr[0][1] + 1] += 1
             _divide(u, mult, fa)
             _divide(c, mult, ca)
-        # 4. prolongation from every coarser neighbour
+        # 4. the coarse level's physical fill, then prolongation from every
+        # coarser neighbour
         for q in range(nb):
             L = self.level[q]
+            if faces is not None and any(faces[q]) and any(n["level"] < L for n in self.nbrs[q]):
+                orc.physical_bc(self.coarse_geom(q), cs[q], faces[q])
             for nb_ in self.nbrs[q]:
                 if nb_["level"] >= L:
                     continue

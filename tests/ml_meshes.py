@@ -76,3 +76,21 @@ def quintic(x, y, z):
     prolongation) plus lower-order terms."""
     return (1.0 + 0.3 * x - 0.2 * y + 0.1 * z + 0.05 * x * y * z
             + 0.01 * (x - 0.3) ** 5 * (y + 0.2) ** 4 * (z - 0.1) ** 3)
+
+
+def quartic(x, y, z):
+    """Degree <= 4 along every axis: reproduced by the 5-point extrapolation
+    of the physical fill as well as by the VC prolongation."""
+    return (1.0 + 0.3 * x - 0.2 * y + 0.1 * z + 0.05 * x * y * z
+            + 0.01 * (x - 0.3) ** 4 * (y + 0.2) ** 3 * (z - 0.1) ** 2)
+
+
+def open_faces(mesh, q):
+    """Six flags (ix1, ox1, ix2, ox2, ix3, ox3): block q's faces on an open
+    face of the domain."""
+    out = []
+    for d in range(3):
+        m = mesh.nrb0[d] << mesh.level[q]
+        out += [(not mesh.periodic[d]) and mesh.loc[q][d] == 0,
+                (not mesh.periodic[d]) and mesh.loc[q][d] == m - 1]
+    return tuple(bool(v) for v in out)

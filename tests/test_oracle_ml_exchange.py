@@ -24,7 +24,7 @@ import pytest
 
 import ml_exchange as mlx
 import oracle as orc
-from ml_meshes import quintic, refined_blocks
+from ml_meshes import open_faces, quartic, quintic, refined_blocks
 
 
 def _mesh(spec, lo, hi, periodic):
@@ -143,7 +143,37 @@ def _case(name):
                               lambda c, L: bool(np.all(np.abs(c) < (1.0 if L == 0 else 0.5))), 3,
                               (False,) * 3)
         return _mesh(spec, [-2.0] * 3, [2.0] * 3, (False,) * 3)
+    if name == "open_corner":
+        spec = refined_blocks(4, 8, [-2.0] * 3, [2.0] * 3,
+                              lambda c, L: bool(np.all(c < -1.0)), 2, (False,) * 3)
+        return _mesh(spec, [-2.0] * 3, [2.0] * 3, (False,) * 3)
     raise KeyError(name)
+
+
+def test_coarse_physical_fill_at_open_faces():
+    """Fine blocks in a refined corner of an open domain prolongate next to
+    the domain boundary, reading coarse-buffer ghosts beyond it; the coarse
+    level's physical fill (5-point extrapolation over the coarse VC ranges)
+    makes a quartic exact at every fine vertex inside the domain, and
+    without it those vertices are wrong."""
+    m = _case("open_corner")
+    faces = [open_faces(m, q) for q in range(len(m.blocks))]
+    fine_at_face = [q for q in range(len(m.blocks))
+                    if m.level[q] == 1 and any(faces[q]) and
+                    any(n["level"] < 1 for n in m.nbrs[q])]
+    assert fine_at_face
+    us = _fill_interiors(m, quartic)
+    m.ghost_fill(us, faces=faces)
+    for q, u in enumerate(us):
+        x, y, z = _coords(m.blocks[q])
+        inside = _inside(m, x, y, z)
+        err = np.abs(u[0] - quartic(x, y, z))[inside].max()
+        assert err <= 1e-13, (q, err)
+    bad = _fill_interiors(m, quartic)
+    m.ghost_fill(bad)
+    worst = max(np.abs(bad[q][0] - quartic(*_coords(m.blocks[q])))[
+        _inside(m, *_coords(m.blocks[q]))].max() for q in fine_at_face)
+    assert worst > 1e-3
 
 
 def _read(m, q):
