@@ -124,6 +124,13 @@ def parse():
                     help="skip the 100-step evolution + 10-step adjoint sensitivity leg")
     ap.add_argument("--no-exchange", action="store_true",
                     help="skip the step-with-exchange leg on a uniform periodic block grid")
+    ap.add_argument("--no-multilevel", action="store_true",
+                    help="skip the multi-level evolution leg (balanced octree, ghost update across "
+                         "levels after every stage)")
+    ap.add_argument("--multilevel-only", action="store_true",
+                    help="run only the multi-level evolution leg and print its object")
+    ap.add_argument("--ml-steps", type=int, default=100,
+                    help="steps of the multi-level evolution leg")
     ap.add_argument("--sensitivity-only", action="store_true",
                     help="run only the sensitivity leg and print its object (for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -209,6 +216,10 @@ def main():
     blocks, params, owned, adm_fn = make_workload(args.nblocks)
     if args.sensitivity_only:
         print(json.dumps({"sensitivity": sensitivity_run(adm_fn, params)}), flush=True)
+        return
+    if args.multilevel_only:
+        print(json.dumps({"multilevel_evolution": multilevel_run(adm_fn, params, args.ml_steps)}),
+              flush=True)
         return
     interior_pts = int(sum((g.nx[0] + 1) * (g.nx[1] + 1) * (g.nx[2] + 1) for g in blocks))
     log(f"{len(blocks)} blocks, owned points {owned}, interior points {interior_pts}")
@@ -351,6 +362,8 @@ def main():
         out["sensitivity"] = sensitivity_run(adm_fn, params)
     if not args.no_exchange:
         out["exchange_step"] = exchange_step_cost(params)
+    if not args.no_multilevel:
+        out["multilevel_evolution"] = multilevel_run(adm_fn, params, args.ml_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(blocks, params, adm_fn, args.cpu_seconds)
     print(json.dumps(out), flush=True)
@@ -429,6 +442,74 @@ def exchange_step_cost(params, nb=8, n=16, steps=3):
            "fused_point_updates_per_s": owned / t_fused * 1e3}
     log(f"exchange step: fused {t_fused:.2f} ms, stage sequence {t_stages:.2f} ms per step")
     z.close()
+    return res
+
+
+def multilevel_run(adm_fn, params, nsteps=100, nblocks=2048):
+    """The Brill-Lindquist binary on a 2:1-balanced block tree of 16^3-cell
+    blocks over the open box [-32, 32]^3 (levels 0-7, refined toward the
+    punctures, no leaf dropped), moving-puncture gauge, CFL 0.25, outflow
+    physical fill on the box faces: ``nsteps`` bt_rk4 steps replayed from a
+    HIP graph with the GR_Z4c ghost update after every stage (restriction,
+    the same-level / coarser / finer gather with the node multiplicity, the
+    coarse level's physical fill, prolongation, the fine physical fill,
+    AlgConstr of the whole block).  Also times the same step with ghosts
+    held at u_n on the same blocks, for the cost of the update."""
+    import torch
+    from z4c_amd.initial_data import balanced_octree_blocks, owned_points
+    from z4c_amd.z4c import Z4c
+    blocks, lo, hi = balanced_octree_blocks(nblocks=nblocks)
+    z = Z4c(blocks, params)
+    t0 = time.perf_counter()
+    z.set_exchange(lo, hi, (False, False, False))
+    t_plan = time.perf_counter() - t0
+    adm = z.empty(17)
+    for b, g in enumerate(blocks):
+        z.block_view(adm, b, 17).copy_(torch.from_numpy(adm_fn(g)))
+    u = z.zeros()
+    z.ADMToZ4c(adm, u)
+    del adm
+    # the ghosts of the initial data (Mesh::FinalizeZ4cADMGhosts order)
+    z.exchange(u)
+    z.physical_bc(u)
+    z.AlgConstr(u)
+    dt = min(g.dx[0] for g in blocks) * 0.25
+    work = z.make_step_work()
+    z.autotune(u, reps=2)
+    z.rk4_step_fused(dt, u, work, graph=True, exchange=True, physical=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        z.rk4_step_fused(dt, u, work, graph=True, exchange=True, physical=True)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    if not torch.isfinite(u).all():
+        raise RuntimeError("non-finite state in the multi-level evolution")
+    J = z.hamiltonian_norm(u)
+    x = u.clone()
+    z.rk4_step_fused(dt, x, work, graph=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(10):
+        z.rk4_step_fused(dt, x, work, graph=True)
+    torch.cuda.synchronize()
+    T_frozen = (time.perf_counter() - t1) / 10
+    owned = owned_points(blocks)
+    levels = sorted({g.level for g in blocks})
+    z.close()
+    res = {"workload": f"Brill-Lindquist binary, {len(blocks)} blocks of 16^3 cells on a 2:1-"
+                       f"balanced tree, levels {levels[0]}-{levels[-1]}, open box [-32, 32]^3 with "
+                       f"outflow faces, moving-puncture gauge, CFL 0.25, {nsteps} steps (HIP "
+                       f"graph), ghost update across levels after every stage",
+           "nblocks": len(blocks), "owned_points": owned, "dt": dt, "steps": nsteps,
+           "plan_build_s": t_plan, "ms_per_step": T / nsteps * 1e3,
+           "point_updates_per_s": owned * nsteps / T,
+           "frozen_ghost_ms_per_step": T_frozen * 1e3,
+           "ghost_update_ms_per_step": T / nsteps * 1e3 - T_frozen * 1e3,
+           "J_hamiltonian_norm": J}
+    log(f"multi-level evolution: {len(blocks)} blocks, {nsteps} steps in {T:.2f} s "
+        f"({T / nsteps * 1e3:.2f} ms per step, frozen ghosts {T_frozen * 1e3:.2f} ms), "
+        f"plan {t_plan:.2f} s, J = {J:.6e}")
     return res
 
 

@@ -139,6 +139,79 @@ def octree_blocks(nblocks: int = 2048, seed: int = 0, h0: float = 1.0,
     return out
 
 
+def balanced_octree_blocks(nblocks: int = 2048, seed: int = 0, h0: float = 1.0,
+                           max_level: int = 7, cells: int = 16,
+                           punctures=((5.0, 0.0, 0.0), (-5.0, 0.0, 0.0)), nroot: int = 4):
+    """Block tree of equal ``cells``^3 blocks refined toward the punctures,
+    2:1 balanced across faces, edges and corners, covering the open box of
+    nroot^3 level-0 blocks of side cells*h0 (centred on the origin), as the
+    reference's mesh refinement builds it (a block is split only after its
+    neighbours at its level exist, src/mesh/meshblock_tree.cpp).
+
+    Leaves are split in the order of octree_blocks (closest to a puncture
+    relative to size, near-ties broken by the seed) until at least
+    ``nblocks`` exist; no leaf is dropped, so the blocks tile the box.
+    Returns (blocks, lo, hi); faces on the box boundary carry the outflow
+    physical fill (extrapolate_outflow)."""
+    rng = np.random.default_rng(seed)
+    side0 = cells * h0
+    lo = -0.5 * nroot * side0
+    P = np.array(punctures)
+    leaves = {(0, i, j, k) for i in range(nroot) for j in range(nroot) for k in range(nroot)}
+
+    def covering(L, t):
+        """The leaf covering logical cell t at level L (or a finer region)."""
+        for up in range(0, L + 1):
+            key = (L - up,) + tuple(v >> up for v in t)
+            if key in leaves:
+                return key
+        return None
+
+    def split(key):
+        L, i, j, k = key
+        # 2:1 balance: every neighbour region at level L must be a leaf of
+        # level >= L before the children (level L + 1) appear
+        for o in np.ndindex(3, 3, 3):
+            if o == (1, 1, 1):
+                continue
+            t = (i + o[0] - 1, j + o[1] - 1, k + o[2] - 1)
+            if any(not 0 <= v < (nroot << L) for v in t):
+                continue
+            nb = covering(L, t)
+            if nb is not None and nb[0] < L:
+                split(nb)
+        leaves.discard(key)
+        for a in range(2):
+            for b in range(2):
+                for c in range(2):
+                    leaves.add((L + 1, 2 * i + a, 2 * j + b, 2 * k + c))
+
+    while len(leaves) < nblocks:
+        best, best_key = np.inf, None
+        for key in sorted(leaves):
+            if key[0] >= max_level:
+                continue
+            side = side0 * 0.5 ** key[0]
+            c = lo + (np.array(key[1:]) + 0.5) * side
+            sc = np.min(np.linalg.norm(P - c, axis=1)) / side + 0.25 * rng.random()
+            if sc < best:
+                best, best_key = sc, key
+        if best_key is None:
+            break
+        split(best_key)
+    out = []
+    for (L, i, j, k) in sorted(leaves):
+        side = side0 * 0.5 ** L
+        n = nroot << L
+        faces = 0
+        for d, t in enumerate((i, j, k)):
+            faces |= (1 << (2 * d)) if t == 0 else 0
+            faces |= (2 << (2 * d)) if t == n - 1 else 0
+        out.append(MeshBlockGeom.cube(cells, side / cells, lo + np.array([i, j, k]) * side,
+                                      level=L, outflow=faces))
+    return out, (lo,) * 3, (-lo,) * 3
+
+
 def owned_points(blocks):
     """Zone count of the reference's counter: nx1*nx2*nx3 per block per cycle
     (src/main.hpp:729-731)."""
