@@ -1342,49 +1342,81 @@ __global__ void __launch_bounds__(256) k_restrict_vc(RestrictArgs a) {
   for (int n = 0; n < a.nvar; ++n) dst[(long long)n * R.cnpts] = src[(long long)n * B.npts];
 }
 
+// A workgroup of the prolongation: entries [e0, e0 + ne) of one block, all
+// of whose coarse reads lie in the box r0 + [0, rn) of the block's coarse
+// buffer, which the workgroup stages in LDS (one field per workgroup).
+struct PrGroup {
+  int blk;
+  int e0, ne;
+  int r0[3], rn[3];
+};
+constexpr int PR_LDS = 8192;  // largest staged box, doubles (64 KiB)
+
 struct ProlongArgs {
   const DevBlock* blocks;
   const RfBlock* cb;
   const RfEntry* ent;
-  const RfChunk* chunks;
-  int nvar;
+  const PrGroup* groups;
   const double* coarse;
   double* u;
+  int nvar;
 };
 
-// ProlongateVertexCenteredValues (src/mesh/mesh_refinement_vc.cpp:283-772)
-// over one coarse index box per entry.  A fine vertex's value depends only on
-// the coarse buffer and its (coarse index, case), so entries whose fine boxes
-// overlap store the same value there.
+// ProlongateVertexCenteredValues (src/mesh/mesh_refinement_vc.cpp:283-772):
+// grid (groups, fields).  The coarse box is copied to LDS once, then each
+// entry's vertices are taken one case at a time (the parities of the three
+// fine indices), so a wave runs one case's stencil; the arithmetic is
+// rf_prolong_point's.  A fine vertex's value depends only on the coarse
+// buffer and its (coarse index, case), so entries whose boxes overlap store
+// the same value there.
 __global__ void __launch_bounds__(256) k_prolong_vc(ProlongArgs a) {
-  const RfChunk ch = a.chunks[blockIdx.x];
-  const RfEntry E = a.ent[ch.ent];
-  const int t = ch.start + (int)threadIdx.x;
-  if (t >= E.fn[0] * E.fn[1] * E.fn[2]) return;
-  const int fi = E.f0[0] + t % E.fn[0], fj = E.f0[1] + (t / E.fn[0]) % E.fn[1],
-            fk = E.f0[2] + t / (E.fn[0] * E.fn[1]);
-  int ci, cj, ck, pi, pj, pk;
-  if (!rf_axis(E, 0, fi, ci, pi) || !rf_axis(E, 1, fj, cj, pj) || !rf_axis(E, 2, fk, ck, pk))
-    return;
-  const int m = pi | (pj << 1) | (pk << 2);
-  const DevBlock& B = a.blocks[E.blk];
-  const RfBlock R = a.cb[E.blk];
-  const long long p = fi + (long long)B.nn1 * (fj + (long long)B.nn2 * fk);
-  const double* C = a.coarse + R.coff * a.nvar;
-  const long s2 = R.cn1, s3 = (long)R.cn1 * R.cn2;
-  double* dst = a.u + B.off * a.nvar + p;
-  for (int n = 0; n < a.nvar; ++n)
-    dst[(long long)n * B.npts] = rf_prolong_point(C + (long long)n * R.cnpts, s2, s3, ci, cj, ck, m);
+  extern __shared__ double cl[];
+  const PrGroup G = a.groups[blockIdx.x];
+  const int f = blockIdx.y;
+  const DevBlock& B = a.blocks[G.blk];
+  const RfBlock& R = a.cb[G.blk];
+  const int s2 = G.rn[0], s3 = G.rn[0] * G.rn[1], nv = s3 * G.rn[2];
+  {
+    const double* C = a.coarse + R.coff * a.nvar + (long long)f * R.cnpts;
+    for (int t = threadIdx.x; t < nv; t += 256) {
+      const int k = t / s3, r = t - k * s3, j = r / s2, i = r - j * s2;
+      cl[t] = C[(G.r0[0] + i) + (long long)R.cn1 * ((G.r0[1] + j) + (long long)R.cn2 * (G.r0[2] + k))];
+    }
+  }
+  __syncthreads();
+  const int nn1 = B.nn1, nn2 = B.nn2;
+  double* dst = a.u + B.off * a.nvar + (long long)f * B.npts;
+  for (int q = G.e0; q < G.e0 + G.ne; ++q) {
+    const RfEntry E = a.ent[q];
+    for (int m = 0; m < 8; ++m) {
+      const int pi = m & 1, pj = (m >> 1) & 1, pk = m >> 2;
+      int c0[3], n[3];
+      rf_case_range(E, 0, pi, c0[0], n[0]);
+      rf_case_range(E, 1, pj, c0[1], n[1]);
+      rf_case_range(E, 2, pk, c0[2], n[2]);
+      if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) continue;
+      const int n01 = n[0] * n[1], np = n01 * n[2];
+      for (int t = threadIdx.x; t < np; t += 256) {
+        const int tk = t / n01, r = t - tk * n01, tj = r / n[0], ti = r - tj * n[0];
+        const int ci = c0[0] + ti, cj = c0[1] + tj, ck = c0[2] + tk;
+        const double v = rf_prolong_point(cl, s2, s3, ci - G.r0[0], cj - G.r0[1], ck - G.r0[2], m);
+        const int fi = 2 * (ci - RF_H) + pi, fj = 2 * (cj - RF_H) + pj, fk = 2 * (ck - RF_H) + pk;
+        dst[fi + (long long)nn1 * (fj + (long long)nn2 * fk)] = v;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Ghost update across refinement levels (the plan of z4c_mesh.hpp): every
 // destination vertex (fine array or coarse buffer) is its ordered source
 // list summed from zero, times 1/multiplicity, all fields.  Sources are
-// interior vertices read before any destination is written (gather into
-// tmp, then scatter), as the reference packs every message before the
-// receivers unpack.  A reference 2*b + A addresses block b's fine array
-// (A = 0) or coarse buffer (A = 1).
+// interior vertices, read before any of them is written: ghost destinations
+// (never sources) are written at once, interior destinations (the first
+// nint; shared boundary vertices, also sources) go through a scratch and a
+// second launch, as the reference packs every message before the receivers
+// unpack.  A reference 2*b + A addresses block b's fine array (A = 0) or
+// coarse buffer (A = 1).
 // ---------------------------------------------------------------------------
 struct MlxArgs {
   const DevBlock* blocks;
@@ -1394,48 +1426,76 @@ struct MlxArgs {
   const int* src;
   const double* inv;
   int ndst;
+  int nint;
   int nvar;
   double* u;
   double* coarse;
-  double* tmp;  // nvar * ndst, field-major
+  double* tmp;  // nvar * nint, field-major
 };
 
-__device__ __forceinline__ long long mlx_addr(const MlxArgs& a, int blkarr, int p, int n) {
+// base pointer and field stride of a vertex reference
+__device__ __forceinline__ double* mlx_ptr(const MlxArgs& a, int blkarr, int p, long long& fs) {
   const int b = blkarr >> 1;
   if (blkarr & 1) {
     const RfBlock& R = a.cb[b];
-    return -1 - (R.coff * a.nvar + (long long)n * R.cnpts + p);  // coarse: negative
+    fs = R.cnpts;
+    return a.coarse + R.coff * a.nvar + p;
   }
   const DevBlock& B = a.blocks[b];
-  return B.off * a.nvar + (long long)n * B.npts + p;
+  fs = B.npts;
+  return a.u + B.off * a.nvar + p;
 }
 
+// NV > 0: exactly NV fields, sources outer (each resolved once) and the
+// fields' sums in registers; NV = 0: any nvar, fields outer.  Every field's
+// sum adds the sources in list order either way.
+template <int NV>
 __global__ void __launch_bounds__(256) k_mlx_gather(MlxArgs a) {
   const int d = blockIdx.x * 256 + threadIdx.x;
   if (d >= a.ndst) return;
   const long long s0 = a.off[d], s1 = a.off[d + 1];
   const double inv = a.inv[d];
-  for (int n = 0; n < a.nvar; ++n) {
-    double acc = 0.0;
+  long long dfs;
+  double* const dp = mlx_ptr(a, a.dst[2 * d], a.dst[2 * d + 1], dfs);
+  if constexpr (NV > 0) {
+    double acc[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n] = 0.0;
     for (long long e = s0; e < s1; ++e) {
-      const long long q = mlx_addr(a, a.src[2 * e], a.src[2 * e + 1], n);
-      acc += q >= 0 ? a.u[q] : a.coarse[-1 - q];
+      long long fs;
+      const double* sp = mlx_ptr(a, a.src[2 * e], a.src[2 * e + 1], fs);
+#pragma unroll
+      for (int n = 0; n < NV; ++n) acc[n] += sp[n * fs];
     }
-    a.tmp[(long long)n * a.ndst + d] = acc * inv;
+    if (d < a.nint) {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) a.tmp[(long long)n * a.nint + d] = acc[n] * inv;
+    } else {
+#pragma unroll
+      for (int n = 0; n < NV; ++n) dp[n * dfs] = acc[n] * inv;
+    }
+  } else {
+    for (int n = 0; n < a.nvar; ++n) {
+      double acc = 0.0;
+      for (long long e = s0; e < s1; ++e) {
+        long long fs;
+        const double* sp = mlx_ptr(a, a.src[2 * e], a.src[2 * e + 1], fs);
+        acc += sp[n * fs];
+      }
+      if (d < a.nint)
+        a.tmp[(long long)n * a.nint + d] = acc * inv;
+      else
+        dp[n * dfs] = acc * inv;
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) k_mlx_scatter(MlxArgs a) {
   const int d = blockIdx.x * 256 + threadIdx.x;
-  if (d >= a.ndst) return;
-  for (int n = 0; n < a.nvar; ++n) {
-    const long long q = mlx_addr(a, a.dst[2 * d], a.dst[2 * d + 1], n);
-    const double v = a.tmp[(long long)n * a.ndst + d];
-    if (q >= 0)
-      a.u[q] = v;
-    else
-      a.coarse[-1 - q] = v;
-  }
+  if (d >= a.nint) return;
+  long long fs;
+  double* const dp = mlx_ptr(a, a.dst[2 * d], a.dst[2 * d + 1], fs);
+  for (int n = 0; n < a.nvar; ++n) dp[n * fs] = a.tmp[(long long)n * a.nint + d];
 }
 
 // ApplyPhysicalBCsOnCoarseLevel (src/comm/physical_bcs.cpp:687-876) for a
@@ -2088,11 +2148,13 @@ struct z4c_ctx {
   int* ml_src = nullptr;        // 2 ints per source
   double* ml_inv = nullptr;     // 1/multiplicity per destination
   int ml_ndst = 0;
+  int ml_nint = 0;              // interior destinations (listed first)
   double* ml_coarse = nullptr;  // NVAR fields of every block's coarse buffer
   double* ml_tmp = nullptr;     // NVAR * ml_ndst
   RfEntry* ml_pent = nullptr;
-  RfChunk* ml_pchunks = nullptr;
-  int ml_npchunks = 0;
+  PrGroup* ml_pgroups = nullptr;
+  int ml_npgroups = 0;
+  int ml_prlds = 0;             // largest staged coarse box, doubles
   PhysLine* mlp_lines[3] = {};  // the coarse level's physical fill, per axis
   int mlp_nent[3] = {};
   long long mlp_nlines1[3] = {};
@@ -2109,7 +2171,7 @@ struct z4c_ctx {
     hipFree(ml_coarse);
     hipFree(ml_tmp);
     hipFree(ml_pent);
-    hipFree(ml_pchunks);
+    hipFree(ml_pgroups);
     for (int ax = 0; ax < 3; ++ax) {
       hipFree(mlp_lines[ax]);
       mlp_lines[ax] = nullptr;
@@ -2120,8 +2182,8 @@ struct z4c_ctx {
     ml_off = nullptr;
     ml_inv = ml_coarse = ml_tmp = nullptr;
     ml_pent = nullptr;
-    ml_pchunks = nullptr;
-    ml_ndst = ml_npchunks = 0;
+    ml_pgroups = nullptr;
+    ml_ndst = ml_nint = ml_npgroups = ml_prlds = 0;
     ml = false;
     ml_nbrs.clear();
     ml_level.clear();
@@ -2134,8 +2196,9 @@ struct z4c_ctx {
   RfChunk* d_rchunks = nullptr;
   int n_rchunks = 0;
   RfEntry* d_pent = nullptr;
-  RfChunk* d_pchunks = nullptr;
-  int n_pchunks = 0;
+  PrGroup* d_pgroups = nullptr;
+  int n_pgroups = 0;
+  int pr_lds = 0;
   std::vector<double> hst_host;
   int n_cu = 256;
   // captured fused step (z4c_rk4_step_graph) and what it was captured for
@@ -2176,10 +2239,10 @@ struct z4c_ctx {
 
   void drop_prolongation() {
     hipFree(d_pent);
-    hipFree(d_pchunks);
+    hipFree(d_pgroups);
     d_pent = nullptr;
-    d_pchunks = nullptr;
-    n_pchunks = 0;
+    d_pgroups = nullptr;
+    n_pgroups = pr_lds = 0;
   }
 
   void drop_refine() {
@@ -2564,12 +2627,17 @@ int launch_ml_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   HIPCALL(hipGetLastError());
   if (c->ml_ndst) {
     MlxArgs a{c->d_blocks, c->d_rf, c->ml_dst, c->ml_off, c->ml_src, c->ml_inv,
-              c->ml_ndst, nvar, u, c->ml_coarse, c->ml_tmp};
+              c->ml_ndst, c->ml_nint, nvar, u, c->ml_coarse, c->ml_tmp};
     const dim3 g((c->ml_ndst + 255) / 256);
-    hipLaunchKernelGGL(k_mlx_gather, g, dim3(256), 0, s, a);
+    if (nvar == NVAR)
+      hipLaunchKernelGGL(k_mlx_gather<NVAR>, g, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_mlx_gather<0>, g, dim3(256), 0, s, a);
     HIPCALL(hipGetLastError());
-    hipLaunchKernelGGL(k_mlx_scatter, g, dim3(256), 0, s, a);
-    HIPCALL(hipGetLastError());
+    if (c->ml_nint) {
+      hipLaunchKernelGGL(k_mlx_scatter, dim3((c->ml_nint + 255) / 256), dim3(256), 0, s, a);
+      HIPCALL(hipGetLastError());
+    }
   }
   for (int ax = 0; ax < 3; ++ax) {
     if (!c->mlp_nent[ax]) continue;
@@ -2579,9 +2647,10 @@ int launch_ml_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
                        s, pc);
     HIPCALL(hipGetLastError());
   }
-  if (c->ml_npchunks) {
-    ProlongArgs pa{c->d_blocks, c->d_rf, c->ml_pent, c->ml_pchunks, nvar, c->ml_coarse, u};
-    hipLaunchKernelGGL(k_prolong_vc, dim3(c->ml_npchunks), dim3(256), 0, s, pa);
+  if (c->ml_npgroups) {
+    ProlongArgs pa{c->d_blocks, c->d_rf, c->ml_pent, c->ml_pgroups, c->ml_coarse, u, nvar};
+    hipLaunchKernelGGL(k_prolong_vc, dim3(c->ml_npgroups, nvar), dim3(256),
+                       sizeof(double) * c->ml_prlds, s, pa);
     HIPCALL(hipGetLastError());
   }
   return Z4C_OK;
@@ -3311,6 +3380,95 @@ int ensure_coarse(z4c_ctx* c) {
   return Z4C_OK;
 }
 
+// Coarse vertices entry E reads on axis a (injected c, prolongated c - RF_H
+// .. c + RF_HSZ); false when the entry writes nothing on that axis.
+bool rf_read_range(const RfEntry& E, int a, int& lo, int& hi) {
+  lo = std::numeric_limits<int>::max();
+  hi = std::numeric_limits<int>::min();
+  if (E.s[a] <= E.e[a]) {
+    lo = E.s[a];
+    hi = E.e[a];
+  }
+  if (E.sp[a] < E.ep[a]) {
+    lo = std::min(lo, E.sp[a] - RF_H);
+    hi = std::max(hi, E.ep[a] - 1 + RF_HSZ);
+  }
+  return lo <= hi;
+}
+
+// The prolongation's workgroups: entries split (halving their coarse range
+// along the axis of the widest read) until each one's read box fits the
+// LDS bound, then packed per block, in order, into groups whose union box
+// fits it.  `ent` is reordered to the groups' order; returns the largest box.
+int prolong_groups(std::vector<RfEntry>& ent, std::vector<PrGroup>& grp) {
+  std::vector<RfEntry> work(ent.rbegin(), ent.rend()), out;
+  while (!work.empty()) {
+    RfEntry E = work.back();
+    work.pop_back();
+    int lo[3], hi[3];
+    bool any = true;
+    for (int d = 0; d < 3; ++d) any = any && rf_read_range(E, d, lo[d], hi[d]);
+    if (!any) continue;
+    if ((long long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) <= PR_LDS) {
+      out.push_back(E);
+      continue;
+    }
+    int ax = -1, wid = 0;
+    for (int d = 0; d < 3; ++d) {
+      const int c0 = std::min(E.s[d], E.sp[d]), c1 = std::max(E.e[d], E.ep[d] - 1);
+      if (c1 > c0 && hi[d] - lo[d] > wid) {
+        ax = d;
+        wid = hi[d] - lo[d];
+      }
+    }
+    // a single coarse index per axis reads at most 6^3 vertices, so some
+    // axis has a range to halve
+    if (ax < 0) return -1;
+    const int c0 = std::min(E.s[ax], E.sp[ax]), c1 = std::max(E.e[ax], E.ep[ax] - 1);
+    const int cm = (c0 + c1 + 1) / 2;
+    RfEntry A = E, Bq = E;
+    A.e[ax] = std::min(E.e[ax], cm - 1);
+    A.ep[ax] = std::min(E.ep[ax], cm);
+    Bq.s[ax] = std::max(E.s[ax], cm);
+    Bq.sp[ax] = std::max(E.sp[ax], cm);
+    work.push_back(Bq);
+    work.push_back(A);
+  }
+  std::stable_sort(out.begin(), out.end(),
+                   [](const RfEntry& x, const RfEntry& y) { return x.blk < y.blk; });
+  grp.clear();
+  int maxv = 0;
+  for (size_t q = 0; q < out.size(); ++q) {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) rf_read_range(out[q], d, lo[d], hi[d]);
+    if (!grp.empty() && grp.back().blk == out[q].blk) {
+      PrGroup& G = grp.back();
+      int ulo[3], uhi[3];
+      long long v = 1;
+      for (int d = 0; d < 3; ++d) {
+        ulo[d] = std::min(lo[d], G.r0[d]);
+        uhi[d] = std::max(hi[d], G.r0[d] + G.rn[d] - 1);
+        v *= uhi[d] - ulo[d] + 1;
+      }
+      if (v <= PR_LDS) {
+        for (int d = 0; d < 3; ++d) {
+          G.r0[d] = ulo[d];
+          G.rn[d] = uhi[d] - ulo[d] + 1;
+        }
+        ++G.ne;
+        maxv = std::max(maxv, (int)v);
+        continue;
+      }
+    }
+    PrGroup G{out[q].blk, (int)q, 1, {lo[0], lo[1], lo[2]},
+              {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1}};
+    grp.push_back(G);
+    maxv = std::max(maxv, G.rn[0] * G.rn[1] * G.rn[2]);
+  }
+  ent.swap(out);
+  return maxv;
+}
+
 // One prolongation box checked against the block's fine and coarse extents:
 // every coarse vertex the 6-point stencils read and every fine vertex the
 // box writes must lie in the buffers.
@@ -3341,8 +3499,6 @@ int make_entry(const z4c_ctx* c, const z4c_prolong_box& in, int q, RfEntry& E) {
     E.e[a] = e[a];
     E.sp[a] = sp;
     E.ep[a] = ep;
-    E.f0[a] = flo;
-    E.fn[a] = fhi - flo + 1;
   }
   return Z4C_OK;
 }
@@ -3390,15 +3546,15 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
   int rc = ensure_coarse(c);
   if (rc) return rc;
   std::vector<RfEntry> ent(P.boxes.size());
-  std::vector<RfChunk> ch;
   for (size_t q = 0; q < P.boxes.size(); ++q) {
     const auto& bx = P.boxes[q];
     z4c_prolong_box box{bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], bx[6]};
     rc = make_entry(c, box, (int)q, ent[q]);
     if (rc) return rc;
-    const long long np = (long long)ent[q].fn[0] * ent[q].fn[1] * ent[q].fn[2];
-    for (long long s0 = 0; s0 < np; s0 += 256) ch.push_back(RfChunk{(int)q, (int)s0});
   }
+  std::vector<PrGroup> grp;
+  const int prlds = prolong_groups(ent, grp);
+  if (prlds < 0) return fail(Z4C_EINVAL, "set_exchange: prolongation box too large");
   const size_t nd = P.dst.size(), ns = P.src.size();
   if (nd > (size_t)std::numeric_limits<int>::max() / NVAR)
     return fail(Z4C_EINVAL, "set_exchange: plan too large");
@@ -3413,9 +3569,10 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
             up(c->ml_off, P.off.data(), P.off.size()) &&
             up(c->ml_src, reinterpret_cast<const int*>(P.src.data()), 2 * ns) &&
             up(c->ml_inv, P.inv.data(), nd) && up(c->ml_pent, ent.data(), ent.size()) &&
-            up(c->ml_pchunks, ch.data(), ch.size());
+            up(c->ml_pgroups, grp.data(), grp.size());
   ok = ok && hipMalloc(&c->ml_coarse, sizeof(double) * NVAR * (size_t)c->rf_total) == hipSuccess;
-  ok = ok && (nd == 0 || hipMalloc(&c->ml_tmp, sizeof(double) * NVAR * nd) == hipSuccess);
+  ok = ok && (P.n_interior == 0 ||
+               hipMalloc(&c->ml_tmp, sizeof(double) * NVAR * (size_t)P.n_interior) == hipSuccess);
   if (!ok) {
     c->drop_ml();
     return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
@@ -3450,7 +3607,9 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
     }
   }
   c->ml_ndst = (int)nd;
-  c->ml_npchunks = (int)ch.size();
+  c->ml_nint = (int)P.n_interior;
+  c->ml_npgroups = (int)grp.size();
+  c->ml_prlds = prlds;
   c->ml_nbrs = t.nbrs;
   c->ml_level = t.level;
   c->ml = true;
@@ -3526,25 +3685,27 @@ int z4c_set_prolongation(z4c_ctx* c, const z4c_prolong_box* boxes, int nbox) {
   rc = ensure_coarse(c);
   if (rc) return rc;
   std::vector<RfEntry> ent(nbox);
-  std::vector<RfChunk> ch;
   for (int q = 0; q < nbox; ++q) {
     rc = make_entry(c, boxes[q], q, ent[q]);
     if (rc) return rc;
-    const long long np = (long long)ent[q].fn[0] * ent[q].fn[1] * ent[q].fn[2];
-    for (long long s0 = 0; s0 < np; s0 += 256) ch.push_back(RfChunk{q, (int)s0});
-    if (ch.size() > (size_t)std::numeric_limits<int>::max())
-      return fail(Z4C_EINVAL, "set_prolongation: too many boxes");
   }
+  std::vector<PrGroup> grp;
+  const int prlds = prolong_groups(ent, grp);
+  if (prlds < 0) return fail(Z4C_EINVAL, "set_prolongation: box too large");
+  if (ent.size() > (size_t)std::numeric_limits<int>::max() ||
+      grp.size() > (size_t)std::numeric_limits<int>::max())
+    return fail(Z4C_EINVAL, "set_prolongation: too many boxes");
   c->drop_prolongation();
-  if (ch.empty()) return Z4C_OK;
+  if (grp.empty()) return Z4C_OK;
   if (hipMalloc(&c->d_pent, sizeof(RfEntry) * ent.size()) != hipSuccess ||
-      hipMalloc(&c->d_pchunks, sizeof(RfChunk) * ch.size()) != hipSuccess) {
+      hipMalloc(&c->d_pgroups, sizeof(PrGroup) * grp.size()) != hipSuccess) {
     c->drop_prolongation();
     return fail(Z4C_ENOMEM, "set_prolongation: allocation failed");
   }
   HIPCALL(hipMemcpy(c->d_pent, ent.data(), sizeof(RfEntry) * ent.size(), hipMemcpyHostToDevice));
-  HIPCALL(hipMemcpy(c->d_pchunks, ch.data(), sizeof(RfChunk) * ch.size(), hipMemcpyHostToDevice));
-  c->n_pchunks = (int)ch.size();
+  HIPCALL(hipMemcpy(c->d_pgroups, grp.data(), sizeof(PrGroup) * grp.size(), hipMemcpyHostToDevice));
+  c->n_pgroups = (int)grp.size();
+  c->pr_lds = prlds;
   return Z4C_OK;
 }
 
@@ -3552,9 +3713,11 @@ int z4c_prolongate(z4c_ctx* c, int nvar, const double* coarse, double* u, void* 
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!u || !coarse || nvar <= 0) return fail(Z4C_EINVAL, "prolongate: bad argument");
-  if (!c->n_pchunks) return Z4C_OK;
-  ProlongArgs a{c->d_blocks, c->d_rf, c->d_pent, c->d_pchunks, nvar, coarse, u};
-  hipLaunchKernelGGL(k_prolong_vc, dim3(c->n_pchunks), dim3(256), 0, (hipStream_t)stream, a);
+  if (!c->n_pgroups) return Z4C_OK;
+  if (nvar > 65535) return fail(Z4C_EINVAL, "prolongate: too many fields");
+  ProlongArgs a{c->d_blocks, c->d_rf, c->d_pent, c->d_pgroups, coarse, u, nvar};
+  hipLaunchKernelGGL(k_prolong_vc, dim3(c->n_pgroups, nvar), dim3(256), sizeof(double) * c->pr_lds,
+                     (hipStream_t)stream, a);
   HIPCALL(hipGetLastError());
   return Z4C_OK;
 }
@@ -3714,7 +3877,10 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
   const GhostArgs keep_ghosts{c->d_blocks, launch_of(c, BOX_GHOST), u, W1, nullptr,
                               p->eps_floor, p->tf_force_regularization,
                               p->force_regularization};
-  if (ghosts && inplace) {
+  // the ghost update across levels writes every ghost (zero, then the
+  // contributions): u_n's ghosts need no keeping
+  const bool all_ghosts = (ghosts & 2) && c->ml;
+  if (ghosts && inplace && !all_ghosts) {
     PassTimer t(c, Z4C_PASS_GHOSTS, s);
     hipLaunchKernelGGL(k_ghost_copy, dim3(grid_for(keep_ghosts.L.ntiles)), dim3(TILE), 0, s,
                        keep_ghosts);
@@ -3753,7 +3919,7 @@ int launch_rk4_step(z4c_ctx* c, const z4c_params* p, double dt, double* u, doubl
       // SumBT's ghost values (u_n) into the stage state, then the ghost
       // update and AlgConstr of the whole block (z4c_rk4_stage's order)
       double* cur = a.out;
-      if (cur != u || (inplace && st > 1)) {
+      if (!all_ghosts && (cur != u || (inplace && st > 1))) {
         GhostArgs gc{c->d_blocks, launch_of(c, BOX_GHOST), inplace ? W1 : u, cur, nullptr,
                      p->eps_floor, p->tf_force_regularization, p->force_regularization};
         PassTimer t(c, Z4C_PASS_GHOSTS, s);

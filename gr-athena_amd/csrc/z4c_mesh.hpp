@@ -329,10 +329,13 @@ struct Ref {
 };
 
 struct Plan {
+  // destinations, interior vertices first (n_interior of them: they are
+  // also sources, so their new values go through a scratch), then ghosts
   std::vector<Ref> dst;
   std::vector<long long> off{0};
   std::vector<Ref> src;
   std::vector<double> inv;
+  long long n_interior = 0;
   std::vector<std::array<int, 7>> boxes;  // block, si, ei, sj, ej, sk, ek (coarse indices)
   std::vector<int> prolongates;           // per block: has a coarser neighbour
 };
@@ -352,6 +355,7 @@ inline std::string build_plan(const Tree& t, Plan& P) {
   }
   const int fpts = fn[0] * fn[1] * fn[2], cpts = cn[0] * cn[1] * cn[2];
   P.prolongates.assign(t.n, 0);
+  Plan G;  // the ghost destinations, appended after the interior ones
   // per destination of the current block: ordered sources, and whether the
   // list starts from the vertex's own value
   std::vector<std::vector<Ref>> lst[2] = {std::vector<std::vector<Ref>>(fpts),
@@ -453,11 +457,14 @@ inline std::string build_plan(const Tree& t, Plan& P) {
             const int p = i + nn[0] * (j + nn[1] * k);
             const int m = mult[zone_of(k, ax[2])][zone_of(j, ax[1])][zone_of(i, ax[0])];
             if (own[A][p] && m <= 1 && lst[A][p].empty()) continue;
-            P.dst.push_back(Ref{2 * q + A, p});
-            if (own[A][p]) P.src.push_back(Ref{2 * q + A, p});
-            for (const Ref& s : lst[A][p]) P.src.push_back(s);
-            P.off.push_back((long long)P.src.size());
-            P.inv.push_back(m > 1 ? 1.0 / (double)m : 1.0);
+            const bool interior = i >= ax[0].vs && i <= ax[0].ve && j >= ax[1].vs &&
+                                  j <= ax[1].ve && k >= ax[2].vs && k <= ax[2].ve;
+            Plan& T = interior ? P : G;
+            T.dst.push_back(Ref{2 * q + A, p});
+            if (own[A][p]) T.src.push_back(Ref{2 * q + A, p});
+            for (const Ref& s : lst[A][p]) T.src.push_back(s);
+            T.off.push_back((long long)T.src.size());
+            T.inv.push_back(m > 1 ? 1.0 / (double)m : 1.0);
           }
     }
     // prolongation boxes of the coarser neighbours, in neighbour order
@@ -488,6 +495,12 @@ inline std::string build_plan(const Tree& t, Plan& P) {
       P.boxes.push_back(box);
     }
   }
+  P.n_interior = (long long)P.dst.size();
+  const long long base = (long long)P.src.size();
+  P.dst.insert(P.dst.end(), G.dst.begin(), G.dst.end());
+  P.src.insert(P.src.end(), G.src.begin(), G.src.end());
+  P.inv.insert(P.inv.end(), G.inv.begin(), G.inv.end());
+  for (size_t d = 1; d < G.off.size(); ++d) P.off.push_back(base + G.off[d]);
   return "";
 }
 

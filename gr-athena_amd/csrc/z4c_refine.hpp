@@ -38,26 +38,26 @@ __host__ __device__ inline int rf_fine_of(int c, int cvs, int cve, int vs, int v
 
 // One prolongation entry: the coarse index box [s, e] of
 // ProlongateVertexCenteredValues on one block, with its derived prolongated
-// ranges [sp, ep) (si_prl / ei_prl of :300-340) and the fine box [f0, f0+fn)
-// that covers every vertex the entry writes.
+// ranges [sp, ep) (si_prl / ei_prl of :300-340).  Per axis, fine vertex
+// 2 (c - RF_H) is injected from coarse c in [s, e] and fine vertex
+// 2 (c - RF_H) + 1 is prolongated around coarse c in [sp, ep); a vertex is
+// written when all three of its indices are.
 struct RfEntry {
   int blk;
   int s[3], e[3];
   int sp[3], ep[3];
-  int f0[3], fn[3];
 };
 
-// Coarse index c and case (0 injected, 1 prolongated) of fine index f on one
-// axis of an entry; false when the entry does not write f.
-__host__ __device__ inline bool rf_axis(const RfEntry& E, int a, int f, int& c, int& prl) {
-  if ((f & 1) == 0) {
-    c = f / 2 + RF_H;
-    prl = 0;
-    return c >= E.s[a] && c <= E.e[a];
+// Coarse range [c0, c0 + n) of case bit `prl` (0 injected, 1 prolongated)
+// on axis a of an entry; n <= 0 when the entry has no such vertex.
+__host__ __device__ inline void rf_case_range(const RfEntry& E, int a, int prl, int& c0, int& n) {
+  if (prl) {
+    c0 = E.sp[a];
+    n = E.ep[a] - E.sp[a];
+  } else {
+    c0 = E.s[a];
+    n = E.e[a] - E.s[a] + 1;
   }
-  c = (f - 1) / 2 + RF_H;
-  prl = 1;
-  return c >= E.sp[a] && c < E.ep[a];
 }
 
 // Value of one fine vertex from the coarse field C (strides 1, s2, s3) at

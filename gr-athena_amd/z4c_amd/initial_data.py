@@ -12,6 +12,8 @@ Arrays are (nvar, nn3, nn2, nn1) in the reference's AthenaArray order.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from .z4c import NGHOST, MeshBlockGeom
@@ -55,10 +57,14 @@ def perturbed_flat(geom: MeshBlockGeom, seed: int = 0, nbumps: int = 8,
 def gauge_wave_adm(geom: MeshBlockGeom, A: float = 0.01, d: float = 1.0):
     """ADM data of ds^2 = -H dt^2 + H dx^2 + dy^2 + dz^2, H = 1 - A sin(2 pi x/d)
     at t = 0 (ADMGaugeWave1 + GaugeGaugeWave1, src/z4c/awa_z4c.cpp:298-351)."""
-    X, _, _ = _mesh(geom)
+    # the wave is formed on the x line with the C library's sin / cos and
+    # broadcast: numpy's vectorised sin can round one argument differently in
+    # its SIMD body and in its tail, which would make the data depend on y, z
+    # (and on the host's instruction set) at the last bit
+    x, _, _ = geom.coords()
+    sw = np.array([A * math.sin(2.0 * math.pi * (v / d)) for v in x])
+    dsw = np.array([-A * 2.0 * math.pi * math.cos(2.0 * math.pi * (v / d)) for v in x])
     adm = np.zeros((NADM,) + geom.shape)
-    sw = A * np.sin(2.0 * np.pi * (X / d))
-    dsw = -A * 2.0 * np.pi * np.cos(2.0 * np.pi * (X / d))
     adm[4] = 1.0 - sw                      # gxx
     adm[7] = 1.0                           # gyy
     adm[9] = 1.0                           # gzz

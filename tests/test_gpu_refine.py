@@ -123,6 +123,34 @@ def test_whole_block_prolongation_matches_oracle():
             assert np.all(inner != -7.0)
 
 
+def test_large_block_prolongation_matches_oracle():
+    """A block whose coarse buffer (33 x 29 x 31 vertices) exceeds what one
+    workgroup stages: the whole-block box and the 26 neighbour boxes are
+    split into pieces, which must write exactly the oracle's values."""
+    nvar = 3
+    gs = [MeshBlockGeom((48, 40, 44), (0.02, 0.025, 0.03), (0.1, -0.3, 0.2)),
+          MeshBlockGeom((8, 8, 8), (0.1, 0.1, 0.1), (0.0, 0.0, 0.0))]
+    z = Z4c(gs, Params())
+    rng = np.random.default_rng(13)
+    ch = _coarse_host(z, nvar, rng)
+    cs = z.coarse_shape(0)
+    for boxes in ([(0, 4, cs[2] - 5, 4, cs[1] - 5, 4, cs[0] - 5)],
+                  [z.ProlongationRange(0, ox, (1, 0, 1)) for ox in OFFSETS] +
+                  [z.ProlongationRange(1, ox, (0, 1, 1)) for ox in OFFSETS]):
+        z.set_prolongation(boxes)
+        fine0 = [rng.standard_normal((nvar,) + g.shape) for g in gs]
+        u = z.upload(fine0)
+        z.prolongate(_upload_coarse(z, ch), u, nvar)
+        torch.cuda.synchronize()
+        got = z.download(u, nvar)
+        for b, g in enumerate(gs):
+            exp = fine0[b].copy()
+            for box in boxes:
+                if box[0] == b:
+                    orc.prolongate_vc(_og(g), np.ascontiguousarray(ch[b]), exp, box[1:])
+            assert np.array_equal(got[b], exp), b
+
+
 def test_out_of_range_boxes_are_rejected():
     gs = _blocks()
     z = Z4c(gs, Params())
