@@ -25,27 +25,6 @@
 
 namespace z4c {
 
-// Linearisation of the state at an interior vertex (the finite-difference
-// values the right-hand side reads), NLIN doubles per vertex:
-//   first derivatives d_a u_n of the 16 fields with one (lin1 index), 48;
-//   second derivatives d_a d_b u_n of the 11 fields with them (lin2), 66;
-//   selected lop-sided derivatives Lsel_a(u_n) of all 22 fields, 66.
-// Derivative cotangents per vertex (NADJ doubles): the 48 first- and 66
-// second-derivative cotangents, in the same order.
-__host__ __device__ constexpr int lin1(int n) {  // chi, g~, Khat, Gam~, Theta, alpha, beta
-  return n <= V_KHAT ? n : (n >= V_GAMX ? n - 6 : -1);
-}
-__host__ __device__ constexpr int lin2(int n) {  // chi, g~, alpha, beta
-  return n < V_KHAT ? n : (n >= V_ALPHA ? n - 11 : -1);
-}
-constexpr int NLIN1 = 16 * 3;
-constexpr int NLIN2 = 11 * 6;
-constexpr int NLIN = NLIN1 + NLIN2 + NVAR * 3;
-constexpr int NADJ = NLIN1 + NLIN2;
-__host__ __device__ constexpr int l_d1(int n, int a) { return lin1(n) * 3 + a; }
-__host__ __device__ constexpr int l_d2(int n, int ab) { return NLIN1 + lin2(n) * 6 + ab; }
-__host__ __device__ constexpr int l_ls(int n, int a) { return NLIN1 + NLIN2 + n * 3 + a; }
-
 // Parameter gradients per cotangent: d<w, rhs>/d(shift_eta, damp_kappa1, damp_kappa2).
 constexpr int NPBAR = 3;
 
@@ -930,59 +909,10 @@ struct SlotOut {
   __host__ __device__ void d2(int n, int ab, double v) const { sb[l_d2(n, ab) * nint + q] = v; }
 };
 
-// Linearisation at interior vertex (i, j, k): derivative values the
-// right-hand side reads, with the reference's stencils (the mixed second
-// derivatives as the nested Dx_a(Dx_b u), a < b).
-__host__ __device__ inline void adj_lin_deriv_at(const DevBlock& B, const double* u, double* lin,
-                                                 int i, int j, int k) {
-  const int N = B.npts, nint = B.nint;
-  const int st[3] = {1, B.nn1, B.nn1 * B.nn2};
-  const int p = i + B.nn1 * (j + B.nn2 * k);
-  const int q = iidx(B, i, j, k);
-  for (int n = 0; n < NVAR; ++n) {
-    const double* f = u + (long long)n * N + p;
-    if (lin1(n) >= 0)
-      for (int a = 0; a < 3; ++a) {
-        double t = 0.0;
-        for (int o = -3; o <= 3; ++o)
-          if (o) t += w_d1(o) * f[o * st[a]];
-        lin[l_d1(n, a) * nint + q] = t * B.cidx1[a];
-      }
-    if (lin2(n) >= 0)
-      for (int a = 0; a < 3; ++a)
-        for (int b = a; b < 3; ++b) {
-          double t = 0.0;
-          if (a == b) {
-            for (int o = -3; o <= 3; ++o) t += w_d2(o) * f[o * st[a]];
-            t *= B.cidx2[a];
-          } else {
-            for (int o = -3; o <= 3; ++o) {
-              if (!o) continue;
-              double in = 0.0;
-              for (int o2 = -3; o2 <= 3; ++o2)
-                if (o2) in += w_d1(o2) * f[o * st[a] + o2 * st[b]];
-              t += w_d1(o) * (in * B.cidx1[b]);
-            }
-            t *= B.cidx1[a];
-          }
-          lin[l_d2(n, sym(a, b)) * nint + q] = t;
-        }
-    for (int a = 0; a < 3; ++a) {
-      const double vx = u[(long long)(V_BETAX + a) * N + p];
-      double t = 0.0;
-      if (vx < 0)
-        for (int o = -4; o <= 2; ++o) t += w_left(o) * f[o * st[a]];
-      else
-        for (int o = -2; o <= 4; ++o) t += w_right(o) * f[o * st[a]];
-      lin[l_ls(n, a) * nint + q] = t * B.lidx[a];
-    }
-  }
-}
-
 // Forward intermediates of the right-hand side at interior vertex (i, j, k),
-// from u and the derivatives adj_lin_deriv_at stored (a separate pass: the
-// derivative gathers then run at full occupancy, the register-heavy algebra
-// at its own).
+// from u and the derivatives the linearisation pass of the stencil z-march
+// stored (a separate pass: the stencils run as a streaming z-march, the
+// register-heavy algebra at its own occupancy).
 __host__ __device__ inline void adj_lin_fwd_at(const RhsParams& o, const DevBlock& B,
                                                const double* u, double* lin, int i, int j, int k) {
   const int N = B.npts, nint = B.nint;
@@ -993,11 +923,6 @@ __host__ __device__ inline void adj_lin_fwd_at(const RhsParams& o, const DevBloc
   rhs_fwd_point(o, X, [&](int s, double v) { fw[(long long)s * nint] = v; });
 }
 
-__host__ __device__ inline void adj_lin_at(const RhsParams& o, const DevBlock& B,
-                                           const double* u, double* lin, int i, int j, int k) {
-  adj_lin_deriv_at(B, u, lin, i, j, k);
-  adj_lin_fwd_at(o, B, u, lin, i, j, k);
-}
 
 // Pointwise adjoint at interior vertex (i, j, k): writes the point part of
 // ubar and the derivative cotangents, adds the parameter partials to pb.

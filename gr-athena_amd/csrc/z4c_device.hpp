@@ -115,6 +115,27 @@ __host__ __device__ constexpr int s_dd(int n, int ab) { return dbase(n) + 3 + ab
 __host__ __device__ constexpr int s_lap(int n) { return dbase(n) + 3; }
 __host__ __device__ constexpr int s_LD(int n) { return dbase(n) + dnslots(dkind(n)) - 1; }
 
+// Linearisation of the state at an interior vertex (the finite-difference
+// values the right-hand side reads), NLIN doubles per vertex:
+//   first derivatives d_a u_n of the 16 fields with one (lin1 index), 48;
+//   second derivatives d_a d_b u_n of the 11 fields with them (lin2), 66;
+//   selected lop-sided derivatives Lsel_a(u_n) of all 22 fields, 66.
+// Derivative cotangents per vertex (NADJ doubles): the 48 first- and 66
+// second-derivative cotangents, in the same order.
+__host__ __device__ constexpr int lin1(int n) {  // chi, g~, Khat, Gam~, Theta, alpha, beta
+  return n <= V_KHAT ? n : (n >= V_GAMX ? n - 6 : -1);
+}
+__host__ __device__ constexpr int lin2(int n) {  // chi, g~, alpha, beta
+  return n < V_KHAT ? n : (n >= V_ALPHA ? n - 11 : -1);
+}
+constexpr int NLIN1 = 16 * 3;
+constexpr int NLIN2 = 11 * 6;
+constexpr int NLIN = NLIN1 + NLIN2 + NVAR * 3;
+constexpr int NADJ = NLIN1 + NLIN2;
+__host__ __device__ constexpr int l_d1(int n, int a) { return lin1(n) * 3 + a; }
+__host__ __device__ constexpr int l_d2(int n, int ab) { return NLIN1 + lin2(n) * 6 + ab; }
+__host__ __device__ constexpr int l_ls(int n, int a) { return NLIN1 + NLIN2 + n * 3 + a; }
+
 // Tile of the interior x-y plane of a block for the z-marching stencil pass
 // (interior-relative origin i0, j0 and extents ni, nj).
 struct PlaneTile {

@@ -231,6 +231,26 @@ __global__ void __launch_bounds__(Cfg::THREADS) k_stencil(StencilArgs a) {
   }
 }
 
+// The linearisation the adjoint reads (first and second derivatives, the
+// selected lop-sided derivatives; the adjoint's slot layout, stride nint),
+// formed by the stencil pass's z-march with its stencils, 256-thread tiles.
+__global__ void __launch_bounds__(StCfg256::THREADS) k_stencil_lin(StencilArgs a) {
+  __shared__ double sm[2 * StCfg256::BUF];
+  const int G = gridDim.x;  // multiple of 8: XCD-contiguous ranges
+  const int w = blockIdx.x;
+  const int t = (w & 7) * (G >> 3) + (w >> 3);
+  if (t >= a.ntiles * NVAR) return;
+  const int tile = t / NVAR;
+  const int f = t - tile * NVAR;
+  const PlaneTile T = a.tiles[tile];
+  switch (dkind(f)) {
+    case DK_METRIC:
+    case DK_FULL: stencil_tile<StCfg256, DK_FULL, true>(a, T, f, sm); break;
+    case DK_GRAD: stencil_tile<StCfg256, DK_GRAD, true>(a, T, f, sm); break;
+    default: stencil_tile<StCfg256, DK_ADV, true>(a, T, f, sm); break;
+  }
+}
+
 __device__ __forceinline__ PointIO point_io(const RhsArgs& a, const DevBlock& B, int p,
                                             int qi) {
   PointIO X;
@@ -1838,12 +1858,6 @@ struct AdjArgs {
   int hseed;         // 1: seed = d/du of sum vol Ht^2 (w unused, may be null)
 };
 
-__global__ void __launch_bounds__(TILE) k_adj_lin(AdjArgs a) {
-  int b, i, j, k;
-  if (!tile_point(a.L, b, i, j, k)) return;
-  const DevBlock B = a.blocks[b];
-  adj_lin_deriv_at(B, a.u + B.off * NVAR, a.lin + B.ioff * NLINF, i, j, k);
-}
 
 __global__ void __launch_bounds__(TILE) k_adj_fwd(AdjArgs a) {
   int b, i, j, k;
@@ -2127,6 +2141,8 @@ struct z4c_ctx {
   std::vector<int> blk_cfg;
   std::vector<DevBlock> hdev;  // host copy of the device block table
   PlaneTile* st_tiles[NSTCFG] = {};
+  PlaneTile* lin_tiles = nullptr;  // 256-thread tiles of every block (the adjoint's linearisation)
+  int lin_ntiles = 0;
   int st_ntiles[NSTCFG] = {};
   long long total_int = 0;  // interior points
   double* snap = nullptr;  // periodic-fill snapshot / exchange scratch (lazy)
@@ -2279,6 +2295,9 @@ struct z4c_ctx {
       st_tiles[q] = nullptr;
       st_ntiles[q] = 0;
     }
+    hipFree(lin_tiles);
+    lin_tiles = nullptr;
+    lin_ntiles = 0;
     hipFree(snap);
     hipFree(hst);
     hst = nullptr;
@@ -2457,6 +2476,17 @@ int build_stencil_plan(z4c_ctx* c, const std::vector<DevBlock>& hb) {
                       hipMemcpyHostToDevice));
     c->st_ntiles[q] = (int)lists[q].size();
   }
+  std::vector<PlaneTile> lin;
+  for (int b = 0; b < (int)hb.size(); ++b)
+    if (!plane_tiles(0, b, hb[b].ni, hb[b].nj, &lin))
+      return fail(Z4C_EINVAL, "stencil plan: no tile fits block " + std::to_string(b));
+  hipFree(c->lin_tiles);
+  c->lin_tiles = nullptr;
+  c->lin_ntiles = 0;
+  if (hipMalloc(&c->lin_tiles, sizeof(PlaneTile) * lin.size()) != hipSuccess)
+    return fail(Z4C_ENOMEM, "stencil plan: allocation failed");
+  HIPCALL(hipMemcpy(c->lin_tiles, lin.data(), sizeof(PlaneTile) * lin.size(), hipMemcpyHostToDevice));
+  c->lin_ntiles = (int)lin.size();
   return Z4C_OK;
 }
 
@@ -2696,6 +2726,17 @@ int check_vjp_args(const z4c_ctx* c, const z4c_params* p, int nw, const char* wh
   return Z4C_OK;
 }
 
+// The derivative part of the adjoint's linearisation of state u (into
+// adj_lin, which the caller has sized).
+int launch_lin(z4c_ctx* c, const z4c_params* p, const double* u, hipStream_t s) {
+  StencilArgs sa{c->d_blocks, c->lin_tiles, c->lin_ntiles, u, c->adj_lin, p->diss,
+                 p->lapse_advect, p->shift_advect, NLINF};
+  hipLaunchKernelGGL(k_stencil_lin, dim3(grid_for(sa.ntiles * NVAR)), dim3(StCfg256::THREADS), 0,
+                     s, sa);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
 // ubar_c = (d rhs / d u)^T w_c for the nw cotangents; pbar (+)= parameter gradients
 int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, const double* w,
                    double* ubar, double* pbar, int accumulate, hipStream_t s) {
@@ -2707,8 +2748,8 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   if (rc) return rc;
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, w, ubar,
             c->adj_sb, c->adj_part, reg, sreg, w ? 0 : 1};
-  hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
-  HIPCALL(hipGetLastError());
+  rc = launch_lin(c, p, u, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
@@ -2730,8 +2771,8 @@ int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hi
   if (rc) return rc;
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, nullptr,
             nullptr, nullptr, c->adj_part, 0, 0, 1};
-  hipLaunchKernelGGL(k_adj_lin, dim3(Gi), dim3(TILE), 0, s, a);
-  HIPCALL(hipGetLastError());
+  rc = launch_lin(c, p, u, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_hnorm, dim3(Gi), dim3(TILE), 0, s, a);

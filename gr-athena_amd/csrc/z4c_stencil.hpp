@@ -65,20 +65,46 @@ struct StencilArgs {
   const PlaneTile* tiles;
   int ntiles;
   const double* u;  // state, 22 fields
-  double* dv;       // derivative slots
+  double* dv;       // derivative slots (the linearisation buffer in LIN mode)
   double diss;
   double adv_alpha, adv_beta;  // lapse_advect, shift_advect
+  int lin_stride;   // LIN mode: doubles per interior vertex of the buffer
 };
 
-template <class Cfg, int KIND>
+// Selected lop-sided derivative: st_lx without the shift factor, the
+// one-sided stencil chosen by the sign of vx.
+__device__ __forceinline__ double st_lsel(const double* v, double lidx, double vx) {
+  double dl = 0.0;
+  dl += 1.0 * v[0];
+  dl += -8.0 * v[1];
+  dl += 30.0 * v[2];
+  dl += -80.0 * v[3];
+  dl += 35.0 * v[4];
+  dl += 24.0 * v[5];
+  dl += -2.0 * v[6];
+  double dr = 0.0;
+  dr += -1.0 * v[8];
+  dr += 8.0 * v[7];
+  dr += -30.0 * v[6];
+  dr += 80.0 * v[5];
+  dr += -35.0 * v[4];
+  dr += -24.0 * v[3];
+  dr += 2.0 * v[2];
+  return ((vx < 0) ? dl : dr) * lidx;
+}
+
+// LIN: the linearisation the adjoint reads instead of the derivative
+// slots (first derivatives, all six second derivatives, the selected
+// lop-sided derivative per axis; slot stride nint), same stencils.
+template <class Cfg, int KIND, bool LIN = false>
 __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTile T,
                                              int f, double* sm) {
   constexpr int ST_THREADS = Cfg::THREADS;
   constexpr int ST_CUR = Cfg::CUR;
   constexpr int ST_EXT = Cfg::EXT;
   constexpr int ST_BUF = Cfg::BUF;
-  constexpr bool metric = KIND == DK_METRIC;
-  constexpr bool full = KIND == DK_FULL || metric;  // second derivatives
+  constexpr bool metric = KIND == DK_METRIC && !LIN;  // P_ab instead of d_a d_b
+  constexpr bool full = KIND == DK_FULL || KIND == DK_METRIC;  // second derivatives
   constexpr bool grad = KIND != DK_ADV;
   const DevBlock B = a.blocks[T.block];
   const int N = B.npts;
@@ -88,8 +114,9 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
   const Buf U = make_buf(a.u + B.off * NVAR + (long long)f * N, (long long)N * 8);
   const Buf BETA = make_buf(a.u + B.off * NVAR + (long long)V_BETAX * N, 3LL * N * 8);
   const Buf GM = make_buf(a.u + B.off * NVAR + (long long)V_GXX * N, 6LL * N * 8);
-  const Buf DVB = make_buf(a.dv + B.doff, (long long)NDV * B.nis * 8);
-  const int NS8 = B.nis * 8;
+  const Buf DVB = LIN ? make_buf(a.dv + B.ioff * a.lin_stride, (long long)NLIN * B.nint * 8)
+                      : make_buf(a.dv + B.doff, (long long)NDV * B.nis * 8);
+  const int NS8 = (LIN ? B.nint : B.nis) * 8;
   const int db = dbase(f), sld = s_LD(f);
   const double lc = (f == V_ALPHA) ? a.adv_alpha : ((f >= V_BETAX) ? a.adv_beta : 1.0);
   const int PX = T.ni + 8, PP = PX * (T.nj + 8);
@@ -232,26 +259,38 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
       double xv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) xv[m] = cur[o + (m - 4)];
-      if (grad) put(st_dx(xv, B.cidx1[0]), db + 0);
+      if (grad) put(st_dx(xv, B.cidx1[0]), LIN ? l_d1(f, 0) : db + 0);
       if (full) dd[0] = st_dxx(xv, B.cidx2[0]);
-      L += st_lx(xv, B.lidx[0], bx);
-      D = st_diss(xv, B.cidxd[0], a.diss);
+      if (LIN) {
+        put(st_lsel(xv, B.lidx[0], bx), l_ls(f, 0));
+      } else {
+        L += st_lx(xv, B.lidx[0], bx);
+        D = st_diss(xv, B.cidxd[0], a.diss);
+      }
     }
     {
       double yv[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) yv[m] = cur[o + (m - 4) * PX];
-      if (grad) put(st_dx(yv, B.cidx1[1]), db + 1);
+      if (grad) put(st_dx(yv, B.cidx1[1]), LIN ? l_d1(f, 1) : db + 1);
       if (full) dd[3] = st_dxx(yv, B.cidx2[1]);
-      L += st_lx(yv, B.lidx[1], by);
-      D += st_diss(yv, B.cidxd[1], a.diss);
+      if (LIN) {
+        put(st_lsel(yv, B.lidx[1], by), l_ls(f, 1));
+      } else {
+        L += st_lx(yv, B.lidx[1], by);
+        D += st_diss(yv, B.cidxd[1], a.diss);
+      }
     }
-    if (grad) put(st_dx(zr, B.cidx1[2]), db + 2);
+    if (grad) put(st_dx(zr, B.cidx1[2]), LIN ? l_d1(f, 2) : db + 2);
     if (full) dd[5] = st_dxx(zr, B.cidx2[2]);
-    // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
-    L += st_lx(zr, B.lidx[2], bz);
-    D += st_diss(zr, B.cidxd[2], a.diss);
-    put(lc * L + D, sld);
+    if (LIN) {
+      put(st_lsel(zr, B.lidx[2], bz), l_ls(f, 2));
+    } else {
+      // advective sum with the shift at the point (calculate_z4c_rhs.cpp:184-226)
+      L += st_lx(zr, B.lidx[2], bz);
+      D += st_diss(zr, B.cidxd[2], a.diss);
+      put(lc * L + D, sld);
+    }
     if (full) {
       double v[9];
       v[0] = v[8] = 0.0;
@@ -275,7 +314,7 @@ __device__ __forceinline__ void stencil_tile(const StencilArgs& a, const PlaneTi
         put(P, db + 3);
       } else {
 #pragma unroll
-        for (int ab = 0; ab < 6; ++ab) put(dd[ab], db + 3 + ab);
+        for (int ab = 0; ab < 6; ++ab) put(dd[ab], LIN ? l_d2(f, ab) : db + 3 + ab);
       }
     }
     q8 += qstep8;
