@@ -58,9 +58,8 @@ def gauge_wave_adm(geom: MeshBlockGeom, A: float = 0.01, d: float = 1.0):
     """ADM data of ds^2 = -H dt^2 + H dx^2 + dy^2 + dz^2, H = 1 - A sin(2 pi x/d)
     at t = 0 (ADMGaugeWave1 + GaugeGaugeWave1, src/z4c/awa_z4c.cpp:298-351)."""
     # the wave is formed on the x line with the C library's sin / cos and
-    # broadcast: numpy's vectorised sin can round one argument differently in
-    # its SIMD body and in its tail, which would make the data depend on y, z
-    # (and on the host's instruction set) at the last bit
+    # broadcast, so that every (y, z) row holds the same bits and the data do
+    # not depend on the host's SIMD code paths
     x, _, _ = geom.coords()
     sw = np.array([A * math.sin(2.0 * math.pi * (v / d)) for v in x])
     dsw = np.array([-A * 2.0 * math.pi * math.cos(2.0 * math.pi * (v / d)) for v in x])
