@@ -1315,10 +1315,10 @@ __global__ void __launch_bounds__(256) k_exchange_scatter(ExArgs a) {
 // ---------------------------------------------------------------------------
 // Vertex-centred restriction and prolongation between levels
 // (src/mesh/mesh_refinement_vc.cpp; the index algebra and the per-vertex
-// operator are in z4c_refine.hpp).  Both kernels take a host-built chunk
-// table: workgroup w handles points start .. start+255 of entry ent (a block
-// for the restriction, a prolongation box for the prolongation), one thread
-// per vertex, all fields.
+// operator are in z4c_refine.hpp).  The restriction takes a host-built chunk
+// table (workgroup w handles coarse vertices start .. start+255 of block ent,
+// one thread per vertex, all fields); the prolongation takes groups of boxes
+// whose coarse reads are staged in LDS (PrGroup below).
 // ---------------------------------------------------------------------------
 struct RfBlock {
   long long coff;     // point offset of the block's coarse buffer
@@ -2205,7 +2205,7 @@ struct z4c_ctx {
     ml_level.clear();
   }
   // refinement (lazy): coarse-buffer table and restriction chunks, and the
-  // prolongation entries set by z4c_set_prolongation with their chunks
+  // prolongation entries set by z4c_set_prolongation with their groups
   std::vector<RfBlock> hrf;
   long long rf_total = 0;
   RfBlock* d_rf = nullptr;
