@@ -80,6 +80,29 @@ def test_ml_exchange_matches_oracle(case):
     z.close()
 
 
+def test_ml_exchange_three_fields_12cubed():
+    """The update of a 3-field buffer (the gather's any-field path) on 3
+    levels of 12^3 blocks, twice, bit-equal to the oracle."""
+    c, geoms, mesh = _blocks("open_3level", nx=12)
+    faces = [open_faces(mesh, q) for q in range(len(geoms))]
+    for g, f in zip(geoms, faces):
+        g.outflow_faces = sum(1 << i for i in range(6) if f[i])
+    rng = np.random.default_rng(9)
+    host = [np.ascontiguousarray(rng.standard_normal((3,) + g.shape)) for g in geoms]
+    z = Z4c(geoms, Params())
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    u = z.upload(host, 3)
+    exp = [h.copy() for h in host]
+    for _ in range(2):
+        z.exchange(u, 3)
+        torch.cuda.synchronize()
+        got = z.download(u, 3)
+        mesh.ghost_fill(exp, faces=faces)
+        for b in range(len(geoms)):
+            assert np.array_equal(got[b], exp[b]), b
+    z.close()
+
+
 def test_ml_exchange_rejects_unbalanced_and_unfilled_open_faces():
     # a level-2 block next to a level-0 block
     gs = [MeshBlockGeom.cube(8, 1.0 / 8, (0.0, 0.0, 0.0)),
