@@ -18,6 +18,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -1311,6 +1312,61 @@ __global__ void __launch_bounds__(256) k_exchange_scatter(ExArgs a) {
     a.u[B.off * a.nvar + (long long)n * B.npts + p] = a.tmp[(long long)n * a.ndst + d];
 }
 
+// Transpose of the same-level exchange on a cotangent u-bar (z4c_exchange_vjp):
+// the exchange sets destination d to w_d * (sum of its sources), w_d = 1/cnt
+// (1 for a single source), and leaves every other vertex alone.  So a
+// source vertex v receives sum_{d reading v} w_d ubar_d, plus its own ubar
+// unless v is itself a destination; a destination that is no source (a
+// ghost) receives zero.  Gather into a scratch, then scatter, so every
+// value read is a pre-transpose value; the sums run in destination order.
+struct ExTArgs {
+  const DevBlock* blocks;
+  const int* dst;      // the exchange's destinations (2 ints each)
+  const int* src_off;  // the exchange's CSR offsets (source counts)
+  const int* v;        // source vertices (2 ints each)
+  const int* flag;     // 1 if the source vertex is also a destination
+  const int* voff;     // CSR offsets into vd
+  const int* vd;       // destinations reading each source vertex
+  const int* z;        // destinations that are no source
+  int nv, nz, nvar;
+  double* ubar;
+  double* tmp;         // nvar * nv
+};
+
+__global__ void __launch_bounds__(256) k_exchange_vjp_gather(ExTArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.nv) return;
+  const DevBlock& V = a.blocks[a.v[2 * t]];
+  const int pv = a.v[2 * t + 1];
+  const int e0 = a.voff[t], e1 = a.voff[t + 1];
+  for (int n = 0; n < a.nvar; ++n) {
+    double acc = a.flag[t] ? 0.0 : a.ubar[V.off * a.nvar + (long long)n * V.npts + pv];
+    for (int e = e0; e < e1; ++e) {
+      const int d = a.vd[e];
+      const int cnt = a.src_off[d + 1] - a.src_off[d];
+      const DevBlock& D = a.blocks[a.dst[2 * d]];
+      const double ud = a.ubar[D.off * a.nvar + (long long)n * D.npts + a.dst[2 * d + 1]];
+      acc += cnt > 1 ? ud * (1.0 / (double)cnt) : ud;
+    }
+    a.tmp[(long long)n * a.nv + t] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_exchange_vjp_scatter(ExTArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < a.nv) {
+    const DevBlock& V = a.blocks[a.v[2 * t]];
+    const int pv = a.v[2 * t + 1];
+    for (int n = 0; n < a.nvar; ++n)
+      a.ubar[V.off * a.nvar + (long long)n * V.npts + pv] = a.tmp[(long long)n * a.nv + t];
+  } else if (t < a.nv + a.nz) {
+    const int d = a.z[t - a.nv];
+    const DevBlock& D = a.blocks[a.dst[2 * d]];
+    for (int n = 0; n < a.nvar; ++n)
+      a.ubar[D.off * a.nvar + (long long)n * D.npts + a.dst[2 * d + 1]] = 0.0;
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Vertex-centred restriction and prolongation between levels
@@ -2028,6 +2084,10 @@ struct AdjStepArgs {
   int stage;             // 5 = final sum, 4..1 = stage input u_s
   double eps_floor;
   int tf_force, force_reg;
+  // exchange mode (k_adj_step_ex): the stage cotangent, transformed in
+  // place, and AlgConstr's input E(y_s)
+  double* ybar_stage;
+  const double* pre;
 };
 
 // un and the stage registers a.k[] share the state layout: both are read at
@@ -2120,6 +2180,65 @@ __global__ void __launch_bounds__(TILE) k_adj_step(AdjStepArgs a) {
       a.kbar[cb + n * N + p] = (bw[s - 2] * a.dt) * a.ybar[cb + n * N + p] + (aw[s] * a.dt) * bar[n];
   }
 }
+
+// The step adjoint with the ghost update after every stage (exchange mode):
+// u_s = AlgConstr(E(y_s)) with y_s the stage's sum and E the exchange, so
+// the stage cotangent goes through AlgConstr's adjoint at E(y_s) (PHASE 1,
+// in place; E(y_s) kept by the forward sweep in a.pre), then through E's
+// transpose (launch_exchange_vjp), then splits into the cotangents of u_n
+// and of the k's (PHASE 2), as k_adj_step does.
+template <int PHASE>
+__global__ void __launch_bounds__(TILE) k_adj_step_ex(AdjStepArgs a) {
+  const long long c = blockIdx.y;
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const long long N = B.npts, p = flat(B, i, j, k);
+  const bool interior = is_interior(B, i, j, k);
+  const long long boff = B.off * NVAR;
+  const long long cb = c * a.reg + boff;
+  double* bar = a.ybar_stage + cb;
+  if (PHASE == 1) {
+    const double* y = a.pre + boff;
+    double g[6], A[6], bg[6], bA[6];
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      g[ab] = y[(V_GXX + ab) * N + p];
+      A[ab] = y[(V_AXX + ab) * N + p];
+      bg[ab] = bar[(V_GXX + ab) * N + p];
+      bA[ab] = bar[(V_AXX + ab) * N + p];
+    }
+    double bchi = bar[V_CHI * N + p], balpha = bar[V_ALPHA * N + p];
+    algconstr_vjp(g, A, y[V_CHI * N + p], y[V_ALPHA * N + p], bg, bA, bchi, balpha, a.eps_floor,
+                  a.tf_force, a.force_reg);
+#pragma unroll
+    for (int ab = 0; ab < 6; ++ab) {
+      bar[(V_GXX + ab) * N + p] = bg[ab];
+      bar[(V_AXX + ab) * N + p] = bA[ab];
+    }
+    bar[V_CHI * N + p] = bchi;
+    bar[V_ALPHA * N + p] = balpha;
+    return;
+  }
+  const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  const double aw[5] = {0., 0., 0.5, 0.5, 1.0};  // a_s of stage s
+  const int s = a.stage;
+#pragma unroll
+  for (int n = 0; n < NVAR; ++n) {
+    const double v = bar[n * N + p];
+    if (s == 5) {
+      a.ubar_in[cb + n * N + p] = v;
+      if (interior) {
+        a.ybar[cb + n * N + p] = v;
+        a.kbar[cb + n * N + p] = (bw[3] * a.dt) * v;
+      }
+    } else {
+      a.ubar_in[cb + n * N + p] += v;
+      if (s >= 2 && interior)
+        a.kbar[cb + n * N + p] = (bw[s - 2] * a.dt) * a.ybar[cb + n * N + p] + (aw[s] * a.dt) * v;
+    }
+  }
+}
 }  // namespace
 
 // ===========================================================================
@@ -2155,6 +2274,27 @@ struct z4c_ctx {
   int* ex_off = nullptr;
   int* ex_src = nullptr;
   int ex_ndst = 0;
+  // its transpose (z4c_exchange_vjp): the source vertices (2 ints each),
+  // whether each is itself a destination, CSR of the destinations that read
+  // it, the destinations that are no source (ghosts), and a scratch of
+  // NVAR values per source vertex
+  int* ext_v = nullptr;
+  int* ext_flag = nullptr;
+  int* ext_off = nullptr;
+  int* ext_d = nullptr;
+  int* ext_z = nullptr;
+  int ext_nv = 0, ext_nz = 0;
+  double* ext_tmp = nullptr;
+  void drop_ext() {
+    int** q[5] = {&ext_v, &ext_flag, &ext_off, &ext_d, &ext_z};
+    for (int** x : q) {
+      hipFree(*x);
+      *x = nullptr;
+    }
+    hipFree(ext_tmp);
+    ext_tmp = nullptr;
+    ext_nv = ext_nz = 0;
+  }
   // ghost update across refinement levels (z4c_set_exchange on blocks of
   // several spacings): gather lists, the context's coarse buffers, the
   // gathered values, and the prolongation entries of the coarser neighbours
@@ -2241,6 +2381,8 @@ struct z4c_ctx {
   double* adj_k = nullptr;
   double* adj_stage = nullptr;
   double* adj_reg = nullptr;
+  double* adj_pre = nullptr;  // step VJP with the exchange: the exchanged sums entering AlgConstr
+  long long adj_pre_n = 0;
   bool adj_k_zero = false;  // adj_k's ghosts hold zeros (set by the first step VJP)
   long long adj_lin_n = 0, adj_sb_n = 0, adj_part_n = 0, adj_k_n = 0, adj_stage_n = 0,
             adj_reg_n = 0;
@@ -2312,6 +2454,7 @@ struct z4c_ctx {
     hipFree(ex_src);
     ex_dst = ex_off = ex_src = nullptr;
     ex_ndst = 0;
+    drop_ext();
     drop_ml();
     drop_refine();
     hst_host.clear();
@@ -2321,6 +2464,9 @@ struct z4c_ctx {
       *q = nullptr;
     }
     adj_lin_n = adj_sb_n = adj_part_n = adj_k_n = adj_stage_n = adj_reg_n = 0;
+    hipFree(adj_pre);
+    adj_pre = nullptr;
+    adj_pre_n = 0;
     adj_k_zero = false;
     d_blocks = nullptr;
     d1 = nullptr;
@@ -2702,6 +2848,23 @@ int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   return Z4C_OK;
 }
 
+// u-bar <- (the same-level exchange)^T u-bar, nvar fields
+int launch_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
+  if (c->ml) return fail(Z4C_EINVAL, "exchange_vjp: the update across levels has no transpose here");
+  if (nvar > NVAR) return fail(Z4C_EINVAL, "exchange_vjp: at most 22 fields");
+  if (!c->ext_nv && !c->ext_nz) return Z4C_OK;
+  ExTArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ext_v, c->ext_flag, c->ext_off, c->ext_d,
+            c->ext_z, c->ext_nv, c->ext_nz, nvar, ubar, c->ext_tmp};
+  if (c->ext_nv) {
+    hipLaunchKernelGGL(k_exchange_vjp_gather, dim3((c->ext_nv + 255) / 256), dim3(256), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_exchange_vjp_scatter, dim3((c->ext_nv + c->ext_nz + 255) / 256), dim3(256),
+                     0, s, a);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
 int ensure_buf(double*& p, long long& have, long long need, const char* what) {
   if (have >= need) return Z4C_OK;
   hipFree(p);
@@ -2859,6 +3022,106 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
     a.bar = usbar;
     a.stage = st;
     hipLaunchKernelGGL(k_adj_step, g, dim3(TILE), 0, s, a);
+    HIPCALL(hipGetLastError());
+  }
+  return Z4C_OK;
+}
+
+// Adjoint of z4c_rk4_step_ghosts with the same-level exchange after every
+// stage (ghosts = 2): the forward sweep forms y_s = u_n + a_s dt k_{s-1}
+// (ghosts at u_n), E(y_s) (kept as AlgConstr's linearisation point) and
+// u_s = AlgConstr(E(y_s)), with k_s = rhs(u_s); the reverse sweep carries
+// each stage cotangent through AlgConstr's adjoint, E's transpose and the
+// stage sums.
+int launch_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u,
+                               int nw, const double* ubar_out, double* ubar_in, double* pbar,
+                               hipStream_t s) {
+  const long long reg = c->total_pts * NVAR;
+  const long long k_n_before = c->adj_k_n;
+  int rc = ensure_buf(c->adj_k, c->adj_k_n, 4 * reg, "rk4_step_ghosts_vjp");
+  if (c->adj_k_n != k_n_before) c->adj_k_zero = false;
+  if (!rc) rc = ensure_buf(c->adj_stage, c->adj_stage_n, 3 * reg, "rk4_step_ghosts_vjp");
+  if (!rc) rc = ensure_buf(c->adj_pre, c->adj_pre_n, 4 * reg, "rk4_step_ghosts_vjp");
+  if (!rc) rc = ensure_buf(c->adj_reg, c->adj_reg_n, 3LL * nw * reg, "rk4_step_ghosts_vjp");
+  if (rc) return rc;
+  double* K[4] = {c->adj_k, c->adj_k + reg, c->adj_k + 2 * reg, c->adj_k + 3 * reg};
+  double* ybar = c->adj_reg;
+  double* kbar = c->adj_reg + (long long)nw * reg;
+  double* usbar = c->adj_reg + 2LL * nw * reg;
+  const double aw[5] = {0., 0., 0.5, 0.5, 1.0};
+  const double bw[4] = {1. / 6., 1. / 3., 1. / 3., 1. / 6.};
+  // the right-hand side writes interior vertices only (see launch_rk4_step_vjp)
+  if (!c->adj_k_zero) {
+    rc = fill(c, c->adj_k, 0.0, c->adj_k_n, s);
+    if (rc) return rc;
+    c->adj_k_zero = true;
+  }
+  auto pre = [&](int st) -> double* { return c->adj_pre + (long long)(st - 2) * reg; };  // 2..5
+  auto stage_buf = [&](int st) -> double* { return c->adj_stage + (long long)(st - 2) * reg; };
+  for (int st = 1; st <= 4; ++st) {
+    const double* in = u;
+    if (st >= 2) {
+      const double* kk[1] = {K[st - 2]};
+      const double wa[1] = {aw[st]};
+      rc = launch_bt(c, NVAR, pre(st), u, 1, kk, wa, dt, s);
+      if (!rc) rc = launch_exchange(c, NVAR, pre(st), s);
+      if (rc) return rc;
+      HIPCALL(hipMemcpyAsync(stage_buf(st), pre(st), sizeof(double) * reg, hipMemcpyDeviceToDevice, s));
+      rc = launch_alg(c, p, stage_buf(st), nullptr, 0, s);
+      if (rc) return rc;
+      in = stage_buf(st);
+    }
+    rc = launch_rhs(c, p, in, nullptr, K[st - 1], false, s);
+    if (rc) return rc;
+  }
+  {
+    const double* kk[4] = {K[0], K[1], K[2], K[3]};
+    rc = launch_bt(c, NVAR, pre(5), u, 4, kk, bw, dt, s);
+    if (!rc) rc = launch_exchange(c, NVAR, pre(5), s);
+    if (rc) return rc;
+  }
+  AdjStepArgs a{};
+  a.blocks = c->d_blocks;
+  a.L = launch_of(c, BOX_WHOLE);
+  a.un = u;
+  for (int q = 0; q < 4; ++q) a.k[q] = K[q];
+  a.ubar_in = ubar_in;
+  a.ybar = ybar;
+  a.kbar = kbar;
+  a.reg = reg;
+  a.dt = dt;
+  a.eps_floor = p->eps_floor;
+  a.tf_force = p->tf_force_regularization;
+  a.force_reg = p->force_regularization;
+  a.ybar_stage = usbar;
+  const dim3 g(grid_for(a.L.ntiles), nw);
+  // cotangent of the exchanged, constrained sums: AlgConstr's adjoint, E^T
+  auto through_update = [&](int st) -> int {
+    a.pre = pre(st);
+    hipLaunchKernelGGL(k_adj_step_ex<1>, g, dim3(TILE), 0, s, a);
+    HIPCALL(hipGetLastError());
+    for (int q = 0; q < nw; ++q) {
+      const int r = launch_exchange_vjp(c, NVAR, usbar + (long long)q * reg, s);
+      if (r) return r;
+    }
+    return Z4C_OK;
+  };
+  HIPCALL(hipMemcpyAsync(usbar, ubar_out, sizeof(double) * reg * nw, hipMemcpyDeviceToDevice, s));
+  a.stage = 5;
+  rc = through_update(5);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_adj_step_ex<2>, g, dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  for (int st = 4; st >= 1; --st) {
+    const double* in = st == 1 ? u : stage_buf(st);
+    rc = launch_rhs_vjp(c, p, in, nw, kbar, usbar, pbar, st == 4 ? 0 : 1, s);
+    if (rc) return rc;
+    a.stage = st;
+    if (st >= 2) {
+      rc = through_update(st);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_adj_step_ex<2>, g, dim3(TILE), 0, s, a);
     HIPCALL(hipGetLastError());
   }
   return Z4C_OK;
@@ -3266,6 +3529,7 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
     hipFree(c->ex_src);
     c->ex_dst = c->ex_off = c->ex_src = nullptr;
     c->ex_ndst = 0;
+    c->drop_ext();
     return build_ml_exchange(c, lo, hi, periodic);
   }
   c->drop_ml();
@@ -3355,6 +3619,7 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
   hipFree(c->ex_src);
   c->ex_dst = c->ex_off = c->ex_src = nullptr;
   c->ex_ndst = 0;
+  c->drop_ext();
   if (dst.empty()) return Z4C_OK;
   if (hipMalloc(&c->ex_dst, sizeof(int) * dst.size()) != hipSuccess ||
       hipMalloc(&c->ex_off, sizeof(int) * off.size()) != hipSuccess ||
@@ -3364,6 +3629,58 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
   HIPCALL(hipMemcpy(c->ex_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   HIPCALL(hipMemcpy(c->ex_src, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
   c->ex_ndst = (int)(dst.size() / 2);
+  {
+    // the transpose: every source vertex with the destinations that read
+    // it (in destination order), and the destinations no list reads
+    const size_t nd = dst.size() / 2;
+    auto vkey = [](int b, int p) { return ((long long)b << 32) | (unsigned)p; };
+    std::unordered_map<long long, int> vid;
+    std::vector<int> tv;
+    std::vector<std::vector<int>> tl;
+    for (size_t d = 0; d < nd; ++d)
+      for (int e = off[d]; e < off[d + 1]; ++e) {
+        const long long k = vkey(src[2 * e], src[2 * e + 1]);
+        auto it = vid.find(k);
+        int id;
+        if (it == vid.end()) {
+          id = (int)tl.size();
+          vid.emplace(k, id);
+          tv.push_back(src[2 * e]);
+          tv.push_back(src[2 * e + 1]);
+          tl.emplace_back();
+        } else {
+          id = it->second;
+        }
+        tl[id].push_back((int)d);
+      }
+    std::vector<int> tflag(tl.size(), 0), toff{0}, td, tz;
+    for (size_t d = 0; d < nd; ++d) {
+      auto it = vid.find(vkey(dst[2 * d], dst[2 * d + 1]));
+      if (it == vid.end())
+        tz.push_back((int)d);
+      else
+        tflag[it->second] = 1;
+    }
+    for (auto& l : tl) {
+      td.insert(td.end(), l.begin(), l.end());
+      toff.push_back((int)td.size());
+    }
+    auto up = [&](int*& dptr, const std::vector<int>& h) {
+      if (h.empty()) return true;
+      if (hipMalloc(&dptr, sizeof(int) * h.size()) != hipSuccess) return false;
+      return hipMemcpy(dptr, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(c->ext_v, tv) && up(c->ext_flag, tflag) && up(c->ext_off, toff) &&
+              up(c->ext_d, td) && up(c->ext_z, tz);
+    ok = ok && (tl.empty() ||
+                hipMalloc(&c->ext_tmp, sizeof(double) * NVAR * tl.size()) == hipSuccess);
+    if (!ok) {
+      c->drop_ext();
+      return fail(Z4C_ENOMEM, "set_exchange: allocation failed");
+    }
+    c->ext_nv = (int)tl.size();
+    c->ext_nz = (int)tz.size();
+  }
   // the gather scratch for up to NVAR fields now, so that z4c_exchange
   // allocates nothing (and a step with the exchange can be captured)
   return ensure_snapshot(c, (long long)c->ex_ndst * NVAR);
@@ -4361,6 +4678,33 @@ int z4c_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u
     if (b.sommerfeld_faces & 63)
       return fail(Z4C_EINVAL, "rk4_step_vjp: Sommerfeld faces have no adjoint here");
   return launch_rk4_step_vjp(c, p, dt, u, nw, ubar_out, ubar_in, pbar, (hipStream_t)stream);
+}
+
+int z4c_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u, int nw,
+                            const double* ubar_out, double* ubar_in, double* pbar, int ghosts,
+                            void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!p || !u || !ubar_out || !ubar_in || !pbar)
+    return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: null argument");
+  rc = check_vjp_args(c, p, nw, "rk4_step_ghosts_vjp");
+  if (rc) return rc;
+  if (ghosts != 2)
+    return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: ghosts must be 2 (the exchange set by "
+                            "z4c_set_exchange)");
+  if (c->ml)
+    return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: the update across levels has no transpose here");
+  for (const z4c_block& b : c->hblocks)
+    if ((b.sommerfeld_faces | b.outflow_faces) & 63)
+      return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: physical faces have no adjoint here");
+  return launch_rk4_step_ghosts_vjp(c, p, dt, u, nw, ubar_out, ubar_in, pbar, (hipStream_t)stream);
+}
+
+int z4c_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ubar || nvar <= 0) return fail(Z4C_EINVAL, "exchange_vjp: bad argument");
+  return launch_exchange_vjp(c, nvar, ubar, (hipStream_t)stream);
 }
 
 }  // extern "C"

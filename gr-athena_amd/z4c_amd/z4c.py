@@ -464,6 +464,24 @@ class Z4c:
               "rk4_step_vjp")
         return ubar, pbar.view(nw, len(_capi.PBAR))
 
+    def exchange_vjp(self, ubar, nvar=Z4C_NVAR, stream=None):
+        """ubar <- (the same-level exchange)^T ubar, in place (z4c_exchange_vjp)."""
+        check(self.L.z4c_exchange_vjp(self._ctx, int(nvar), _ptr(ubar), _stream(stream)),
+              "exchange_vjp")
+
+    def rk4_step_ghosts_vjp(self, dt, u, ubar_out, nw=1, params=None, stream=None):
+        """VJP of one step with the same-level exchange after every stage
+        (``rk4_step_fused(..., exchange=True)``) from u_n = ``u``; returns
+        (ubar_in, pbar) as ``rk4_step_vjp``."""
+        p = params if params is not None else self.params
+        ubar = torch.empty(nw * Z4C_NVAR * self.total_points, dtype=torch.float64, device=self.device)
+        pbar = torch.empty(nw * len(_capi.PBAR), dtype=torch.float64, device=self.device)
+        check(self.L.z4c_rk4_step_ghosts_vjp(self._ctx, C.byref(p), float(dt), _ptr(u), int(nw),
+                                             _ptr(ubar_out), _ptr(ubar), _ptr(pbar), 2,
+                                             _stream(stream)),
+              "rk4_step_ghosts_vjp")
+        return ubar, pbar.view(nw, len(_capi.PBAR))
+
     def hamiltonian_norm(self, u, params=None, stream=None):
         """sum over blocks and the history's cell loop of dx dy dz Ht^2, with
         Ht the Hamiltonian constraint of the right-hand side (waits)."""
