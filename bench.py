@@ -435,12 +435,18 @@ def exchange_step_cost(params, nb=8, n=16, steps=3):
 
     t_fused = timed(lambda: z.rk4_step_fused(dt, u, work, exchange=True))
     t_stages = timed(lambda: z.rk4_step(dt, u, un, k, exchange=True))
+    # the VJP of the same step (one cotangent), the exchange's transpose
+    # after every stage
+    w = torch.randn_like(u)
+    t_vjp = timed(lambda: z.rk4_step_ghosts_vjp(dt, u, w, 1))
     owned = len(gs) * n ** 3
     res = {"workload": f"gauge wave on a periodic [-0.5, 0.5]^3 split into {nb}^3 blocks of "
                        f"{n}^3 cells, exchange after every stage",
            "fused_ms_per_step": t_fused, "stage_sequence_ms_per_step": t_stages,
-           "fused_point_updates_per_s": owned / t_fused * 1e3}
-    log(f"exchange step: fused {t_fused:.2f} ms, stage sequence {t_stages:.2f} ms per step")
+           "fused_point_updates_per_s": owned / t_fused * 1e3,
+           "step_vjp_ms": t_vjp, "step_vjp_over_step": t_vjp / t_fused}
+    log(f"exchange step: fused {t_fused:.2f} ms, stage sequence {t_stages:.2f} ms per step; "
+        f"its VJP {t_vjp:.2f} ms ({t_vjp / t_fused:.1f}x)")
     z.close()
     return res
 
