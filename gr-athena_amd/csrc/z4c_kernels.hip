@@ -1634,6 +1634,256 @@ __global__ void __launch_bounds__(256) k_physical_coarse(PhysCArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Transpose of the ghost update across levels (z4c_exchange_vjp on several
+// spacings).  The update is linear in the fine state: u' = P Phi_c E_g R u
+// (restriction into the coarse buffers, the gather, the coarse level's
+// physical fill, prolongation; vertices none of them writes pass through),
+// so its transpose runs P^T, Phi_c^T, E_g^T, R^T on (u-bar, c-bar) with
+// c-bar = 0 to start.  Every step is a fixed-order gather (deterministic).
+// ---------------------------------------------------------------------------
+
+// P^T: each coarse vertex the prolongation reads gathers w * u-bar_f over the
+// fine vertices f the prolongation writes (the mask: each counted once,
+// however many boxes overlap there) whose stencil holds it.  Per axis a
+// fine vertex either injects coarse index c (even f = 2 (c - RF_H), weight
+// 1) or interpolates around base i (odd f = 2 (i - RF_H) + 1, reading
+// i - 2 .. i + 3 with the 6-point midpoint weights), and its weight is the
+// product over the axes.  The fine cotangents written are then zeroed.
+struct PrTArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const int* cl;   // coarse vertices (block, point)
+  int ncl;
+  const int* fl;   // fine vertices the prolongation writes (block, point)
+  int nfl;
+  const unsigned char* fmask;
+  int nvar;
+  double* ubar;
+  double* cbar;
+};
+
+__device__ __forceinline__ int prt_axis(int c, int nn, int* f, double* w) {
+  int m = 0;
+  const int fe = 2 * (c - RF_H);
+  if (fe >= 0 && fe < nn) {
+    f[m] = fe;
+    w[m] = 1.0;
+    ++m;
+  }
+#pragma unroll
+  for (int o = -2; o <= 3; ++o) {
+    const int i = c - o;  // base of the prolongated vertex
+    const int fo = 2 * (i - RF_H) + 1;
+    if (fo >= 0 && fo < nn) {
+      f[m] = fo;
+      w[m] = rf_lag(o >= 1 ? 3 - o : 2 + o);
+      ++m;
+    }
+  }
+  return m;
+}
+
+__global__ void __launch_bounds__(256) k_prolong_vjp(PrTArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.ncl) return;
+  const int b = a.cl[2 * t], cp = a.cl[2 * t + 1];
+  const DevBlock& B = a.blocks[b];
+  const RfBlock& R = a.cb[b];
+  const int cx = cp % R.cn1, cy = (cp / R.cn1) % R.cn2, cz = cp / (R.cn1 * R.cn2);
+  int fx[7], fy[7], fz[7];
+  double wx[7], wy[7], wz[7];
+  const int nx = prt_axis(cx, B.nn1, fx, wx), ny = prt_axis(cy, B.nn2, fy, wy),
+            nz = prt_axis(cz, B.nn3, fz, wz);
+  double acc[NVAR];
+  for (int n = 0; n < a.nvar; ++n) acc[n] = 0.0;
+  const unsigned char* mk = a.fmask + B.off;
+  const double* ub = a.ubar + B.off * a.nvar;
+  for (int qz = 0; qz < nz; ++qz)
+    for (int qy = 0; qy < ny; ++qy)
+      for (int qx = 0; qx < nx; ++qx) {
+        const long long p = fx[qx] + (long long)B.nn1 * (fy[qy] + (long long)B.nn2 * fz[qz]);
+        if (!mk[p]) continue;
+        const double w = (wz[qz] * wy[qy]) * wx[qx];
+        for (int n = 0; n < a.nvar; ++n) acc[n] += w * ub[(long long)n * B.npts + p];
+      }
+  double* cbo = a.cbar + R.coff * a.nvar + cp;
+  for (int n = 0; n < a.nvar; ++n) cbo[(long long)n * R.cnpts] = acc[n];
+}
+
+__global__ void __launch_bounds__(256) k_prolong_vjp_zero(PrTArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.nfl) return;
+  const DevBlock& B = a.blocks[a.fl[2 * t]];
+  double* u = a.ubar + B.off * a.nvar + a.fl[2 * t + 1];
+  for (int n = 0; n < a.nvar; ++n) u[(long long)n * B.npts] = 0.0;
+}
+
+// Transpose of one 5-point extrapolation line (the physical fill, fine or
+// coarse): the forward fills the lower face's ghosts outward, then the upper
+// face's (from the last cell-centred interior vertex on); the transpose runs
+// those assignments in reverse order, each v[g] = sum c_p v[g -+ (p + 1)]
+// becoming v-bar[g -+ (p + 1)] += c_p v-bar[g], then v-bar[g] = 0.
+__device__ __forceinline__ void phys_line_vjp(double* v, long long s, bool lower, bool upper,
+                                              int lo_v, int ie, int ng) {
+  const double c[5] = {5., -10., 10., -5., 1.};
+  if (upper) {
+    for (int i = ie + ng; i >= ie + 1; --i) {
+      const double g = v[i * s];
+      v[i * s] = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) v[(i - p - 1) * s] += c[p] * g;
+    }
+  }
+  if (lower) {
+    for (int i = lo_v - ng; i <= lo_v - 1; ++i) {
+      const double g = v[i * s];
+      v[i * s] = 0.0;
+#pragma unroll
+      for (int p = 0; p < 5; ++p) v[(i + p + 1) * s] += c[p] * g;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_physical_vjp(PhysArgs a) {
+  const long long Lg = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (Lg >= a.nlines) return;
+  const int n = (int)(Lg / a.nlines1);
+  const long long L = Lg - (long long)n * a.nlines1;
+  int lo = 0, hi = a.nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.lines[mid].start <= L) lo = mid; else hi = mid - 1;
+  }
+  const PhysLine E = a.lines[lo];
+  const DevBlock B = a.blocks[E.block];
+  const int r = (int)(L - E.start);
+  const int t1 = E.lo1 + r / E.n0, t0 = E.lo0 + r % E.n0;
+  const long long str[3] = {1, B.nn1, (long long)B.nn1 * B.nn2};
+  const int ax = a.axis;
+  const int o0 = ax == 0 ? 1 : 0, o1 = ax == 2 ? 1 : 2;
+  double* v = a.u + B.off * a.nvar + (long long)n * B.npts + t0 * str[o0] + t1 * str[o1];
+  const int lo_v = ax == 0 ? B.il : (ax == 1 ? B.jl : B.kl);
+  const int ie = (ax == 0 ? B.iu : (ax == 1 ? B.ju : B.ku)) - 1;
+  phys_line_vjp(v, str[ax], B.pfaces & (1 << (2 * ax)), B.pfaces & (2 << (2 * ax)), lo_v, ie, NG);
+}
+
+__global__ void __launch_bounds__(256) k_physical_coarse_vjp(PhysCArgs a) {
+  const long long Lg = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (Lg >= a.nlines) return;
+  const int n = (int)(Lg / a.nlines1);
+  const long long L = Lg - (long long)n * a.nlines1;
+  int lo = 0, hi = a.nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.lines[mid].start <= L) lo = mid; else hi = mid - 1;
+  }
+  const PhysLine E = a.lines[lo];
+  const RfBlock R = a.cb[E.block];
+  const int pf = a.blocks[E.block].pfaces;
+  const int r = (int)(L - E.start);
+  const int t1 = E.lo1 + r / E.n0, t0 = E.lo0 + r % E.n0;
+  const long long str[3] = {1, R.cn1, (long long)R.cn1 * R.cn2};
+  const int ax = a.axis;
+  const int o0 = ax == 0 ? 1 : 0, o1 = ax == 2 ? 1 : 2;
+  double* v = a.coarse + R.coff * a.nvar + (long long)n * R.cnpts + t0 * str[o0] + t1 * str[o1];
+  const int cni = ax == 0 ? R.cni : (ax == 1 ? R.cnj : R.cnk);
+  phys_line_vjp(v, str[ax], pf & (1 << (2 * ax)), pf & (2 << (2 * ax)), RF_NCG,
+                RF_NCG + cni - 1, RF_NCG);
+}
+
+// E_g^T over fine and coarse references (a reference 2 b + A: block b's fine
+// array, A = 0, or coarse buffer, A = 1): a source receives sum inv_d
+// ubar_d over the destinations reading it, plus its own unless it is a
+// destination; a destination that is no source receives zero.
+struct MltArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const int* dst;
+  const double* inv;
+  const int* v;
+  const int* flag;
+  const int* voff;
+  const int* vd;
+  const int* z;
+  int nv, nz, nvar;
+  double* ubar;
+  double* cbar;
+  double* tmp;  // nvar * nv
+};
+
+__device__ __forceinline__ double* mlt_ptr(const MltArgs& a, int blkarr, int p, long long& fs) {
+  const int b = blkarr >> 1;
+  if (blkarr & 1) {
+    const RfBlock& R = a.cb[b];
+    fs = R.cnpts;
+    return a.cbar + R.coff * a.nvar + p;
+  }
+  const DevBlock& B = a.blocks[b];
+  fs = B.npts;
+  return a.ubar + B.off * a.nvar + p;
+}
+
+__global__ void __launch_bounds__(256) k_mlx_vjp_gather(MltArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= a.nv) return;
+  long long fs;
+  const double* vp = mlt_ptr(a, a.v[2 * t], a.v[2 * t + 1], fs);
+  const int e0 = a.voff[t], e1 = a.voff[t + 1];
+  for (int n = 0; n < a.nvar; ++n) {
+    double acc = a.flag[t] ? 0.0 : vp[n * fs];
+    for (int e = e0; e < e1; ++e) {
+      const int d = a.vd[e];
+      long long dfs;
+      const double* dp = mlt_ptr(a, a.dst[2 * d], a.dst[2 * d + 1], dfs);
+      acc += a.inv[d] * dp[n * dfs];
+    }
+    a.tmp[(long long)n * a.nv + t] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlx_vjp_scatter(MltArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  long long fs;
+  if (t < a.nv) {
+    double* vp = mlt_ptr(a, a.v[2 * t], a.v[2 * t + 1], fs);
+    for (int n = 0; n < a.nvar; ++n) vp[n * fs] = a.tmp[(long long)n * a.nv + t];
+  } else if (t < a.nv + a.nz) {
+    const int d = a.z[t - a.nv];
+    double* dp = mlt_ptr(a, a.dst[2 * d], a.dst[2 * d + 1], fs);
+    for (int n = 0; n < a.nvar; ++n) dp[n * fs] = 0.0;
+  }
+}
+
+// R^T: the restriction injects the fine vertex coincident with each coarse
+// interior vertex, so that fine vertex receives the coarse cotangent.
+struct RestrictTArgs {
+  const DevBlock* blocks;
+  const RfBlock* cb;
+  const RfChunk* chunks;
+  int nvar;
+  double* ubar;
+  const double* cbar;
+};
+
+__global__ void __launch_bounds__(256) k_restrict_vjp(RestrictTArgs a) {
+  const RfChunk ch = a.chunks[blockIdx.x];
+  const RfBlock R = a.cb[ch.ent];
+  const int t = ch.start + (int)threadIdx.x;
+  if (t >= R.cni * R.cnj * R.cnk) return;
+  const DevBlock& B = a.blocks[ch.ent];
+  const int ci = RF_NCG + t % R.cni, cj = RF_NCG + (t / R.cni) % R.cnj,
+            ck = RF_NCG + t / (R.cni * R.cnj);
+  const int i = rf_fine_of(ci, RF_NCG, RF_NCG + R.cni - 1, B.il, B.iu);
+  const int j = rf_fine_of(cj, RF_NCG, RF_NCG + R.cnj - 1, B.jl, B.ju);
+  const int k = rf_fine_of(ck, RF_NCG, RF_NCG + R.cnk - 1, B.kl, B.ku);
+  const long long p = i + (long long)B.nn1 * (j + (long long)B.nn2 * k);
+  const long long cp = ci + (long long)R.cn1 * (cj + (long long)R.cn2 * ck);
+  double* ub = a.ubar + B.off * a.nvar + p;
+  const double* cb = a.cbar + R.coff * a.nvar + cp;
+  for (int n = 0; n < a.nvar; ++n) ub[(long long)n * B.npts] += cb[(long long)n * R.cnpts];
+}
+
+// ---------------------------------------------------------------------------
 // Z4c::Z4cWeyl (src/z4c/calculate_weyl_scalars.cpp:44-628): Re / Im psi4 at
 // interior vertices, one thread per vertex, in the reference's operation
 // order: physical metric derivatives by the chain rule from the prepared
@@ -2311,12 +2561,43 @@ struct z4c_ctx {
   PrGroup* ml_pgroups = nullptr;
   int ml_npgroups = 0;
   int ml_prlds = 0;             // largest staged coarse box, doubles
+  int ml_nent = 0;              // prolongation entries (ml_pent)
   PhysLine* mlp_lines[3] = {};  // the coarse level's physical fill, per axis
   int mlp_nent[3] = {};
   long long mlp_nlines1[3] = {};
   std::vector<std::vector<z4c_mesh::Nbr>> ml_nbrs;  // per block, for inspection
   std::vector<int> ml_level;
 
+  // transpose of the update across levels (built on its first use)
+  bool mlt_built = false;
+  int* mlt_v = nullptr;      // source references (2 ints each)
+  int* mlt_flag = nullptr;   // 1 if the source is also a destination
+  int* mlt_off = nullptr;    // CSR offsets into mlt_d
+  int* mlt_d = nullptr;      // destinations reading each source
+  int* mlt_z = nullptr;      // destinations that are no source
+  int mlt_nv = 0, mlt_nz = 0;
+  double* mlt_tmp = nullptr;            // NVAR * mlt_nv
+  unsigned char* ml_fmask = nullptr;    // fine vertices the prolongation writes (per point)
+  int* ml_fl = nullptr;                 // those vertices (block, point)
+  int ml_nfl = 0;
+  int* ml_cl = nullptr;                 // coarse vertices the prolongation reads (block, point)
+  int ml_ncl = 0;
+  double* ml_cbar = nullptr;            // coarse cotangents, NVAR * rf_total
+  void drop_mlt() {
+    int** q[7] = {&mlt_v, &mlt_flag, &mlt_off, &mlt_d, &mlt_z, &ml_fl, &ml_cl};
+    for (int** x : q) {
+      hipFree(*x);
+      *x = nullptr;
+    }
+    hipFree(mlt_tmp);
+    hipFree(ml_fmask);
+    hipFree(ml_cbar);
+    mlt_tmp = nullptr;
+    ml_fmask = nullptr;
+    ml_cbar = nullptr;
+    mlt_nv = mlt_nz = ml_nfl = ml_ncl = 0;
+    mlt_built = false;
+  }
   void drop_ml() {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
@@ -2339,7 +2620,8 @@ struct z4c_ctx {
     ml_inv = ml_coarse = ml_tmp = nullptr;
     ml_pent = nullptr;
     ml_pgroups = nullptr;
-    ml_ndst = ml_nint = ml_npgroups = ml_prlds = 0;
+    ml_ndst = ml_nint = ml_npgroups = ml_prlds = ml_nent = 0;
+    drop_mlt();
     ml = false;
     ml_nbrs.clear();
     ml_level.clear();
@@ -2848,10 +3130,165 @@ int launch_exchange(z4c_ctx* c, int nvar, double* u, hipStream_t s) {
   return Z4C_OK;
 }
 
-// u-bar <- (the same-level exchange)^T u-bar, nvar fields
+bool rf_read_range(const RfEntry& E, int a, int& lo, int& hi);
+
+// The transpose data of the update across levels, built on first use from
+// the device plan: the transposed gather lists (sources in first-use order,
+// each with its destinations in destination order), the fine vertices the
+// prolongation writes (mask and list) and the coarse vertices it reads.
+int build_ml_transpose(z4c_ctx* c) {
+  if (c->mlt_built) return Z4C_OK;
+  const int nd = c->ml_ndst;
+  std::vector<long long> off(nd + 1, 0);
+  std::vector<int> dst(2 * (size_t)nd);
+  if (nd) {
+    HIPCALL(hipMemcpy(off.data(), c->ml_off, sizeof(long long) * (nd + 1), hipMemcpyDeviceToHost));
+    HIPCALL(hipMemcpy(dst.data(), c->ml_dst, sizeof(int) * 2 * nd, hipMemcpyDeviceToHost));
+  }
+  const long long ns = off[nd];
+  if (ns >= (long long)std::numeric_limits<int>::max())
+    return fail(Z4C_EINVAL, "exchange_vjp: plan too large");
+  std::vector<int> src(2 * ns);
+  if (ns) HIPCALL(hipMemcpy(src.data(), c->ml_src, sizeof(int) * 2 * ns, hipMemcpyDeviceToHost));
+  std::vector<RfEntry> ent(c->ml_nent);
+  if (c->ml_nent)
+    HIPCALL(hipMemcpy(ent.data(), c->ml_pent, sizeof(RfEntry) * ent.size(), hipMemcpyDeviceToHost));
+  const long long nf = c->total_pts, ncp = c->rf_total;
+  auto slot = [&](int blkarr, int p) -> long long {
+    const int b = blkarr >> 1;
+    return (blkarr & 1) ? nf + c->hrf[b].coff + p : c->hdev[b].off + p;
+  };
+  std::vector<int> id(nf + ncp, -1), tv, cnt;
+  for (long long e = 0; e < ns; ++e) {
+    const long long k = slot(src[2 * e], src[2 * e + 1]);
+    if (id[k] < 0) {
+      id[k] = (int)cnt.size();
+      tv.push_back(src[2 * e]);
+      tv.push_back(src[2 * e + 1]);
+      cnt.push_back(0);
+    }
+    ++cnt[id[k]];
+  }
+  const int nv = (int)cnt.size();
+  std::vector<int> toff(nv + 1, 0);
+  for (int q = 0; q < nv; ++q) toff[q + 1] = toff[q] + cnt[q];
+  std::vector<int> td(ns), pos(toff.begin(), toff.end() - 1);
+  for (int d = 0; d < nd; ++d)
+    for (long long e = off[d]; e < off[d + 1]; ++e) td[pos[id[slot(src[2 * e], src[2 * e + 1])]]++] = d;
+  std::vector<int> tflag(nv, 0), tz;
+  for (int d = 0; d < nd; ++d) {
+    const int q = id[slot(dst[2 * d], dst[2 * d + 1])];
+    if (q < 0)
+      tz.push_back(d);
+    else
+      tflag[q] = 1;
+  }
+  std::vector<unsigned char> mask(nf, 0), cmark(ncp, 0);
+  std::vector<int> fl, cl;
+  for (const RfEntry& E : ent) {
+    const DevBlock& B = c->hdev[E.blk];
+    const RfBlock& R = c->hrf[E.blk];
+    for (int m = 0; m < 8; ++m) {
+      int c0[3], n[3];
+      for (int d = 0; d < 3; ++d) rf_case_range(E, d, (m >> d) & 1, c0[d], n[d]);
+      if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) continue;
+      for (int ck = c0[2]; ck < c0[2] + n[2]; ++ck)
+        for (int cj = c0[1]; cj < c0[1] + n[1]; ++cj)
+          for (int ci = c0[0]; ci < c0[0] + n[0]; ++ci) {
+            const int fi = 2 * (ci - RF_H) + (m & 1), fj = 2 * (cj - RF_H) + ((m >> 1) & 1),
+                      fk = 2 * (ck - RF_H) + (m >> 2);
+            const int p = fi + B.nn1 * (fj + B.nn2 * fk);
+            if (!mask[B.off + p]) {
+              mask[B.off + p] = 1;
+              fl.push_back(E.blk);
+              fl.push_back(p);
+            }
+          }
+    }
+    int lo[3], hi[3];
+    bool any = true;
+    for (int d = 0; d < 3; ++d) any = any && rf_read_range(E, d, lo[d], hi[d]);
+    if (!any) continue;
+    for (int z = lo[2]; z <= hi[2]; ++z)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          const int cp = x + R.cn1 * (y + R.cn2 * z);
+          if (!cmark[R.coff + cp]) {
+            cmark[R.coff + cp] = 1;
+            cl.push_back(E.blk);
+            cl.push_back(cp);
+          }
+        }
+  }
+  auto up = [&](auto*& d, const auto& h) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(d)>>;
+    if (h.empty()) return true;
+    if (hipMalloc(&d, sizeof(T) * h.size()) != hipSuccess) return false;
+    return hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) == hipSuccess;
+  };
+  bool ok = up(c->mlt_v, tv) && up(c->mlt_flag, tflag) && up(c->mlt_off, toff) &&
+            up(c->mlt_d, td) && up(c->mlt_z, tz) && up(c->ml_fmask, mask) && up(c->ml_fl, fl) &&
+            up(c->ml_cl, cl);
+  ok = ok && (nv == 0 || hipMalloc(&c->mlt_tmp, sizeof(double) * NVAR * (size_t)nv) == hipSuccess);
+  ok = ok && hipMalloc(&c->ml_cbar, sizeof(double) * NVAR * (size_t)ncp) == hipSuccess;
+  if (!ok) {
+    c->drop_mlt();
+    return fail(Z4C_ENOMEM, "exchange_vjp: allocation failed");
+  }
+  c->mlt_nv = nv;
+  c->mlt_nz = (int)tz.size();
+  c->ml_nfl = (int)(fl.size() / 2);
+  c->ml_ncl = (int)(cl.size() / 2);
+  c->mlt_built = true;
+  return Z4C_OK;
+}
+
+// u-bar <- (the update across levels)^T u-bar, nvar fields: P^T, Phi_c^T,
+// E_g^T, R^T on (u-bar, c-bar), c-bar = 0 to start
+int launch_ml_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
+  int rc = build_ml_transpose(c);
+  if (rc) return rc;
+  rc = fill(c, c->ml_cbar, 0.0, (long long)nvar * c->rf_total, s);
+  if (rc) return rc;
+  PrTArgs pa{c->d_blocks, c->d_rf, c->ml_cl, c->ml_ncl, c->ml_fl, c->ml_nfl, c->ml_fmask,
+             nvar, ubar, c->ml_cbar};
+  if (c->ml_ncl) {
+    hipLaunchKernelGGL(k_prolong_vjp, dim3((c->ml_ncl + 255) / 256), dim3(256), 0, s, pa);
+    HIPCALL(hipGetLastError());
+  }
+  if (c->ml_nfl) {
+    hipLaunchKernelGGL(k_prolong_vjp_zero, dim3((c->ml_nfl + 255) / 256), dim3(256), 0, s, pa);
+    HIPCALL(hipGetLastError());
+  }
+  for (int ax = 2; ax >= 0; --ax) {
+    if (!c->mlp_nent[ax]) continue;
+    PhysCArgs pc{c->d_blocks, c->d_rf, c->mlp_lines[ax], c->mlp_nent[ax], c->mlp_nlines1[ax],
+                 c->mlp_nlines1[ax] * nvar, ax, nvar, c->ml_cbar};
+    hipLaunchKernelGGL(k_physical_coarse_vjp, dim3((unsigned)((pc.nlines + 255) / 256)), dim3(256),
+                       0, s, pc);
+    HIPCALL(hipGetLastError());
+  }
+  MltArgs ma{c->d_blocks, c->d_rf, c->ml_dst, c->ml_inv, c->mlt_v, c->mlt_flag, c->mlt_off,
+             c->mlt_d, c->mlt_z, c->mlt_nv, c->mlt_nz, nvar, ubar, c->ml_cbar, c->mlt_tmp};
+  if (c->mlt_nv) {
+    hipLaunchKernelGGL(k_mlx_vjp_gather, dim3((c->mlt_nv + 255) / 256), dim3(256), 0, s, ma);
+    HIPCALL(hipGetLastError());
+  }
+  if (c->mlt_nv + c->mlt_nz) {
+    hipLaunchKernelGGL(k_mlx_vjp_scatter, dim3((c->mlt_nv + c->mlt_nz + 255) / 256), dim3(256), 0,
+                       s, ma);
+    HIPCALL(hipGetLastError());
+  }
+  RestrictTArgs ra{c->d_blocks, c->d_rf, c->d_rchunks, nvar, ubar, c->ml_cbar};
+  hipLaunchKernelGGL(k_restrict_vjp, dim3(c->n_rchunks), dim3(256), 0, s, ra);
+  HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+// u-bar <- (the exchange set by z4c_set_exchange)^T u-bar, nvar fields
 int launch_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
-  if (c->ml) return fail(Z4C_EINVAL, "exchange_vjp: the update across levels has no transpose here");
   if (nvar > NVAR) return fail(Z4C_EINVAL, "exchange_vjp: at most 22 fields");
+  if (c->ml) return launch_ml_exchange_vjp(c, nvar, ubar, s);
   if (!c->ext_nv && !c->ext_nz) return Z4C_OK;
   ExTArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ext_v, c->ext_flag, c->ext_off, c->ext_d,
             c->ext_z, c->ext_nv, c->ext_nz, nvar, ubar, c->ext_tmp};
@@ -2862,6 +3299,19 @@ int launch_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
   hipLaunchKernelGGL(k_exchange_vjp_scatter, dim3((c->ext_nv + c->ext_nz + 255) / 256), dim3(256),
                      0, s, a);
   HIPCALL(hipGetLastError());
+  return Z4C_OK;
+}
+
+// u-bar <- (the physical fill)^T u-bar: the axes in reverse order
+int launch_physical_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
+  for (int ax = 2; ax >= 0; --ax) {
+    if (!c->ph_nent[ax]) continue;
+    PhysArgs a{c->d_blocks, c->ph_lines[ax], c->ph_nent[ax], c->ph_nlines1[ax],
+               c->ph_nlines1[ax] * nvar, ax, nvar, ubar};
+    hipLaunchKernelGGL(k_physical_vjp, dim3((unsigned)((a.nlines + 255) / 256)), dim3(256), 0, s,
+                       a);
+    HIPCALL(hipGetLastError());
+  }
   return Z4C_OK;
 }
 
@@ -3027,15 +3477,16 @@ int launch_rk4_step_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double
   return Z4C_OK;
 }
 
-// Adjoint of z4c_rk4_step_ghosts with the same-level exchange after every
-// stage (ghosts = 2): the forward sweep forms y_s = u_n + a_s dt k_{s-1}
-// (ghosts at u_n), E(y_s) (kept as AlgConstr's linearisation point) and
-// u_s = AlgConstr(E(y_s)), with k_s = rhs(u_s); the reverse sweep carries
-// each stage cotangent through AlgConstr's adjoint, E's transpose and the
-// stage sums.
+// Adjoint of z4c_rk4_step_ghosts with the exchange after every stage
+// (ghosts = 2, + 4 for the physical fill): the forward sweep forms
+// y_s = u_n + a_s dt k_{s-1} (ghosts at u_n), its update G(y_s) (the
+// exchange, on several spacings the update across levels, then the fill;
+// kept as AlgConstr's linearisation point) and u_s = AlgConstr(G(y_s)), with
+// k_s = rhs(u_s); the reverse sweep carries each stage cotangent through
+// AlgConstr's adjoint, G's transpose and the stage sums.
 int launch_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const double* u,
                                int nw, const double* ubar_out, double* ubar_in, double* pbar,
-                               hipStream_t s) {
+                               int ghosts, hipStream_t s) {
   const long long reg = c->total_pts * NVAR;
   const long long k_n_before = c->adj_k_n;
   int rc = ensure_buf(c->adj_k, c->adj_k_n, 4 * reg, "rk4_step_ghosts_vjp");
@@ -3065,6 +3516,7 @@ int launch_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const
       const double wa[1] = {aw[st]};
       rc = launch_bt(c, NVAR, pre(st), u, 1, kk, wa, dt, s);
       if (!rc) rc = launch_exchange(c, NVAR, pre(st), s);
+      if (!rc && (ghosts & 4)) rc = launch_physical(c, NVAR, pre(st), s);
       if (rc) return rc;
       HIPCALL(hipMemcpyAsync(stage_buf(st), pre(st), sizeof(double) * reg, hipMemcpyDeviceToDevice, s));
       rc = launch_alg(c, p, stage_buf(st), nullptr, 0, s);
@@ -3078,6 +3530,7 @@ int launch_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const
     const double* kk[4] = {K[0], K[1], K[2], K[3]};
     rc = launch_bt(c, NVAR, pre(5), u, 4, kk, bw, dt, s);
     if (!rc) rc = launch_exchange(c, NVAR, pre(5), s);
+    if (!rc && (ghosts & 4)) rc = launch_physical(c, NVAR, pre(5), s);
     if (rc) return rc;
   }
   AdjStepArgs a{};
@@ -3095,13 +3548,16 @@ int launch_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const
   a.force_reg = p->force_regularization;
   a.ybar_stage = usbar;
   const dim3 g(grid_for(a.L.ntiles), nw);
-  // cotangent of the exchanged, constrained sums: AlgConstr's adjoint, E^T
+  // cotangent of the exchanged, filled, constrained sums: AlgConstr's
+  // adjoint, then the physical fill's and the exchange's transposes
   auto through_update = [&](int st) -> int {
     a.pre = pre(st);
     hipLaunchKernelGGL(k_adj_step_ex<1>, g, dim3(TILE), 0, s, a);
     HIPCALL(hipGetLastError());
     for (int q = 0; q < nw; ++q) {
-      const int r = launch_exchange_vjp(c, NVAR, usbar + (long long)q * reg, s);
+      double* ub = usbar + (long long)q * reg;
+      int r = (ghosts & 4) ? launch_physical_vjp(c, NVAR, ub, s) : Z4C_OK;
+      if (!r) r = launch_exchange_vjp(c, NVAR, ub, s);
       if (r) return r;
     }
     return Z4C_OK;
@@ -3968,6 +4424,7 @@ int build_ml_exchange(z4c_ctx* c, const double* lo, const double* hi, const int*
   c->ml_nint = (int)P.n_interior;
   c->ml_npgroups = (int)grp.size();
   c->ml_prlds = prlds;
+  c->ml_nent = (int)ent.size();
   c->ml_nbrs = t.nbrs;
   c->ml_level = t.level;
   c->ml = true;
@@ -4689,15 +5146,17 @@ int z4c_rk4_step_ghosts_vjp(z4c_ctx* c, const z4c_params* p, double dt, const do
     return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: null argument");
   rc = check_vjp_args(c, p, nw, "rk4_step_ghosts_vjp");
   if (rc) return rc;
-  if (ghosts != 2)
+  if (ghosts != 2 && ghosts != 6)
     return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: ghosts must be 2 (the exchange set by "
-                            "z4c_set_exchange)");
-  if (c->ml)
-    return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: the update across levels has no transpose here");
-  for (const z4c_block& b : c->hblocks)
-    if ((b.sommerfeld_faces | b.outflow_faces) & 63)
-      return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: physical faces have no adjoint here");
-  return launch_rk4_step_ghosts_vjp(c, p, dt, u, nw, ubar_out, ubar_in, pbar, (hipStream_t)stream);
+                            "z4c_set_exchange) or 6 (and the physical fill)");
+  for (const z4c_block& b : c->hblocks) {
+    if (b.sommerfeld_faces & 63)
+      return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: Sommerfeld faces have no adjoint here");
+    if ((b.outflow_faces & 63) && !(ghosts & 4))
+      return fail(Z4C_EINVAL, "rk4_step_ghosts_vjp: outflow faces need ghosts = 6");
+  }
+  return launch_rk4_step_ghosts_vjp(c, p, dt, u, nw, ubar_out, ubar_in, pbar, ghosts,
+                                    (hipStream_t)stream);
 }
 
 int z4c_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, void* stream) {
@@ -4705,6 +5164,13 @@ int z4c_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, void* stream) {
   if (rc) return rc;
   if (!ubar || nvar <= 0) return fail(Z4C_EINVAL, "exchange_vjp: bad argument");
   return launch_exchange_vjp(c, nvar, ubar, (hipStream_t)stream);
+}
+
+int z4c_physical_bc_vjp(z4c_ctx* c, int nvar, double* ubar, void* stream) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ubar || nvar <= 0) return fail(Z4C_EINVAL, "physical_bc_vjp: bad argument");
+  return launch_physical_vjp(c, nvar, ubar, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -26,7 +26,7 @@ EXPORTS = [
     "z4c_set_timing", "z4c_get_timing", "z4c_history", "z4c_set_exchange", "z4c_exchange",
     "z4c_autotune", "z4c_stencil_threads", "z4c_set_stencil_threads", "z4c_rk4_step_graph",
     "z4c_physical_bc", "z4c_rhs_vjp", "z4c_rk4_step_vjp", "z4c_hamiltonian_norm",
-    "z4c_exchange_vjp", "z4c_rk4_step_ghosts_vjp",
+    "z4c_exchange_vjp", "z4c_rk4_step_ghosts_vjp", "z4c_physical_bc_vjp",
     "z4c_hamiltonian_norm_vjp", "z4c_weyl", "z4c_coarse_layout", "z4c_restrict",
     "z4c_prolongation_range", "z4c_set_prolongation", "z4c_prolongate", "z4c_rk4_step_ghosts",
     "z4c_ls_tableau_get", "z4c_ls_stage", "z4c_ls_step", "z4c_history_noalloc",
@@ -149,6 +149,7 @@ def load(path: str | None = None):
         "z4c_set_exchange": [vp, C.POINTER(d), C.POINTER(d), C.POINTER(i)],
         "z4c_exchange": [vp, i, vp, vp],
         "z4c_exchange_vjp": [vp, i, vp, vp],
+        "z4c_physical_bc_vjp": [vp, i, vp, vp],
         "z4c_autotune": [vp, pp, vp, i, vp],
         "z4c_stencil_threads": [vp, i, C.POINTER(i)],
         "z4c_set_stencil_threads": [vp, i, i],

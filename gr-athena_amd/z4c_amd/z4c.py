@@ -465,20 +465,27 @@ class Z4c:
         return ubar, pbar.view(nw, len(_capi.PBAR))
 
     def exchange_vjp(self, ubar, nvar=Z4C_NVAR, stream=None):
-        """ubar <- (the same-level exchange)^T ubar, in place (z4c_exchange_vjp)."""
+        """ubar <- (the exchange)^T ubar, in place (z4c_exchange_vjp; one
+        spacing or across levels)."""
         check(self.L.z4c_exchange_vjp(self._ctx, int(nvar), _ptr(ubar), _stream(stream)),
               "exchange_vjp")
 
-    def rk4_step_ghosts_vjp(self, dt, u, ubar_out, nw=1, params=None, stream=None):
-        """VJP of one step with the same-level exchange after every stage
-        (``rk4_step_fused(..., exchange=True)``) from u_n = ``u``; returns
-        (ubar_in, pbar) as ``rk4_step_vjp``."""
+    def physical_bc_vjp(self, ubar, nvar=Z4C_NVAR, stream=None):
+        """ubar <- (the physical fill)^T ubar, in place (z4c_physical_bc_vjp)."""
+        check(self.L.z4c_physical_bc_vjp(self._ctx, int(nvar), _ptr(ubar), _stream(stream)),
+              "physical_bc_vjp")
+
+    def rk4_step_ghosts_vjp(self, dt, u, ubar_out, nw=1, params=None, stream=None,
+                            physical=False):
+        """VJP of one step with the exchange after every stage
+        (``rk4_step_fused(..., exchange=True, physical=physical)``) from
+        u_n = ``u``; returns (ubar_in, pbar) as ``rk4_step_vjp``."""
         p = params if params is not None else self.params
         ubar = torch.empty(nw * Z4C_NVAR * self.total_points, dtype=torch.float64, device=self.device)
         pbar = torch.empty(nw * len(_capi.PBAR), dtype=torch.float64, device=self.device)
         check(self.L.z4c_rk4_step_ghosts_vjp(self._ctx, C.byref(p), float(dt), _ptr(u), int(nw),
-                                             _ptr(ubar_out), _ptr(ubar), _ptr(pbar), 2,
-                                             _stream(stream)),
+                                             _ptr(ubar_out), _ptr(ubar), _ptr(pbar),
+                                             6 if physical else 2, _stream(stream)),
               "rk4_step_ghosts_vjp")
         return ubar, pbar.view(nw, len(_capi.PBAR))
 

@@ -216,16 +216,26 @@ int z4c_periodic_fill(z4c_ctx* ctx, int nvar, double* u, void* stream);
 int z4c_set_exchange(z4c_ctx* ctx, const double* lo, const double* hi, const int* periodic);
 int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
 
-/* Transpose of z4c_exchange on one block set with a single spacing (no
-   counterpart in the reference, which has no derivatives): ubar <- E^T ubar
-   in place, nvar <= 22 fields, where E is the same-level update (each
-   destination vertex set to the mean of its lattice point's interior
-   copies, every other vertex unchanged).  A source vertex receives the
-   cotangents of the destinations that read it (times 1/multiplicity), plus
-   its own unless it is a destination itself; a ghost destination receives
-   zero.  Deterministic (a gather in a fixed order).  Z4C_EINVAL on a block
-   list of several spacings. */
+/* Transpose of z4c_exchange (no counterpart in the reference, which has no
+   derivatives): ubar <- G^T ubar in place, nvar <= 22 fields.  On one
+   spacing G is the same-level update (each destination vertex set to the
+   mean of its lattice point's interior copies, every other vertex
+   unchanged): a source vertex receives the cotangents of the destinations
+   that read it (times 1/multiplicity), plus its own unless it is a
+   destination itself; a ghost destination receives zero.  On several
+   spacings G is the update across levels, P Phi_c E_g R (restriction, the
+   gather, the coarse level's physical fill, prolongation), and the
+   transpose runs their transposes in reverse order on ubar and a coarse
+   cotangent; the prolongation's transpose counts each fine vertex it
+   writes once however many boxes overlap there.  Deterministic (gathers in
+   a fixed order).  The transpose data of the update across levels are
+   built on the first call (device memory: about 22 doubles per source
+   vertex and per coarse vertex). */
 int z4c_exchange_vjp(z4c_ctx* ctx, int nvar, double* ubar, void* stream);
+
+/* Transpose of z4c_physical_bc (no counterpart in the reference): the
+   5-point extrapolation lines in reverse order, axes z, y, x. */
+int z4c_physical_bc_vjp(z4c_ctx* ctx, int nvar, double* ubar, void* stream);
 
 /* Physical ghost fill (PHY_BVAL of GR_Z4c: CommRegistry::ApplyPhysicalBCs,
    src/comm/comm_registry.cpp:516-527 -> comm::ApplyPhysicalBCs,
@@ -503,15 +513,16 @@ int z4c_rhs_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, int nw, cons
 int z4c_rk4_step_vjp(z4c_ctx* ctx, const z4c_params* p, double dt, const double* u, int nw,
                      const double* ubar_out, double* ubar_in, double* pbar, void* stream);
 
-/* VJP of one z4c_rk4_step_ghosts with ghosts = 2 (the same-level exchange
-   after every stage, then AlgConstr), no counterpart in the reference.
-   Arguments as z4c_rk4_step_vjp.  The forward sweep keeps each stage's
-   exchanged sum as AlgConstr's linearisation point; the reverse sweep
-   applies AlgConstr's adjoint, z4c_exchange_vjp's transpose and the stage
-   sums' adjoint at every stage.  Z4C_EINVAL for ghosts other than 2, block
-   lists of several spacings, faces with a physical boundary condition, or
+/* VJP of one z4c_rk4_step_ghosts with ghosts = 2 (the exchange after every
+   stage, then AlgConstr) or 6 (the exchange, the physical fill, then
+   AlgConstr), on one spacing or across levels; no counterpart in the
+   reference.  Arguments as z4c_rk4_step_vjp.  The forward sweep keeps each
+   stage's updated sum as AlgConstr's linearisation point; the reverse sweep
+   applies AlgConstr's adjoint, z4c_physical_bc_vjp's and z4c_exchange_vjp's
+   transposes and the stage sums' adjoint at every stage.  Z4C_EINVAL for
+   other ghosts values, Sommerfeld faces, outflow faces with ghosts = 2, or
    cowling.  Scratch: 11 + 3 nw registers of 22 * elems_per_var doubles on
-   top of z4c_rhs_vjp's. */
+   top of z4c_rhs_vjp's and z4c_exchange_vjp's. */
 int z4c_rk4_step_ghosts_vjp(z4c_ctx* ctx, const z4c_params* p, double dt, const double* u,
                             int nw, const double* ubar_out, double* ubar_in, double* pbar,
                             int ghosts, void* stream);

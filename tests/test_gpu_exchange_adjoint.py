@@ -66,18 +66,89 @@ def test_exchange_vjp_dot_product(name, nvar):
     z.close()
 
 
-def test_exchange_vjp_rejects_levels():
-    """The update across levels (prolongation included) has no transpose."""
-    spec = refined_blocks(2, 8, [-0.5] * 3, [0.5] * 3, lambda c, L: bool(np.all(c < 0.0)), 2,
-                          (True,) * 3)
+ML_CASES = {
+    "periodic_corner": dict(nrb=4, lo=-2.0, hi=2.0, periodic=(True,) * 3, levels=2,
+                            refine=lambda c, L: bool(np.all(c < -1.0))),
+    "open_centre": dict(nrb=4, lo=-2.0, hi=2.0, periodic=(False,) * 3, levels=2,
+                        refine=lambda c, L: bool(np.all(np.abs(c) < 1.0))),
+    "open_3level": dict(nrb=4, lo=-2.0, hi=2.0, periodic=(False,) * 3, levels=3,
+                        refine=lambda c, L: bool(np.all(np.abs(c) < (1.0 if L == 0 else 0.5)))),
+    "open_corner": dict(nrb=4, lo=-2.0, hi=2.0, periodic=(False,) * 3, levels=2,
+                        refine=lambda c, L: bool(np.all(c < -1.0))),
+    "periodic_2level": dict(nrb=2, lo=-0.5, hi=0.5, periodic=(True,) * 3, levels=2,
+                            refine=lambda c, L: bool(np.all(c < 0.0))),
+}
+
+
+def _ml_geoms(case, nx=8):
+    """The block tree of an ML case; every face on an open domain boundary
+    carries the outflow fill (the update across levels needs it there)."""
+    c = ML_CASES[case]
+    spec = refined_blocks(c["nrb"], nx, [c["lo"]] * 3, [c["hi"]] * 3, c["refine"], c["levels"],
+                          c["periodic"])
     geoms = [MeshBlockGeom(tuple(n), tuple(dx), tuple(x0)) for n, dx, x0 in spec]
+    h0 = max(g.dx[0] for g in geoms)
+    nrb = c["nrb"]
+    for g in geoms:
+        lev = int(round(np.log2(h0 / g.dx[0])))
+        m = nrb << lev
+        f = 0
+        for d in range(3):
+            lx = int(round((g.xmin[d] - c["lo"]) / (g.nx[d] * g.dx[d])))
+            if not c["periodic"][d]:
+                f |= (1 << (2 * d)) if lx == 0 else 0
+                f |= (2 << (2 * d)) if lx == m - 1 else 0
+        g.outflow_faces = f
+    return c, geoms
+
+
+@pytest.mark.parametrize("case", ["periodic_corner", "open_centre", "open_3level", "open_corner"])
+@pytest.mark.parametrize("nvar", [22, 3])
+def test_ml_exchange_vjp_dot_product(case, nvar):
+    """<G x, y> = <x, G^T y> for the update across levels (restriction,
+    gather, coarse physical fill, prolongation)."""
+    c, geoms = _ml_geoms(case)
     z = Z4c(geoms, Params())
-    z.set_exchange([-0.5] * 3, [0.5] * 3, (True,) * 3)
-    ub = z.zeros()
-    with pytest.raises(Z4cError):
-        z.exchange_vjp(ub)
-    with pytest.raises(Z4cError):
-        z.rk4_step_ghosts_vjp(0.01, z.zeros(), ub, 1)
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    rng = np.random.default_rng(23)
+    x = z.upload([rng.standard_normal((nvar,) + g.shape) for g in geoms], nvar)
+    y = z.upload([rng.standard_normal((nvar,) + g.shape) for g in geoms], nvar)
+    gx = x.clone()
+    z.exchange(gx, nvar)
+    gty = y.clone()
+    z.exchange_vjp(gty, nvar)
+    gty2 = y.clone()
+    z.exchange_vjp(gty2, nvar)
+    torch.cuda.synchronize()
+    lhs, rhs = float((gx * y).sum()), float((x * gty).sum())
+    scale = float((gx.abs() * y.abs()).sum())
+    print(f"{case}, {nvar} fields, {len(geoms)} blocks: <Gx, y> = {lhs:.15e}, "
+          f"<x, G^T y> = {rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+    assert torch.equal(gty, gty2)
+    z.close()
+
+
+def test_physical_bc_vjp_dot_product():
+    """<F x, y> = <x, F^T y> for the physical fill on blocks with outflow
+    faces (every face of one block; two faces of a second)."""
+    g0 = MeshBlockGeom.cube(8, 1.0 / 8, (0.0, 0.0, 0.0))
+    g0.outflow_faces = 63
+    g1 = MeshBlockGeom((10, 8, 12), (0.1, 0.12, 0.08), (2.0, 0.0, 0.0))
+    g1.outflow_faces = 2 | 16
+    geoms = [g0, g1]
+    z = Z4c(geoms, Params())
+    rng = np.random.default_rng(29)
+    x = z.upload([rng.standard_normal((22,) + g.shape) for g in geoms])
+    y = z.upload([rng.standard_normal((22,) + g.shape) for g in geoms])
+    fx = x.clone()
+    z.physical_bc(fx)
+    fty = y.clone()
+    z.physical_bc_vjp(fty)
+    torch.cuda.synchronize()
+    lhs, rhs = float((fx * y).sum()), float((x * fty).sum())
+    scale = float((fx.abs() * y.abs()).sum())
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
     z.close()
 
 
@@ -138,7 +209,9 @@ def test_step_ghosts_vjp_finite_differences():
     z.close()
 
 
-def test_step_ghosts_vjp_rejects_physical_faces():
+def test_step_ghosts_vjp_rejects_unfilled_and_sommerfeld_faces():
+    """Outflow faces need the fill in the step (ghosts = 6); Sommerfeld
+    faces have no adjoint here."""
     geoms, lo, hi, _ = _grid((2, 1, 1), 8, 1.0 / 8, (False, True, True))
     geoms[0].outflow_faces = 1
     p = Params(**PUNCTURE)
@@ -147,4 +220,98 @@ def test_step_ghosts_vjp_rejects_physical_faces():
     u = z.zeros()
     with pytest.raises(Z4cError):
         z.rk4_step_ghosts_vjp(0.01, u, z.zeros(), 1)
+    z.close()
+    geoms, lo, hi, _ = _grid((2, 1, 1), 8, 1.0 / 8, (False, True, True))
+    geoms[0].sommerfeld_faces = 1
+    z = Z4c(geoms, p)
+    z.set_exchange(lo, hi, (False, True, True))
+    with pytest.raises(Z4cError):
+        z.rk4_step_ghosts_vjp(0.01, z.zeros(), z.zeros(), 1, physical=True)
+    z.close()
+
+
+def _smooth_state(z, geoms, p):
+    """The gauge wave of period 4 along x (smooth across all blocks) through
+    ADMToZ4c, a constant shift added (fixed upwind selection), AlgConstr."""
+    from z4c_amd.initial_data import gauge_wave_adm
+    adm = z.upload([gauge_wave_adm(g, A=0.05, d=4.0) for g in geoms], 17)
+    u = z.zeros()
+    z.ADMToZ4c(adm, u)
+    for b in range(len(geoms)):
+        v = z.block_view(u, b, 22)
+        for a, sh in enumerate((0.3, -0.25, 0.2)):
+            v[19 + a] += sh
+    z.AlgConstr(u)
+    torch.cuda.synchronize()
+    return u
+
+
+@pytest.mark.parametrize("case,physical", [("periodic_2level", False), ("open_centre", True)])
+def test_ml_step_ghosts_vjp_finite_differences(case, physical):
+    """Central differences of <w, step(u)> for the step with the update
+    across levels after every stage.  AlgConstr's conformal factor takes a
+    constant branch where det(gt) - 1 < eps_floor
+    (src/utils/floating_point.hpp:120-131); a constrained state has
+    det(gt) = 1 to rounding, so at the default eps_floor (1e-12) the stage
+    states sit on that kink and differences straddle it.  Periodic mesh:
+    per-block bumps and eps_floor far below zero (the pow branch
+    everywhere).  Open mesh with the outflow fill: a state smooth across all
+    blocks (the fill extrapolates it outward without amplifying jumps),
+    eps_floor = -0.5 (the kink at det(gt) = 1/2, away from the state, while
+    the ghost layer the fill leaves at zero, the upper faces' outermost as
+    in the reference, takes the finite constant branch) and w, v at interior
+    vertices (ghost corners extrapolate that zero layer)."""
+    c, geoms = _ml_geoms(case)
+    if physical:
+        p = Params(**dict(PUNCTURE, eps_floor=-0.5, damp_kappa2=0.05))
+    else:
+        p = Params(**dict(PUNCTURE, eps_floor=-1e30, damp_kappa2=0.05))
+    z = Z4c(geoms, p)
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    u_d = _smooth_state(z, geoms, p) if physical else z.upload(_state(geoms, p, seed=31))
+    dt = 0.25 * min(g.dx[0] for g in geoms)
+    rng = np.random.default_rng(6)
+
+    def field(g):
+        """Random values; on the open mesh at interior vertices only."""
+        a = rng.standard_normal((22,) + g.shape)
+        if physical:
+            m = np.zeros(g.shape, bool)
+            m[4:-4, 4:-4, 4:-4] = True
+            a *= m
+        return a
+
+    w_d = z.upload([field(g) for g in geoms])
+    v_d = z.upload([field(g) for g in geoms])
+    ubar, pbar = z.rk4_step_ghosts_vjp(dt, u_d, w_d, 1, physical=physical)
+    work = z.make_step_work()
+    eps = 1e-6
+
+    def phi(uu, **kw):
+        q = Params(**dict(p.as_dict(), **kw))
+        out = uu.clone()
+        saved = z.params
+        z.params = q
+        z.rk4_step_fused(dt, out, work, exchange=True, physical=physical)
+        z.params = saved
+        return float((out * w_d).sum())
+
+    # fourth order: on the open mesh the step's curvature leaves the
+    # two-point difference ~1e-5 off at eps = 1e-6
+    fd = (-phi(u_d + 2 * eps * v_d) + 8 * phi(u_d + eps * v_d) - 8 * phi(u_d - eps * v_d)
+          + phi(u_d - 2 * eps * v_d)) / (12 * eps)
+    an = float((ubar * v_d).sum())
+    print(f"{case}: {len(geoms)} blocks; state: fd {fd:.12e}, adjoint {an:.12e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    for t, name in enumerate(PBAR[:2]):
+        x = getattr(p, name)
+        # Theta ~ 0 in the constraint-satisfying open-mesh state leaves the
+        # damping derivative ~1e-3 of the sum's magnitude: a step of 1e-2
+        # keeps rounding in the sum below the tolerance
+        h = 1e-2 * max(1.0, abs(x))
+        fd = (-phi(u_d, **{name: x + 2 * h}) + 8 * phi(u_d, **{name: x + h})
+              - 8 * phi(u_d, **{name: x - h}) + phi(u_d, **{name: x - 2 * h})) / (12 * h)
+        an = float(pbar[0, t])
+        print(f"{name}: fd {fd:.12e}, adjoint {an:.12e}", flush=True)
+        assert abs(fd - an) <= 1e-6 * abs(an), (name, fd, an)
     z.close()
