@@ -133,6 +133,10 @@ def parse():
                     help="steps of the multi-level evolution leg")
     ap.add_argument("--sensitivity-only", action="store_true",
                     help="run only the sensitivity leg and print its object (for profiling)")
+    ap.add_argument("--sensitivity-tree", choices=("independent", "balanced"),
+                    default="independent",
+                    help="block set of --sensitivity-only: independent 16^3 blocks with ghosts "
+                         "held at u_n, or the balanced tree with the ghost update")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one device copy of the state (known bytes) for counter calibration")
     return ap.parse_args()
@@ -215,7 +219,8 @@ def main():
 
     blocks, params, owned, adm_fn = make_workload(args.nblocks)
     if args.sensitivity_only:
-        print(json.dumps({"sensitivity": sensitivity_run(adm_fn, params)}), flush=True)
+        print(json.dumps({"sensitivity": sensitivity_run(adm_fn, params,
+                                                         tree=args.sensitivity_tree)}), flush=True)
         return
     if args.multilevel_only:
         print(json.dumps({"multilevel_evolution": multilevel_run(adm_fn, params, args.ml_steps)}),
@@ -360,6 +365,8 @@ def main():
         del u, work
         torch.cuda.empty_cache()
         out["sensitivity"] = sensitivity_run(adm_fn, params)
+        torch.cuda.empty_cache()
+        out["sensitivity_balanced_tree"] = sensitivity_run(adm_fn, params, tree="balanced")
     if not args.no_exchange:
         out["exchange_step"] = exchange_step_cost(params)
     if not args.no_multilevel:
@@ -519,50 +526,72 @@ def multilevel_run(adm_fn, params, nsteps=100, nblocks=2048):
     return res
 
 
-def sensitivity_run(adm_fn, params, nsteps=100, window=10):
-    """Brill-Lindquist binary on 2048 blocks of 16^3 cells at levels 0-7
-    (octree refined toward the punctures), moving-puncture gauge (1+log,
-    Gamma-driver, eta = 2, kappa1 = 0.02), CFL 0.25: ``nsteps`` fused steps
-    replayed from a HIP graph, then J = sum dx dy dz Ht^2 and the adjoint
-    through the last ``window`` steps (dJ/d eta, kappa1, kappa2 with the
-    window's start state held fixed).  Also times single forward and adjoint
-    steps and checks dJ/d eta and dJ/d kappa1 against central differences
-    over the window (steps 1e-3 and 1e-4 of the parameter scale)."""
+def sensitivity_run(adm_fn, params, nsteps=100, window=10, tree="independent"):
+    """Brill-Lindquist binary, moving-puncture gauge (1+log, Gamma-driver,
+    eta = 2, kappa1 = 0.02), CFL 0.25: ``nsteps`` fused steps replayed from
+    a HIP graph, then J = sum dx dy dz Ht^2 and the adjoint through the last
+    ``window`` steps (dJ/d eta, kappa1, kappa2 with the window's start state
+    held fixed).  Also times single forward and adjoint steps and checks
+    dJ/d eta and dJ/d kappa1 against central differences over the window
+    (steps 1e-3 and 1e-4 of the parameter scale).
+
+    tree "independent": 2048 blocks of 16^3 cells at levels 0-7 (octree
+    refined toward the punctures, leaves dropped to reach the count), every
+    block's ghosts held at u_n.  tree "balanced": the 2:1-balanced tree of
+    2150 blocks of 16^3 cells (levels 0-7, open box [-32, 32]^3 with outflow
+    faces) with the ghost update across levels and the outflow fill after
+    every stage, forward and adjoint (z4c_rk4_step_ghosts_vjp)."""
     import torch
     from z4c_amd import Params
-    from z4c_amd.initial_data import octree_blocks, owned_points
+    from z4c_amd.initial_data import balanced_octree_blocks, octree_blocks, owned_points
     from z4c_amd.sensitivity import hamiltonian_sensitivity
     from z4c_amd.z4c import Z4c
-    blocks = octree_blocks(nblocks=2048, seed=0, sizes=(16,), mix=(1.0,))
+    ghosts = tree == "balanced"
+    if ghosts:
+        blocks, lo, hi = balanced_octree_blocks(nblocks=2048)
+    else:
+        blocks = octree_blocks(nblocks=2048, seed=0, sizes=(16,), mix=(1.0,))
     z = Z4c(blocks, params)
+    if ghosts:
+        z.set_exchange(lo, hi, (False, False, False))
     adm = z.empty(17)
     for b, g in enumerate(blocks):
         z.block_view(adm, b, 17).copy_(torch.from_numpy(adm_fn(g)))
     u0 = z.zeros()
     z.ADMToZ4c(adm, u0)
     del adm
+    if ghosts:
+        # the ghosts of the initial data (Mesh::FinalizeZ4cADMGhosts order)
+        z.exchange(u0)
+        z.physical_bc(u0)
+        z.AlgConstr(u0)
     dt = min(z.NewBlockTimeStep(0.25), 1.0)
     z.autotune(u0, reps=2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = hamiltonian_sensitivity(z, u0, dt, nsteps, window)
+    res = hamiltonian_sensitivity(z, u0, dt, nsteps, window, exchange=ghosts, physical=ghosts)
     torch.cuda.synchronize()
     t_total = time.perf_counter() - t0
     # single-step costs on the window's start state
     w = res["u_window"]
     work = z.make_step_work()
     x = w.clone()
-    z.rk4_step_fused(dt, x, work, graph=True)
+    z.rk4_step_fused(dt, x, work, graph=True, exchange=ghosts, physical=ghosts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(5):
-        z.rk4_step_fused(dt, x, work, graph=True)
+        z.rk4_step_fused(dt, x, work, graph=True, exchange=ghosts, physical=ghosts)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / 5
+    del x
     t0 = time.perf_counter()
-    z.rk4_step_vjp(dt, w, res["ubar"], 1)
+    if ghosts:
+        z.rk4_step_ghosts_vjp(dt, w, res["ubar"], 1, physical=True)
+    else:
+        z.rk4_step_vjp(dt, w, res["ubar"], 1)
     torch.cuda.synchronize()
     t_vjp = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
 
     def J_of(**kw):
         q = Params(**dict(params.as_dict(), **kw))
@@ -570,14 +599,17 @@ def sensitivity_run(adm_fn, params, nsteps=100, window=10):
         z.params = q
         y = w.clone()
         for _ in range(window):
-            z.rk4_step_fused(dt, y, work)
+            z.rk4_step_fused(dt, y, work, exchange=ghosts, physical=ghosts)
         J = z.hamiltonian_norm(y)
         z.params = saved
         return J
 
-    # Central differences at two steps: agreement at both shows that the
-    # comparison is limited neither by the difference's truncation error
-    # (O(h^2)) nor by its rounding error (O(1/h)).
+    # Central differences at two small steps (agreement at both shows that
+    # the comparison is limited neither by truncation, O(h^2), nor by
+    # rounding, O(1/h)), and the 5-point difference at half the parameter's
+    # scale: over a window of t ~ 0.02 J is close to a quartic in the
+    # parameter, and where dJ/d kappa1 is ~1e-7 of J (Theta ~ 0) only a
+    # wide step lifts the difference of J above its rounding.
     fd = {}
     for name in ("shift_eta", "damp_kappa1"):
         x0 = getattr(params, name)
@@ -588,22 +620,32 @@ def sensitivity_run(adm_fn, params, nsteps=100, window=10):
             d = (J_of(**{name: x0 + h}) - J_of(**{name: x0 - h})) / (2 * h)
             fd[name][tag] = {"central_difference": d,
                              "rel_diff": abs(an - d) / max(abs(d), 1e-300)}
-        fd[name]["rel_diff"] = fd[name]["h1e-3"]["rel_diff"]
+        h = 0.5 * max(abs(x0), 1e-2)
+        d = (-J_of(**{name: x0 + 2 * h}) + 8 * J_of(**{name: x0 + h})
+             - 8 * J_of(**{name: x0 - h}) + J_of(**{name: x0 - 2 * h})) / (12 * h)
+        fd[name]["five_point_h0.5"] = {"difference": d,
+                                       "rel_diff": abs(an - d) / max(abs(d), 1e-300)}
+        fd[name]["rel_diff"] = fd[name]["five_point_h0.5"]["rel_diff"]
     z.close()
+    form = ("2:1-balanced tree, open box [-32, 32]^3 with outflow faces, ghost update across "
+            "levels and outflow fill after every stage (forward and adjoint)" if ghosts else
+            "independent blocks (ghosts held at u_n)")
     out = {
         "workload": f"Brill-Lindquist binary, {len(blocks)} blocks of 16^3 cells, levels 0-7, "
-                    f"moving-puncture gauge, CFL 0.25, {nsteps} steps (HIP graph), adjoint "
-                    f"through the last {window}",
+                    f"{form}, moving-puncture gauge, CFL 0.25, {nsteps} steps (HIP graph), "
+                    f"adjoint through the last {window}",
+        "tree": tree, "nblocks": len(blocks),
+        "device_memory_in_use_gb": (total - free) / 1e9,
         "owned_points": owned_points(blocks), "dt": dt, "J": res["J"], "dJ": res["pbar"],
         "seconds_total": t_total, "forward_step_ms": t_step * 1e3, "adjoint_step_ms": t_vjp * 1e3,
         "adjoint_step_over_forward_step": t_vjp / t_step, "fd_check": fd,
     }
-    log(f"sensitivity: {nsteps} steps + {window}-step adjoint in {t_total:.2f} s, "
+    log(f"sensitivity ({tree} tree): {nsteps} steps + {window}-step adjoint in {t_total:.2f} s, "
         f"J = {res['J']:.6e}, dJ/deta = {res['pbar']['shift_eta']:.6e}, "
         f"dJ/dkappa1 = {res['pbar']['damp_kappa1']:.6e}; step {t_step*1e3:.2f} ms, "
         f"adjoint step {t_vjp*1e3:.2f} ms; FD rel diff "
         + ", ".join(f"{k} {v['h1e-3']['rel_diff']:.2e} (h 1e-3), {v['h1e-4']['rel_diff']:.2e} "
-                    f"(h 1e-4)" for k, v in fd.items()))
+                    f"(h 1e-4), {v['rel_diff']:.2e} (5-point, h 0.5)" for k, v in fd.items()))
     return out
 
 

@@ -331,6 +331,91 @@ def test_sensitivity_2048_blocks_fd():
     z.close()
 
 
+def test_sensitivity_balanced_tree_with_ghost_update_fd():
+    """The sensitivity on 2150 blocks of 16^3 cells on a 2:1-balanced tree
+    (levels 0-7, open box with outflow faces) with the ghost update across
+    levels and the outflow fill after every stage, forward and adjoint
+    (z4c_rk4_step_ghosts_vjp): dJ/d eta, dJ/d kappa1 and dJ/du along a
+    direction at interior vertices against 5-point central differences
+    over the window.  eps_floor as in the independent-block check.  The
+    initial data get a constant shift and a Theta bump: from zero shift on
+    the conformally flat slice (Gamma~ = 0) eta acts only at second order,
+    and from Theta = 0 kappa1 barely acts, so over a short window dJ/d eta
+    and dJ/d kappa1 are ~1e-12 and ~1e-8 of J, below what differences of J
+    resolve."""
+    from z4c_amd.initial_data import balanced_octree_blocks
+    from z4c_amd.sensitivity import hamiltonian_sensitivity
+    params = Params(**dict(MP_GAUGE, eps_floor=-1e-3))
+    blocks, lo, hi = balanced_octree_blocks(nblocks=2048)
+    z = Z4c(blocks, params)
+    z.set_exchange(lo, hi, (False, False, False))
+    adm = z.empty(17)
+    for b, g in enumerate(blocks):
+        z.block_view(adm, b, 17).copy_(torch.from_numpy(brill_lindquist_adm(g)))
+    u0 = z.zeros()
+    z.ADMToZ4c(adm, u0)
+    del adm
+    for b in range(len(blocks)):
+        v = z.block_view(u0, b, 22)
+        for a, sh in enumerate((0.05, -0.03, 0.02)):
+            v[19 + a] += sh
+        x = blocks[b].coords()
+        r2 = (x[0][None, None, :] ** 2 + x[1][None, :, None] ** 2 + x[2][:, None, None] ** 2)
+        v[14] += torch.from_numpy(0.05 * np.exp(-r2 / 128.0)).to(z.device)
+    z.exchange(u0)
+    z.physical_bc(u0)
+    z.AlgConstr(u0)
+    dt = 0.25 * min(g.dx[0] for g in blocks)
+    nsteps, window = 12, 6
+    res = hamiltonian_sensitivity(z, u0, dt, nsteps, window, exchange=True, physical=True)
+    print(f"{len(blocks)} blocks: J = {res['J']:.6e}, dJ = {res['pbar']}", flush=True)
+    w0 = res["u_window"]
+    work = z.make_step_work()
+
+    def J_of(u=None, **kw):
+        saved = z.params
+        z.params = Params(**dict(params.as_dict(), **kw))
+        y = (w0 if u is None else u).clone()
+        for _ in range(window):
+            z.rk4_step_fused(dt, y, work, exchange=True, physical=True)
+        J = z.hamiltonian_norm(y)
+        z.params = saved
+        return J
+
+    # 5-point differences at half the parameter's scale: over a window of
+    # t ~ 0.05 J is close to a quartic in eta and kappa1 (the stencil's
+    # truncation error is O(h^4) times powers of t), while a small h would
+    # leave differences of J at the level of its rounding
+    for name in ("shift_eta", "damp_kappa1"):
+        x0 = getattr(params, name)
+        h = 0.5 * max(abs(x0), 1e-2)
+        Jp2, Jp1 = J_of(**{name: x0 + 2 * h}), J_of(**{name: x0 + h})
+        Jm1, Jm2 = J_of(**{name: x0 - h}), J_of(**{name: x0 - 2 * h})
+        fd = (-Jp2 + 8.0 * Jp1 - 8.0 * Jm1 + Jm2) / (12.0 * h)
+        an = res["pbar"][name]
+        rel = abs(an - fd) / abs(fd)
+        print(f"dJ/d{name}: adjoint {an:.9e}, central difference {fd:.9e}, rel {rel:.2e}",
+              flush=True)
+        assert rel <= 1e-6
+    # a direction at interior vertices, scaled per field to the state
+    gen = torch.Generator(device=z.device).manual_seed(3)
+    v = torch.randn(w0.shape, generator=gen, device=z.device, dtype=torch.float64)
+    for b in range(len(blocks)):
+        vb = z.block_view(v, b, 22)
+        m = torch.zeros_like(vb[0], dtype=torch.bool)
+        m[4:-4, 4:-4, 4:-4] = True
+        vb.mul_(m)
+        vb.mul_(z.block_view(w0, b, 22).abs().amax(dim=(1, 2, 3), keepdim=True) + 1e-3)
+    eps = 1e-6
+    fd = (-J_of(w0 + 2 * eps * v) + 8.0 * J_of(w0 + eps * v) - 8.0 * J_of(w0 - eps * v)
+          + J_of(w0 - 2 * eps * v)) / (12.0 * eps)
+    an = float((res["ubar"] * v).sum())
+    rel = abs(an - fd) / abs(fd)
+    print(f"dJ/du . v: adjoint {an:.9e}, central difference {fd:.9e}, rel {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+    z.close()
+
+
 # ---------------------------------------------------------------------------
 # Brill-Lindquist on a balanced block tree: ghost update across levels
 # ---------------------------------------------------------------------------
