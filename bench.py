@@ -133,6 +133,12 @@ def parse():
                     help="steps of the multi-level evolution leg")
     ap.add_argument("--sensitivity-only", action="store_true",
                     help="run only the sensitivity leg and print its object (for profiling)")
+    ap.add_argument("--sens-steps", type=int, default=100,
+                    help="steps of --sensitivity-only's evolution")
+    ap.add_argument("--sens-window", type=int, default=10,
+                    help="adjoint window of --sensitivity-only")
+    ap.add_argument("--sens-no-fd", action="store_true",
+                    help="skip --sensitivity-only's difference check (short profiled runs)")
     ap.add_argument("--sensitivity-tree", choices=("independent", "balanced"),
                     default="independent",
                     help="block set of --sensitivity-only: independent 16^3 blocks with ghosts "
@@ -219,8 +225,9 @@ def main():
 
     blocks, params, owned, adm_fn = make_workload(args.nblocks)
     if args.sensitivity_only:
-        print(json.dumps({"sensitivity": sensitivity_run(adm_fn, params,
-                                                         tree=args.sensitivity_tree)}), flush=True)
+        print(json.dumps({"sensitivity": sensitivity_run(
+            adm_fn, params, args.sens_steps, args.sens_window, tree=args.sensitivity_tree,
+            fd_check=not args.sens_no_fd)}), flush=True)
         return
     if args.multilevel_only:
         print(json.dumps({"multilevel_evolution": multilevel_run(adm_fn, params, args.ml_steps)}),
@@ -526,7 +533,7 @@ def multilevel_run(adm_fn, params, nsteps=100, nblocks=2048):
     return res
 
 
-def sensitivity_run(adm_fn, params, nsteps=100, window=10, tree="independent"):
+def sensitivity_run(adm_fn, params, nsteps=100, window=10, tree="independent", fd_check=True):
     """Brill-Lindquist binary, moving-puncture gauge (1+log, Gamma-driver,
     eta = 2, kappa1 = 0.02), CFL 0.25: ``nsteps`` fused steps replayed from
     a HIP graph, then J = sum dx dy dz Ht^2 and the adjoint through the last
@@ -611,7 +618,7 @@ def sensitivity_run(adm_fn, params, nsteps=100, window=10, tree="independent"):
     # parameter, and where dJ/d kappa1 is ~1e-7 of J (Theta ~ 0) only a
     # wide step lifts the difference of J above its rounding.
     fd = {}
-    for name in ("shift_eta", "damp_kappa1"):
+    for name in ("shift_eta", "damp_kappa1") if fd_check else ():
         x0 = getattr(params, name)
         an = res["pbar"][name]
         fd[name] = {"adjoint": an}

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1333,12 +1334,37 @@ struct ExTArgs {
   double* tmp;         // nvar * nv
 };
 
+// NV > 0: nvar == NV with the accumulators in registers; NV == 0: any
+// nvar, one field at a time (the same order of sums per field)
+template <int NV>
 __global__ void __launch_bounds__(256) k_exchange_vjp_gather(ExTArgs a) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= a.nv) return;
   const DevBlock& V = a.blocks[a.v[2 * t]];
   const int pv = a.v[2 * t + 1];
   const int e0 = a.voff[t], e1 = a.voff[t + 1];
+  if constexpr (NV > 0) {
+    double acc[NV];
+    const bool self = !a.flag[t];
+    const double* vp = a.ubar + V.off * a.nvar + pv;
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n] = self ? vp[(long long)n * V.npts] : 0.0;
+    for (int e = e0; e < e1; ++e) {
+      const int d = a.vd[e];
+      const int cnt = a.src_off[d + 1] - a.src_off[d];
+      const DevBlock& D = a.blocks[a.dst[2 * d]];
+      const double* dp = a.ubar + D.off * a.nvar + a.dst[2 * d + 1];
+      const double w = 1.0 / (double)cnt;
+#pragma unroll
+      for (int n = 0; n < NV; ++n) {
+        const double ud = dp[(long long)n * D.npts];
+        acc[n] += cnt > 1 ? ud * w : ud;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) a.tmp[(long long)n * a.nv + t] = acc[n];
+    return;
+  }
   for (int n = 0; n < a.nvar; ++n) {
     double acc = a.flag[t] ? 0.0 : a.ubar[V.off * a.nvar + (long long)n * V.npts + pv];
     for (int e = e0; e < e1; ++e) {
@@ -1642,18 +1668,28 @@ __global__ void __launch_bounds__(256) k_physical_coarse(PhysCArgs a) {
 // c-bar = 0 to start.  Every step is a fixed-order gather (deterministic).
 // ---------------------------------------------------------------------------
 
-// P^T: each coarse vertex the prolongation reads gathers w * u-bar_f over the
-// fine vertices f the prolongation writes (the mask: each counted once,
+// P^T: each coarse vertex c the prolongation reads gathers w * u-bar_f over
+// the fine vertices f the prolongation writes (the mask: each counted once,
 // however many boxes overlap there) whose stencil holds it.  Per axis a
 // fine vertex either injects coarse index c (even f = 2 (c - RF_H), weight
 // 1) or interpolates around base i (odd f = 2 (i - RF_H) + 1, reading
 // i - 2 .. i + 3 with the 6-point midpoint weights), and its weight is the
-// product over the axes.  The fine cotangents written are then zeroed.
+// product over the axes, so P^T is three 1-D transposes of at most 7 terms
+// (z, then y, then x).  One workgroup per tile (a block's coarse rows
+// [cy0, cy0 + ncy) x [cz0, cz0 + ncz) over its coarse x range) and field:
+// the masked fine cotangents the tile reads are staged in LDS, the z and y
+// transposes go through two LDS intermediates, and every coarse vertex of
+// the tile is written once (zero where nothing reads it).  The fine
+// cotangents written are then zeroed.
+struct PrTTile {
+  int blk;
+  int cx0, ncx, cy0, ncy, cz0, ncz;  // coarse vertices written
+  int fx0, nfx, fy0, nfy, fz0, nfz;  // fine vertices staged
+};
 struct PrTArgs {
   const DevBlock* blocks;
   const RfBlock* cb;
-  const int* cl;   // coarse vertices (block, point)
-  int ncl;
+  const PrTTile* tiles;
   const int* fl;   // fine vertices the prolongation writes (block, point)
   int nfl;
   const unsigned char* fmask;
@@ -1662,52 +1698,64 @@ struct PrTArgs {
   double* cbar;
 };
 
-__device__ __forceinline__ int prt_axis(int c, int nn, int* f, double* w) {
-  int m = 0;
-  const int fe = 2 * (c - RF_H);
-  if (fe >= 0 && fe < nn) {
-    f[m] = fe;
-    w[m] = 1.0;
-    ++m;
-  }
+// sum over the fine candidates of coarse index c on one axis: get(k) reads
+// the staged value at fine offset k = f - f0 (0 <= k < nf)
+template <class Get>
+__device__ __forceinline__ double prt_axis_sum(int c, int f0, int nf, Get get) {
+  double acc = 0.0;
+  const int fe = 2 * (c - RF_H) - f0;
+  if (fe >= 0 && fe < nf) acc += get(fe);
 #pragma unroll
   for (int o = -2; o <= 3; ++o) {
-    const int i = c - o;  // base of the prolongated vertex
-    const int fo = 2 * (i - RF_H) + 1;
-    if (fo >= 0 && fo < nn) {
-      f[m] = fo;
-      w[m] = rf_lag(o >= 1 ? 3 - o : 2 + o);
-      ++m;
-    }
+    const int fo = fe - 2 * o + 1;  // odd vertex around base c - o
+    if (fo >= 0 && fo < nf) acc += rf_lag(o >= 1 ? 3 - o : 2 + o) * get(fo);
   }
-  return m;
+  return acc;
 }
 
-__global__ void __launch_bounds__(256) k_prolong_vjp(PrTArgs a) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= a.ncl) return;
-  const int b = a.cl[2 * t], cp = a.cl[2 * t + 1];
-  const DevBlock& B = a.blocks[b];
-  const RfBlock& R = a.cb[b];
-  const int cx = cp % R.cn1, cy = (cp / R.cn1) % R.cn2, cz = cp / (R.cn1 * R.cn2);
-  int fx[7], fy[7], fz[7];
-  double wx[7], wy[7], wz[7];
-  const int nx = prt_axis(cx, B.nn1, fx, wx), ny = prt_axis(cy, B.nn2, fy, wy),
-            nz = prt_axis(cz, B.nn3, fz, wz);
-  double acc[NVAR];
-  for (int n = 0; n < a.nvar; ++n) acc[n] = 0.0;
+constexpr int PRT_THREADS = 512;
+
+__global__ void __launch_bounds__(PRT_THREADS) k_prolong_vjp(PrTArgs a) {
+  extern __shared__ double lds[];
+  const PrTTile T = a.tiles[blockIdx.x];
+  const int n = blockIdx.y;
+  const DevBlock& B = a.blocks[T.blk];
+  const RfBlock& R = a.cb[T.blk];
+  const int nfx = T.nfx, nfy = T.nfy, nfz = T.nfz;
+  double* F = lds;                        // [z][y][x] staged fine cotangents
+  double* T1 = F + nfx * nfy * nfz;       // [kz][y][x] after the z transpose
+  double* T2 = T1 + nfx * nfy * T.ncz;    // [kz][ky][x] after the y transpose
+  const double* ub = a.ubar + B.off * a.nvar + (long long)n * B.npts;
   const unsigned char* mk = a.fmask + B.off;
-  const double* ub = a.ubar + B.off * a.nvar;
-  for (int qz = 0; qz < nz; ++qz)
-    for (int qy = 0; qy < ny; ++qy)
-      for (int qx = 0; qx < nx; ++qx) {
-        const long long p = fx[qx] + (long long)B.nn1 * (fy[qy] + (long long)B.nn2 * fz[qz]);
-        if (!mk[p]) continue;
-        const double w = (wz[qz] * wy[qy]) * wx[qx];
-        for (int n = 0; n < a.nvar; ++n) acc[n] += w * ub[(long long)n * B.npts + p];
-      }
-  double* cbo = a.cbar + R.coff * a.nvar + cp;
-  for (int n = 0; n < a.nvar; ++n) cbo[(long long)n * R.cnpts] = acc[n];
+  const int nF = nfx * nfy * nfz;
+  for (int q = threadIdx.x; q < nF; q += PRT_THREADS) {
+    const int x = q % nfx, r = q / nfx, y = r % nfy, z = r / nfy;
+    const long long p = (T.fx0 + x) + (long long)B.nn1 * ((T.fy0 + y) + (long long)B.nn2 * (T.fz0 + z));
+    F[q] = mk[p] ? ub[p] : 0.0;
+  }
+  __syncthreads();
+  const int n1 = nfx * nfy * T.ncz;
+  for (int q = threadIdx.x; q < n1; q += PRT_THREADS) {
+    const int xy = q % (nfx * nfy), kz = q / (nfx * nfy);
+    T1[q] = prt_axis_sum(T.cz0 + kz, T.fz0, nfz,
+                         [&](int k) { return F[xy + nfx * nfy * k]; });
+  }
+  __syncthreads();
+  const int n2 = nfx * T.ncy * T.ncz;
+  for (int q = threadIdx.x; q < n2; q += PRT_THREADS) {
+    const int x = q % nfx, r = q / nfx, ky = r % T.ncy, kz = r / T.ncy;
+    const double* t1 = T1 + x + nfx * nfy * kz;
+    T2[q] = prt_axis_sum(T.cy0 + ky, T.fy0, nfy, [&](int k) { return t1[nfx * k]; });
+  }
+  __syncthreads();
+  const int n3 = T.ncx * T.ncy * T.ncz;
+  double* cbo = a.cbar + R.coff * a.nvar + (long long)n * R.cnpts;
+  for (int q = threadIdx.x; q < n3; q += PRT_THREADS) {
+    const int kx = q % T.ncx, r = q / T.ncx, ky = r % T.ncy, kz = r / T.ncy;
+    const double* t2 = T2 + nfx * (ky + T.ncy * kz);
+    const double v = prt_axis_sum(T.cx0 + kx, T.fx0, nfx, [&](int k) { return t2[k]; });
+    cbo[(T.cx0 + kx) + R.cn1 * ((T.cy0 + ky) + R.cn2 * (T.cz0 + kz))] = v;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_prolong_vjp_zero(PrTArgs a) {
@@ -1823,12 +1871,33 @@ __device__ __forceinline__ double* mlt_ptr(const MltArgs& a, int blkarr, int p, 
   return a.ubar + B.off * a.nvar + p;
 }
 
+// NV > 0: nvar == NV, the accumulators in registers and each destination
+// resolved once; NV == 0: any nvar, one field at a time.  Both sum each
+// field in the same order.
+template <int NV>
 __global__ void __launch_bounds__(256) k_mlx_vjp_gather(MltArgs a) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= a.nv) return;
   long long fs;
   const double* vp = mlt_ptr(a, a.v[2 * t], a.v[2 * t + 1], fs);
   const int e0 = a.voff[t], e1 = a.voff[t + 1];
+  if constexpr (NV > 0) {
+    double acc[NV];
+    const bool self = !a.flag[t];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) acc[n] = self ? vp[n * fs] : 0.0;
+    for (int e = e0; e < e1; ++e) {
+      const int d = a.vd[e];
+      long long dfs;
+      const double* dp = mlt_ptr(a, a.dst[2 * d], a.dst[2 * d + 1], dfs);
+      const double w = a.inv[d];
+#pragma unroll
+      for (int n = 0; n < NV; ++n) acc[n] += w * dp[n * dfs];
+    }
+#pragma unroll
+    for (int n = 0; n < NV; ++n) a.tmp[(long long)n * a.nv + t] = acc[n];
+    return;
+  }
   for (int n = 0; n < a.nvar; ++n) {
     double acc = a.flag[t] ? 0.0 : vp[n * fs];
     for (int e = e0; e < e1; ++e) {
@@ -2580,11 +2649,12 @@ struct z4c_ctx {
   unsigned char* ml_fmask = nullptr;    // fine vertices the prolongation writes (per point)
   int* ml_fl = nullptr;                 // those vertices (block, point)
   int ml_nfl = 0;
-  int* ml_cl = nullptr;                 // coarse vertices the prolongation reads (block, point)
-  int ml_ncl = 0;
+  PrTTile* ml_ptiles = nullptr;         // tiles of the prolongation's transpose
+  int ml_nptiles = 0;
+  int ml_ptlds = 0;                     // LDS doubles of the largest tile
   double* ml_cbar = nullptr;            // coarse cotangents, NVAR * rf_total
   void drop_mlt() {
-    int** q[7] = {&mlt_v, &mlt_flag, &mlt_off, &mlt_d, &mlt_z, &ml_fl, &ml_cl};
+    int** q[6] = {&mlt_v, &mlt_flag, &mlt_off, &mlt_d, &mlt_z, &ml_fl};
     for (int** x : q) {
       hipFree(*x);
       *x = nullptr;
@@ -2592,10 +2662,12 @@ struct z4c_ctx {
     hipFree(mlt_tmp);
     hipFree(ml_fmask);
     hipFree(ml_cbar);
+    hipFree(ml_ptiles);
     mlt_tmp = nullptr;
     ml_fmask = nullptr;
     ml_cbar = nullptr;
-    mlt_nv = mlt_nz = ml_nfl = ml_ncl = 0;
+    ml_ptiles = nullptr;
+    mlt_nv = mlt_nz = ml_nfl = ml_nptiles = ml_ptlds = 0;
     mlt_built = false;
   }
   void drop_ml() {
@@ -3158,18 +3230,25 @@ int build_ml_transpose(z4c_ctx* c) {
     const int b = blkarr >> 1;
     return (blkarr & 1) ? nf + c->hrf[b].coff + p : c->hdev[b].off + p;
   };
-  std::vector<int> id(nf + ncp, -1), tv, cnt;
+  // the sources in memory order (coalesced scatter), each numbered once
+  std::vector<int> id(nf + ncp, -1), tv;
+  std::vector<std::pair<long long, long long>> first;  // (slot, entry)
   for (long long e = 0; e < ns; ++e) {
     const long long k = slot(src[2 * e], src[2 * e + 1]);
-    if (id[k] < 0) {
-      id[k] = (int)cnt.size();
-      tv.push_back(src[2 * e]);
-      tv.push_back(src[2 * e + 1]);
-      cnt.push_back(0);
+    if (id[k] == -1) {
+      id[k] = -2;
+      first.emplace_back(k, e);
     }
-    ++cnt[id[k]];
   }
-  const int nv = (int)cnt.size();
+  std::sort(first.begin(), first.end());
+  const int nv = (int)first.size();
+  for (int q = 0; q < nv; ++q) {
+    id[first[q].first] = q;
+    tv.push_back(src[2 * first[q].second]);
+    tv.push_back(src[2 * first[q].second + 1]);
+  }
+  std::vector<int> cnt(nv, 0);
+  for (long long e = 0; e < ns; ++e) ++cnt[id[slot(src[2 * e], src[2 * e + 1])]];
   std::vector<int> toff(nv + 1, 0);
   for (int q = 0; q < nv; ++q) toff[q + 1] = toff[q] + cnt[q];
   std::vector<int> td(ns), pos(toff.begin(), toff.end() - 1);
@@ -3183,8 +3262,11 @@ int build_ml_transpose(z4c_ctx* c) {
     else
       tflag[q] = 1;
   }
-  std::vector<unsigned char> mask(nf, 0), cmark(ncp, 0);
-  std::vector<int> fl, cl;
+  std::vector<unsigned char> mask(nf, 0);
+  std::vector<int> fl;
+  // per block, the bounding box of the coarse vertices the prolongation reads
+  const int nb = (int)c->hdev.size();
+  std::vector<std::array<int, 6>> cbox(nb, {INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN});
   for (const RfEntry& E : ent) {
     const DevBlock& B = c->hdev[E.blk];
     const RfBlock& R = c->hrf[E.blk];
@@ -3209,16 +3291,76 @@ int build_ml_transpose(z4c_ctx* c) {
     bool any = true;
     for (int d = 0; d < 3; ++d) any = any && rf_read_range(E, d, lo[d], hi[d]);
     if (!any) continue;
-    for (int z = lo[2]; z <= hi[2]; ++z)
-      for (int y = lo[1]; y <= hi[1]; ++y)
-        for (int x = lo[0]; x <= hi[0]; ++x) {
-          const int cp = x + R.cn1 * (y + R.cn2 * z);
-          if (!cmark[R.coff + cp]) {
-            cmark[R.coff + cp] = 1;
-            cl.push_back(E.blk);
-            cl.push_back(cp);
-          }
+    auto& bx = cbox[E.blk];
+    for (int d = 0; d < 3; ++d) {
+      bx[2 * d] = std::min(bx[2 * d], lo[d]);
+      bx[2 * d + 1] = std::max(bx[2 * d + 1], hi[d]);
+    }
+    (void)R;
+  }
+  // tiles of ty x tz coarse rows over each box, one (ty, tz) for the launch:
+  // the fewest staged fine values whose largest tile fits in LDS
+  auto fine_range = [](int c_lo, int c_hi, int nn, int& f0, int& nfv) {
+    f0 = std::max(0, 2 * (c_lo - RF_H) - 5);
+    nfv = std::min(nn - 1, 2 * (c_hi - RF_H) + 5) - f0 + 1;
+  };
+  auto make_tiles = [&](int ty, int tz, std::vector<PrTTile>* out, long long& staged,
+                        int& lds) {
+    staged = 0;
+    lds = 0;
+    for (int b = 0; b < nb; ++b) {
+      const auto& bx = cbox[b];
+      if (bx[0] > bx[1]) continue;
+      const DevBlock& B = c->hdev[b];
+      int fx0, nfx;
+      fine_range(bx[0], bx[1], B.nn1, fx0, nfx);
+      for (int cz = bx[4]; cz <= bx[5]; cz += tz)
+        for (int cy = bx[2]; cy <= bx[3]; cy += ty) {
+          PrTTile t{};
+          t.blk = b;
+          t.cx0 = bx[0];
+          t.ncx = bx[1] - bx[0] + 1;
+          t.cy0 = cy;
+          t.ncy = std::min(ty, bx[3] - cy + 1);
+          t.cz0 = cz;
+          t.ncz = std::min(tz, bx[5] - cz + 1);
+          t.fx0 = fx0;
+          t.nfx = nfx;
+          fine_range(cy, cy + t.ncy - 1, B.nn2, t.fy0, t.nfy);
+          fine_range(cz, cz + t.ncz - 1, B.nn3, t.fz0, t.nfz);
+          staged += (long long)t.nfx * t.nfy * t.nfz;
+          lds = std::max(lds, t.nfx * t.nfy * t.nfz + t.nfx * t.nfy * t.ncz + t.nfx * t.ncy * t.ncz);
+          if (out) out->push_back(t);
         }
+    }
+  };
+  int smem = 0;
+  HIPCALL(hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  const int lds_max = std::min(16384, smem / (int)sizeof(double));  // doubles, at most 128 KiB
+  int best_ty = 0, best_tz = 0;
+  long long best = LLONG_MAX;
+  int ymax = 1, zmax = 1;
+  for (const auto& bx : cbox)
+    if (bx[0] <= bx[1]) {
+      ymax = std::max(ymax, bx[3] - bx[2] + 1);
+      zmax = std::max(zmax, bx[5] - bx[4] + 1);
+    }
+  for (int tz = 1; tz <= std::min(zmax, 8); ++tz)
+    for (int ty = 1; ty <= ymax; ++ty) {
+      long long staged;
+      int lds;
+      make_tiles(ty, tz, nullptr, staged, lds);
+      if (lds <= lds_max && staged < best) {
+        best = staged;
+        best_ty = ty;
+        best_tz = tz;
+      }
+    }
+  std::vector<PrTTile> tiles;
+  if (!ent.empty()) {
+    if (!best_ty) return fail(Z4C_EINVAL, "exchange_vjp: blocks too large for the prolongation's transpose");
+    long long staged;
+    make_tiles(best_ty, best_tz, &tiles, staged, c->ml_ptlds);
   }
   auto up = [&](auto*& d, const auto& h) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(d)>>;
@@ -3228,7 +3370,7 @@ int build_ml_transpose(z4c_ctx* c) {
   };
   bool ok = up(c->mlt_v, tv) && up(c->mlt_flag, tflag) && up(c->mlt_off, toff) &&
             up(c->mlt_d, td) && up(c->mlt_z, tz) && up(c->ml_fmask, mask) && up(c->ml_fl, fl) &&
-            up(c->ml_cl, cl);
+            up(c->ml_ptiles, tiles);
   ok = ok && (nv == 0 || hipMalloc(&c->mlt_tmp, sizeof(double) * NVAR * (size_t)nv) == hipSuccess);
   ok = ok && hipMalloc(&c->ml_cbar, sizeof(double) * NVAR * (size_t)ncp) == hipSuccess;
   if (!ok) {
@@ -3238,7 +3380,7 @@ int build_ml_transpose(z4c_ctx* c) {
   c->mlt_nv = nv;
   c->mlt_nz = (int)tz.size();
   c->ml_nfl = (int)(fl.size() / 2);
-  c->ml_ncl = (int)(cl.size() / 2);
+  c->ml_nptiles = (int)tiles.size();
   c->mlt_built = true;
   return Z4C_OK;
 }
@@ -3250,10 +3392,11 @@ int launch_ml_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
   if (rc) return rc;
   rc = fill(c, c->ml_cbar, 0.0, (long long)nvar * c->rf_total, s);
   if (rc) return rc;
-  PrTArgs pa{c->d_blocks, c->d_rf, c->ml_cl, c->ml_ncl, c->ml_fl, c->ml_nfl, c->ml_fmask,
+  PrTArgs pa{c->d_blocks, c->d_rf, c->ml_ptiles, c->ml_fl, c->ml_nfl, c->ml_fmask,
              nvar, ubar, c->ml_cbar};
-  if (c->ml_ncl) {
-    hipLaunchKernelGGL(k_prolong_vjp, dim3((c->ml_ncl + 255) / 256), dim3(256), 0, s, pa);
+  if (c->ml_nptiles) {
+    hipLaunchKernelGGL(k_prolong_vjp, dim3(c->ml_nptiles, nvar), dim3(PRT_THREADS),
+                       sizeof(double) * c->ml_ptlds, s, pa);
     HIPCALL(hipGetLastError());
   }
   if (c->ml_nfl) {
@@ -3271,7 +3414,10 @@ int launch_ml_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
   MltArgs ma{c->d_blocks, c->d_rf, c->ml_dst, c->ml_inv, c->mlt_v, c->mlt_flag, c->mlt_off,
              c->mlt_d, c->mlt_z, c->mlt_nv, c->mlt_nz, nvar, ubar, c->ml_cbar, c->mlt_tmp};
   if (c->mlt_nv) {
-    hipLaunchKernelGGL(k_mlx_vjp_gather, dim3((c->mlt_nv + 255) / 256), dim3(256), 0, s, ma);
+    if (nvar == NVAR)
+      hipLaunchKernelGGL(k_mlx_vjp_gather<NVAR>, dim3((c->mlt_nv + 255) / 256), dim3(256), 0, s, ma);
+    else
+      hipLaunchKernelGGL(k_mlx_vjp_gather<0>, dim3((c->mlt_nv + 255) / 256), dim3(256), 0, s, ma);
     HIPCALL(hipGetLastError());
   }
   if (c->mlt_nv + c->mlt_nz) {
@@ -3293,7 +3439,11 @@ int launch_exchange_vjp(z4c_ctx* c, int nvar, double* ubar, hipStream_t s) {
   ExTArgs a{c->d_blocks, c->ex_dst, c->ex_off, c->ext_v, c->ext_flag, c->ext_off, c->ext_d,
             c->ext_z, c->ext_nv, c->ext_nz, nvar, ubar, c->ext_tmp};
   if (c->ext_nv) {
-    hipLaunchKernelGGL(k_exchange_vjp_gather, dim3((c->ext_nv + 255) / 256), dim3(256), 0, s, a);
+    if (nvar == NVAR)
+      hipLaunchKernelGGL(k_exchange_vjp_gather<NVAR>, dim3((c->ext_nv + 255) / 256), dim3(256), 0, s,
+                         a);
+    else
+      hipLaunchKernelGGL(k_exchange_vjp_gather<0>, dim3((c->ext_nv + 255) / 256), dim3(256), 0, s, a);
     HIPCALL(hipGetLastError());
   }
   hipLaunchKernelGGL(k_exchange_vjp_scatter, dim3((c->ext_nv + c->ext_nz + 255) / 256), dim3(256),
@@ -4109,6 +4259,25 @@ int z4c_set_exchange(z4c_ctx* c, const double* lo, const double* hi, const int* 
         }
         tl[id].push_back((int)d);
       }
+    {
+      // renumber the sources in memory order (coalesced scatter)
+      const size_t nv = tl.size();
+      std::vector<int> order(nv), rank(nv);
+      for (size_t q = 0; q < nv; ++q) order[q] = (int)q;
+      auto addr = [&](int q) { return c->hdev[tv[2 * q]].off + tv[2 * q + 1]; };
+      std::sort(order.begin(), order.end(), [&](int x, int y) { return addr(x) < addr(y); });
+      std::vector<int> tv2(2 * nv);
+      std::vector<std::vector<int>> tl2(nv);
+      for (size_t q = 0; q < nv; ++q) {
+        rank[order[q]] = (int)q;
+        tv2[2 * q] = tv[2 * order[q]];
+        tv2[2 * q + 1] = tv[2 * order[q] + 1];
+        tl2[q] = std::move(tl[order[q]]);
+      }
+      for (auto& kv : vid) kv.second = rank[kv.second];
+      tv = std::move(tv2);
+      tl = std::move(tl2);
+    }
     std::vector<int> tflag(tl.size(), 0), toff{0}, td, tz;
     for (size_t d = 0; d < nd; ++d) {
       auto it = vid.find(vkey(dst[2 * d], dst[2 * d + 1]));
