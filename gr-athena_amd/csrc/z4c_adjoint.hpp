@@ -956,13 +956,14 @@ __host__ __device__ inline void adj_point_at(const RhsParams& o, const DevBlock&
 // source are formed once for all fields.  Accessors: S(slot, q) derivative
 // cotangent at interior index q, (*W)(n, p) the cotangent of rhs_n at padded
 // index p (W == nullptr: zero, no advective / KO part), Bx(a, p) beta^a.
-// out(n, v) receives the sums.
+// out(n, v) receives the sums.  Fields N0 <= n < N1 (a subset keeps fewer
+// accumulators live).
 struct AdjAxis {
   int c, n, qs, ps;
   bool others;  // the other two coordinates are interior
 };
 
-template <class SIn, class WIn, class BIn, class Out>
+template <int N0 = 0, int N1 = NVAR, class SIn, class WIn, class BIn, class Out>
 __host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBlock& B,
                                                const SIn& S, const WIn* W, const BIn& Bx,
                                                int i, int j, int k, Out&& out) {
@@ -974,9 +975,9 @@ __host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBloc
       ax2{ck, B.nk, B.ni * B.nj, B.nn1 * B.nn2, ix && iy};
   // selected by value (no indexed local array, which would live in scratch)
   auto AX = [&](int a) -> const AdjAxis& { return a == 0 ? ax0 : (a == 1 ? ax1 : ax2); };
-  double acc[NVAR];
+  double acc[N1 - N0];
 #pragma unroll
-  for (int n = 0; n < NVAR; ++n) acc[n] = 0.0;
+  for (int n = 0; n < N1 - N0; ++n) acc[n] = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (!AX(a).others) continue;
@@ -987,9 +988,9 @@ __host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBloc
       const int sq = q0 - oa * AX(a).qs;
       const double c1 = w_d1(oa) * B.cidx1[a], c2 = w_d2(oa) * B.cidx2[a];
 #pragma unroll
-      for (int n = 0; n < NVAR; ++n) {
-        if (lin1(n) >= 0 && oa != 0) acc[n] += c1 * S(l_d1(n, a), sq);
-        if (lin2(n) >= 0) acc[n] += c2 * S(l_d2(n, sym(a, a)), sq);
+      for (int n = N0; n < N1; ++n) {
+        if (lin1(n) >= 0 && oa != 0) acc[n - N0] += c1 * S(l_d1(n, a), sq);
+        if (lin2(n) >= 0) acc[n - N0] += c2 * S(l_d2(n, sym(a, a)), sq);
       }
     }
   }
@@ -1011,8 +1012,8 @@ __host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBloc
           const int sq = q0 - oa * AX(a).qs - ob * AX(b).qs;
           const double cw = (w_d1(oa) * w_d1(ob)) * cab;
 #pragma unroll
-          for (int n = 0; n < NVAR; ++n)
-            if (lin2(n) >= 0) acc[n] += cw * S(l_d2(n, sym(a, b)), sq);
+          for (int n = N0; n < N1; ++n)
+            if (lin2(n) >= 0) acc[n - N0] += cw * S(l_d2(n, sym(a, b)), sq);
         }
       }
     }
@@ -1029,15 +1030,15 @@ __host__ __device__ inline void adj_stencil_at(const RhsParams& o, const DevBloc
         const double cl = (vx < 0 ? w_left(oa) : w_right(oa)) * vx * B.lidx[a];
         const double ck2 = w_ko(oa) * (B.cidxd[a] * o.diss);
 #pragma unroll
-        for (int n = 0; n < NVAR; ++n) {
+        for (int n = N0; n < N1; ++n) {
           const double wn = (*W)(n, ps);
-          acc[n] += (adv_coef(o, n) * cl + ck2) * wn;
+          acc[n - N0] += (adv_coef(o, n) * cl + ck2) * wn;
         }
       }
     }
   }
 #pragma unroll
-  for (int n = 0; n < NVAR; ++n) out(n, acc[n]);
+  for (int n = N0; n < N1; ++n) out(n, acc[n - N0]);
 }
 
 }  // namespace z4c

@@ -2316,8 +2316,11 @@ __global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
   }
 }
 
+// blockIdx.y = 2 c + h: cotangent c, fields [0, 7) (h = 0: chi and g~, the
+// second-derivative-heavy half) or [7, 22) (h = 1)
 __global__ void __launch_bounds__(TILE) k_adj_stencil(AdjArgs a) {
-  const long long c = blockIdx.y;
+  const long long c = blockIdx.y >> 1;
+  const int half = blockIdx.y & 1;
   int b, i, j, k;
   if (!tile_point(a.L, b, i, j, k)) return;
   const DevBlock B = a.blocks[b];
@@ -2332,10 +2335,14 @@ __global__ void __launch_bounds__(TILE) k_adj_stencil(AdjArgs a) {
   auto Bx = [&](int d, int p) { return bload(UB, (unsigned)p * 8u, (V_BETAX + d) * fu); };
   const unsigned vp = (unsigned)flat(B, i, j, k) * 8u;
   const bool interior = is_interior(B, i, j, k);
-  adj_stencil_at(a.o, B, S, a.hseed ? nullptr : &W, Bx, i, j, k, [&](int n, double v) {
+  auto out = [&](int n, double v) {
     const double pt = interior ? bload(OB, vp, n * fu) : 0.0;
     bstore(OB, pt + v, vp, n * fu);
-  });
+  };
+  if (half == 0)
+    adj_stencil_at<0, 7>(a.o, B, S, a.hseed ? nullptr : &W, Bx, i, j, k, out);
+  else
+    adj_stencil_at<7, NVAR>(a.o, B, S, a.hseed ? nullptr : &W, Bx, i, j, k, out);
 }
 
 // pbar[c*NPBAR + t] (+)= sum of the partials of cotangent c: one workgroup
@@ -3518,7 +3525,7 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   a.L = launch_of(c, BOX_WHOLE);
-  hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), nw), dim3(TILE), 0, s, a);
+  hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), 2 * nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   hipLaunchKernelGGL(k_adj_pbar, dim3(nw * NPBAR), dim3(256), 0, s,
                      (const double*)c->adj_part, Gi, nw, pbar, accumulate);
