@@ -3485,6 +3485,7 @@ int ensure_buf(double*& p, long long& have, long long need, const char* what) {
 
 constexpr int MAX_NW = Z4C_MAX_NW;  // grid.y / grid.x limits of the adjoint kernels
 static_assert(MAX_NW * NPBAR <= 65535 && Z4C_NPBAR == NPBAR, "cotangent limit");
+static_assert(2 * MAX_NW <= 65535, "k_adj_stencil's grid.y holds two field halves per cotangent");
 
 int check_vjp_args(const z4c_ctx* c, const z4c_params* p, int nw, const char* what) {
   if (nw < 1 || nw > MAX_NW)
