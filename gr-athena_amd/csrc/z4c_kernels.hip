@@ -1728,10 +1728,14 @@ __global__ void __launch_bounds__(PRT_THREADS) k_prolong_vjp(PrTArgs a) {
   const double* ub = a.ubar + B.off * a.nvar + (long long)n * B.npts;
   const unsigned char* mk = a.fmask + B.off;
   const int nF = nfx * nfy * nfz;
+  // both loads unconditional (every staged vertex lies in the block), so
+  // the iterations' loads overlap
+#pragma unroll 4
   for (int q = threadIdx.x; q < nF; q += PRT_THREADS) {
     const int x = q % nfx, r = q / nfx, y = r % nfy, z = r / nfy;
     const long long p = (T.fx0 + x) + (long long)B.nn1 * ((T.fy0 + y) + (long long)B.nn2 * (T.fz0 + z));
-    F[q] = mk[p] ? ub[p] : 0.0;
+    const double v = ub[p];
+    F[q] = mk[p] ? v : 0.0;
   }
   __syncthreads();
   const int n1 = nfx * nfy * T.ncz;
@@ -3343,7 +3347,7 @@ int build_ml_transpose(z4c_ctx* c) {
   };
   int smem = 0;
   HIPCALL(hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const int lds_max = std::min(16384, smem / (int)sizeof(double));  // doubles, at most 128 KiB
+  const int lds_max = std::min(9728, smem / (int)sizeof(double));  // doubles, at most 76 KiB
   int best_ty = 0, best_tz = 0;
   long long best = LLONG_MAX;
   int ymax = 1, zmax = 1;
