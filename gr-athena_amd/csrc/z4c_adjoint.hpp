@@ -280,23 +280,35 @@ __host__ __device__ __forceinline__ void rhs_fwd_point(const RhsParams& o, const
   F(F_DIV, div);
 }
 
-// Reverse mode of rhs_curvature_point + rhs_assembly_point (vacuum), from
-// the stored forward intermediates.
-//   X.u(n)       state at the point
-//   X.d1(n, a)   d_a u_n, X.d2(n, ab) d_a d_b u_n, X.ls(n, a) Lsel_a(u_n)
-//   Fw(s)        forward intermediate s (rhs_fwd_point)
-//   w[n]         cotangent of rhs_n at the point
-//   wH           cotangent of the Hamiltonian constraint Ht = R + 2/3 K^2 -
-//                A~_ab A~^ab formed inside the right-hand side (:606-624)
-// Outputs: ub(n, v) stores the pointwise cotangent of u_n(p), S.d1(n, a, v)
-// and S.d2(n, ab, v) the derivative cotangents (each exactly once, as soon
-// as final); pb[0..2] += parameter partials.  Forward values are read where
-// they are used rather than held in registers.
+// Accumulators of rhs_vjp_point live between its two phases (split before
+// the Ricci adjoint); phase a also adds the Gudd adjoints bG[0..17].
+struct AdjCarry {
+  double b_RR[6], b_g[6], b_gu[6], b_dphi[3], b_Gu[3], b_dchi[3];
+  double b_oochi, b_chig, b_oopsi4;
+};
+constexpr int NCARRY = 30;  // doubles of AdjCarry
+// element e of the carry in a fixed order (e a compile-time constant once
+// the callers' loops are unrolled)
+__host__ __device__ __forceinline__ double& carry_at(AdjCarry& cy, int e) {
+  if (e < 6) return cy.b_RR[e];
+  if (e < 12) return cy.b_g[e - 6];
+  if (e < 18) return cy.b_gu[e - 12];
+  if (e < 21) return cy.b_dphi[e - 18];
+  if (e < 24) return cy.b_Gu[e - 21];
+  if (e < 27) return cy.b_dchi[e - 24];
+  if (e == 27) return cy.b_oochi;
+  if (e == 28) return cy.b_chig;
+  return cy.b_oopsi4;
+}
+
+// Phase a of rhs_vjp_point: the right-hand side terms down to D_aD_b alpha
+// (every w-dependent term, the parameter partials, the Gudd part of bG).
 template <class XIn, class FIn, class UOut, class SOut>
-__host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const XIn& X,
-                                                       const FIn& Fw, const double* w,
-                                                       UOut&& ub, SOut&& S, double* pb,
-                                                       double* bG, int bgs, double wH) {
+__host__ __device__ __forceinline__ void rhs_vjp_point_a(const RhsParams& o, const XIn& X,
+                                                         const FIn& Fw, const double* w,
+                                                         UOut&& ub, SOut&& S, double* pb,
+                                                         double* bG, int bgs, double wH,
+                                                         AdjCarry& cy) {
   const double pw = o.chi_psi_power;
   const double k1 = o.damp_kappa1, k2 = o.damp_kappa2;
   auto g = [&](int ab) { return X.u(V_GXX + ab); };
@@ -601,6 +613,76 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
 #pragma unroll
   for (int a = 0; a < 3; ++a) S.d1(V_ALPHA, a, b_dalpha[a]);
 
+  // the accumulators live across the split
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    cy.b_RR[q] = b_RR[q];
+    cy.b_g[q] = b_g[q];
+    cy.b_gu[q] = b_gu[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    cy.b_dphi[q] = b_dphi[q];
+    cy.b_Gu[q] = b_Gu[q];
+    cy.b_dchi[q] = b_dchi[q];
+  }
+  cy.b_oochi = b_oochi;
+  cy.b_chig = b_chig;
+  cy.b_oopsi4 = b_oopsi4;
+}
+
+// Phase b of rhs_vjp_point: the Ricci tensor, the conformal factor, the
+// Christoffels and the inverse metric; bG holds phase a's Gudd adjoints and
+// zeros above.
+template <class XIn, class FIn, class UOut, class SOut>
+__host__ __device__ __forceinline__ void rhs_vjp_point_b(const RhsParams& o, const XIn& X,
+                                                         const FIn& Fw, UOut&& ub, SOut&& S,
+                                                         double* bG, int bgs,
+                                                         const AdjCarry& cy) {
+  const double pw = o.chi_psi_power;
+  const double k1 = o.damp_kappa1, k2 = o.damp_kappa2;
+  auto g = [&](int ab) { return X.u(V_GXX + ab); };
+  auto A = [&](int ab) { return X.u(V_AXX + ab); };
+  auto Gam = [&](int a) { return X.u(V_GAMX + a); };
+  auto beta = [&](int a) { return X.u(V_BETAX + a); };
+  auto dchi = [&](int a) { return X.d1(V_CHI, a); };
+  auto dalpha = [&](int a) { return X.d1(V_ALPHA, a); };
+  auto dbeta = [&](int b, int a) { return X.d1(V_BETAX + a, b); };
+  auto gu = [&](int ab) { return Fw(f_gu(ab)); };
+  auto Gddd = [&](int c, int ab) { return Fw(f_Gddd(c, ab)); };
+  auto Gudd = [&](int c, int ab) { return Fw(f_Gudd(c, ab)); };
+  auto Gu = [&](int a) { return Fw(f_Gu(a)); };
+  auto dphi = [&](int a) { return Fw(f_dphi(a)); };
+  auto Ddphi = [&](int ab) { return Fw(f_Ddphi(ab)); };
+  auto RR = [&](int ab) { return Fw(f_RR(ab)); };
+  auto DDa = [&](int ab) { return Fw(f_DDa(ab)); };
+  auto Aud = [&](int c, int b) { return Fw(f_Aud(c, b)); };
+  auto AA = [&](int ab) { return Fw(f_AA(ab)); };
+  auto Auu = [&](int ab) { return Fw(f_Auu(ab)); };
+  auto DA = [&](int a) { return Fw(f_DA(a)); };
+  auto ddbd = [&](int a) { return Fw(f_ddbd(a)); };
+  const bool chi_floored = X.u(V_CHI) < o.chi_div_floor;
+  const double chi_g = Fw(F_CHIG), oochi = Fw(F_OOCHI), oopsi4 = Fw(F_OOPSI4);
+  const double trphi = Fw(F_TRPHI), Dda = Fw(F_DDA), Ddalpha = Fw(F_DDALPHA), R = Fw(F_R);
+  const double trAA = Fw(F_TRAA), div = Fw(F_DIV);
+  const double alpha = X.u(V_ALPHA), Khat = X.u(V_KHAT), Theta = X.u(V_THETA);
+  const double K = Khat + 2. * Theta;
+  const double cpow = -6.0 / pw;
+
+  double b_RR[6], b_g[6], b_gu[6], b_dphi[3], b_Gu[3], b_dchi[3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    b_RR[q] = cy.b_RR[q];
+    b_g[q] = cy.b_g[q];
+    b_gu[q] = cy.b_gu[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    b_dphi[q] = cy.b_dphi[q];
+    b_Gu[q] = cy.b_Gu[q];
+    b_dchi[q] = cy.b_dchi[q];
+  }
+  double b_oochi = cy.b_oochi, b_chig = cy.b_chig, b_oopsi4 = cy.b_oopsi4;
   // Ricci (:357-458)
   double b_Ddphi[6] = {};
   double b_trphi = 0.0;
@@ -771,6 +853,28 @@ __host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const
   ub(V_CHI, chi_floored ? 0.0 : b_chig);
 #pragma unroll
   for (int ab = 0; ab < 6; ++ab) ub(V_GXX + ab, b_g[ab]);
+}
+
+// Reverse mode of rhs_curvature_point + rhs_assembly_point (vacuum), from
+// the stored forward intermediates.
+//   X.u(n)       state at the point
+//   X.d1(n, a)   d_a u_n, X.d2(n, ab) d_a d_b u_n, X.ls(n, a) Lsel_a(u_n)
+//   Fw(s)        forward intermediate s (rhs_fwd_point)
+//   w[n]         cotangent of rhs_n at the point
+//   wH           cotangent of the Hamiltonian constraint Ht = R + 2/3 K^2 -
+//                A~_ab A~^ab formed inside the right-hand side (:606-624)
+// Outputs: ub(n, v) stores the pointwise cotangent of u_n(p), S.d1(n, a, v)
+// and S.d2(n, ab, v) the derivative cotangents (each exactly once, as soon
+// as final); pb[0..2] += parameter partials.  Forward values are read where
+// they are used rather than held in registers.
+template <class XIn, class FIn, class UOut, class SOut>
+__host__ __device__ __forceinline__ void rhs_vjp_point(const RhsParams& o, const XIn& X,
+                                                       const FIn& Fw, const double* w,
+                                                       UOut&& ub, SOut&& S, double* pb,
+                                                       double* bG, int bgs, double wH) {
+  AdjCarry cy;
+  rhs_vjp_point_a(o, X, Fw, w, ub, S, pb, bG, bgs, wH, cy);
+  rhs_vjp_point_b(o, X, Fw, ub, S, bG, bgs, cy);
 }
 
 // Hamiltonian constraint of the Z4c right-hand side at a vertex

@@ -2235,6 +2235,8 @@ struct AdjArgs {
   long long reg;     // NVAR * total_pts
   long long sreg;    // NADJ * total_int
   int hseed;         // 1: seed = d/du of sum vol Ht^2 (w unused, may be null)
+  double* carry;     // nw sets of NCARRY + 18 values per interior vertex
+  long long creg;    // (NCARRY + 18) * total_int
 };
 
 
@@ -2271,9 +2273,15 @@ struct AdjSBuf {
   __device__ void d2(int n, int ab, double v) const { bstore(S, v, vq, l_d2(n, ab) * fl); }
 };
 
-__global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
+// The pointwise adjoint in two kernels (the reverse sweep split before the
+// Ricci adjoint, each phase with its own register budget).  Phase a: the
+// cotangent w, every term down to D_aD_b alpha, the parameter partials
+// (one sum per workgroup), the Gudd adjoints in registers; it stores the
+// accumulators live across the split and the Gudd adjoints, slot-major.
+constexpr int NCARRY_ALL = NCARRY + 18;
+
+__global__ void __launch_bounds__(TILE) k_adj_point_a(AdjArgs a) {
   __shared__ double red[NPBAR][TILE / 64];
-  __shared__ double shG[42 * TILE];  // per-thread Christoffel adjoints
   const long long c = blockIdx.y;
   double pb[NPBAR] = {0.0, 0.0, 0.0};
   int b, i, j, k;
@@ -2302,8 +2310,16 @@ __global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
     }
     const Buf UB = make_buf(a.ubar + c * a.reg + B.off * NVAR, NVAR * N * 8);
     const int fu = X.fu;
-    rhs_vjp_point(a.o, X, F, wp, [&](int n, double v) { bstore(UB, v, vp, n * fu); }, S, pb,
-                  shG + threadIdx.x, TILE, wH);
+    double bG[42];
+    AdjCarry cy;
+    rhs_vjp_point_a(a.o, X, F, wp, [&](int n, double v) { bstore(UB, v, vp, n * fu); }, S, pb, bG,
+                    1, wH, cy);
+    const Buf CB = make_buf(a.carry + c * a.creg + B.ioff * NCARRY_ALL, NCARRY_ALL * nint * 8);
+    const int fs = (int)(nint * 8);
+#pragma unroll
+    for (int e = 0; e < NCARRY; ++e) bstore(CB, carry_at(cy, e), vq, e * fs);
+#pragma unroll
+    for (int e = 0; e < 18; ++e) bstore(CB, bG[e], vq, (NCARRY + e) * fs);
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -2318,6 +2334,37 @@ __global__ void __launch_bounds__(TILE) k_adj_point(AdjArgs a) {
     for (int q = 0; q < TILE / 64; ++q) v += red[threadIdx.x][q];
     a.part[(c * gridDim.x + blockIdx.x) * NPBAR + threadIdx.x] = v;
   }
+}
+
+// Phase b: the Ricci adjoint onward, the Christoffel adjoints in a per-thread
+// LDS column (42 doubles) seeded from phase a.
+__global__ void __launch_bounds__(TILE) k_adj_point_b(AdjArgs a) {
+  __shared__ double shG[42 * TILE];
+  const long long c = blockIdx.y;
+  int b, i, j, k;
+  if (!tile_point(a.L, b, i, j, k)) return;
+  const DevBlock B = a.blocks[b];
+  const long long N = B.npts, nint = B.nint;
+  const unsigned vp = (unsigned)flat(B, i, j, k) * 8u, vq = (unsigned)iidx(B, i, j, k) * 8u;
+  const AdjXBuf X{make_buf(a.u + B.off * NVAR, NVAR * N * 8),
+                  make_buf(a.lin + B.ioff * NLINF, NLINF * nint * 8), vp, vq, (int)(N * 8),
+                  (int)(nint * 8)};
+  const AdjFBuf F{X.L, vq, (int)(nint * 8)};
+  const AdjSBuf S{make_buf(a.sb + c * a.sreg + B.ioff * NADJ, NADJ * nint * 8), vq,
+                  (int)(nint * 8)};
+  const Buf CB = make_buf(a.carry + c * a.creg + B.ioff * NCARRY_ALL, NCARRY_ALL * nint * 8);
+  const int fs = (int)(nint * 8);
+  AdjCarry cy;
+#pragma unroll
+  for (int e = 0; e < NCARRY; ++e) carry_at(cy, e) = bload(CB, vq, e * fs);
+  double* bG = shG + threadIdx.x;
+#pragma unroll
+  for (int e = 0; e < 18; ++e) bG[e * TILE] = bload(CB, vq, (NCARRY + e) * fs);
+#pragma unroll
+  for (int e = 18; e < 42; ++e) bG[e * TILE] = 0.0;
+  const Buf UB = make_buf(a.ubar + c * a.reg + B.off * NVAR, NVAR * N * 8);
+  const int fu = X.fu;
+  rhs_vjp_point_b(a.o, X, F, [&](int n, double v) { bstore(UB, v, vp, n * fu); }, S, bG, TILE, cy);
 }
 
 // blockIdx.y = 2 c + h: cotangent c, fields [0, 7) (h = 0: chi and g~, the
@@ -2743,6 +2790,8 @@ struct z4c_ctx {
   double* adj_lin = nullptr;
   double* adj_sb = nullptr;
   double* adj_part = nullptr;
+  double* adj_carry = nullptr;  // the pointwise adjoint's values between its two kernels
+  long long adj_carry_n = 0;
   double* adj_k = nullptr;
   double* adj_stage = nullptr;
   double* adj_reg = nullptr;
@@ -2829,6 +2878,9 @@ struct z4c_ctx {
       *q = nullptr;
     }
     adj_lin_n = adj_sb_n = adj_part_n = adj_k_n = adj_stage_n = adj_reg_n = 0;
+    hipFree(adj_carry);
+    adj_carry = nullptr;
+    adj_carry_n = 0;
     hipFree(adj_pre);
     adj_pre = nullptr;
     adj_pre_n = 0;
@@ -3520,14 +3572,20 @@ int launch_rhs_vjp(z4c_ctx* c, const z4c_params* p, const double* u, int nw, con
   int rc = ensure_buf(c->adj_lin, c->adj_lin_n, c->total_int * NLINF, "rhs_vjp");
   if (!rc) rc = ensure_buf(c->adj_sb, c->adj_sb_n, (long long)nw * sreg, "rhs_vjp");
   if (!rc) rc = ensure_buf(c->adj_part, c->adj_part_n, (long long)nw * Gi * NPBAR, "rhs_vjp");
+  if (!rc)
+    rc = ensure_buf(c->adj_carry, c->adj_carry_n, (long long)nw * c->total_int * NCARRY_ALL,
+                    "rhs_vjp");
   if (rc) return rc;
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, w, ubar,
-            c->adj_sb, c->adj_part, reg, sreg, w ? 0 : 1};
+            c->adj_sb, c->adj_part, reg, sreg, w ? 0 : 1, c->adj_carry,
+            c->total_int * NCARRY_ALL};
   rc = launch_lin(c, p, u, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
-  hipLaunchKernelGGL(k_adj_point, dim3(Gi, nw), dim3(TILE), 0, s, a);
+  hipLaunchKernelGGL(k_adj_point_a, dim3(Gi, nw), dim3(TILE), 0, s, a);
+  HIPCALL(hipGetLastError());
+  hipLaunchKernelGGL(k_adj_point_b, dim3(Gi, nw), dim3(TILE), 0, s, a);
   HIPCALL(hipGetLastError());
   a.L = launch_of(c, BOX_WHOLE);
   hipLaunchKernelGGL(k_adj_stencil, dim3(grid_for(a.L.ntiles), 2 * nw), dim3(TILE), 0, s, a);
@@ -3545,7 +3603,7 @@ int launch_hnorm(z4c_ctx* c, const z4c_params* p, const double* u, double* J, hi
   if (!rc) rc = ensure_buf(c->adj_part, c->adj_part_n, (long long)Gi * NPBAR, "hamiltonian_norm");
   if (rc) return rc;
   AdjArgs a{c->d_blocks, launch_of(c, BOX_INTERIOR), rhs_params(p), u, c->adj_lin, nullptr,
-            nullptr, nullptr, c->adj_part, 0, 0, 1};
+            nullptr, nullptr, c->adj_part, 0, 0, 1, nullptr, 0};
   rc = launch_lin(c, p, u, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_adj_fwd, dim3(Gi), dim3(TILE), 0, s, a);
