@@ -1490,20 +1490,37 @@ __global__ void __launch_bounds__(256) k_prolong_vc(ProlongArgs a) {
   double* dst = a.u + B.off * a.nvar + (long long)f * B.npts;
   for (int q = G.e0; q < G.e0 + G.ne; ++q) {
     const RfEntry E = a.ent[q];
-    for (int m = 0; m < 8; ++m) {
-      const int pi = m & 1, pj = (m >> 1) & 1, pk = m >> 2;
-      int c0[3], n[3];
-      rf_case_range(E, 0, pi, c0[0], n[0]);
-      rf_case_range(E, 1, pj, c0[1], n[1]);
-      rf_case_range(E, 2, pk, c0[2], n[2]);
-      if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) continue;
-      const int n01 = n[0] * n[1], np = n01 * n[2];
+    // both x parities of a coarse index per thread: the two cases differ
+    // only in their x stencil, and the thread writes the adjacent fine
+    // vertices 2 (ci - RF_H) and 2 (ci - RF_H) + 1 (coalesced stores)
+    int x0[2], xn[2];
+    rf_case_range(E, 0, 0, x0[0], xn[0]);
+    rf_case_range(E, 0, 1, x0[1], xn[1]);
+    int xlo = INT_MAX, xhi = INT_MIN;
+    for (int h = 0; h < 2; ++h)
+      if (xn[h] > 0) {
+        xlo = min(xlo, x0[h]);
+        xhi = max(xhi, x0[h] + xn[h]);
+      }
+    if (xlo >= xhi) continue;
+    const int nx = xhi - xlo;
+    for (int m2 = 0; m2 < 4; ++m2) {
+      const int pj = m2 & 1, pk = m2 >> 1;
+      int cy0, ny, cz0, nz;
+      rf_case_range(E, 1, pj, cy0, ny);
+      rf_case_range(E, 2, pk, cz0, nz);
+      if (ny <= 0 || nz <= 0) continue;
+      const int mb = (pj << 1) | (pk << 2);
+      const int nxy = nx * ny, np = nxy * nz;
       for (int t = threadIdx.x; t < np; t += 256) {
-        const int tk = t / n01, r = t - tk * n01, tj = r / n[0], ti = r - tj * n[0];
-        const int ci = c0[0] + ti, cj = c0[1] + tj, ck = c0[2] + tk;
-        const double v = rf_prolong_point(cl, s2, s3, ci - G.r0[0], cj - G.r0[1], ck - G.r0[2], m);
-        const int fi = 2 * (ci - RF_H) + pi, fj = 2 * (cj - RF_H) + pj, fk = 2 * (ck - RF_H) + pk;
-        dst[fi + (long long)nn1 * (fj + (long long)nn2 * fk)] = v;
+        const int tk = t / nxy, r = t - tk * nxy, tj = r / nx, ti = r - tj * nx;
+        const int ci = xlo + ti, cj = cy0 + tj, ck = cz0 + tk;
+        const int li = ci - G.r0[0], lj = cj - G.r0[1], lk = ck - G.r0[2];
+        const int fj = 2 * (cj - RF_H) + pj, fk = 2 * (ck - RF_H) + pk;
+        double* row = dst + (long long)nn1 * (fj + (long long)nn2 * fk) + 2 * (ci - RF_H);
+        if (ci >= x0[0] && ci < x0[0] + xn[0]) row[0] = rf_prolong_point(cl, s2, s3, li, lj, lk, mb);
+        if (ci >= x0[1] && ci < x0[1] + xn[1])
+          row[1] = rf_prolong_point(cl, s2, s3, li, lj, lk, mb | 1);
       }
     }
   }
