@@ -497,7 +497,9 @@ enum { Z4C_PBAR_SHIFT_ETA = 0, Z4C_PBAR_DAMP_KAPPA1 = 1, Z4C_PBAR_DAMP_KAPPA2 = 
    vertex of u, ghosts included; pbar (device, nw * Z4C_NPBAR doubles)
    receives d<w_c, rhs>/d(shift_eta, damp_kappa1, damp_kappa2).  Vacuum only;
    Z4C_EINVAL with cowling set.  The context allocates its scratch on the
-   first call (about (180 + 114 nw) doubles per interior vertex). */
+   first call (282 + 162 nw doubles per interior vertex: the linearisation's
+   derivatives and forward intermediates, and per cotangent the derivative
+   cotangents and the pointwise adjoint's values between its two kernels). */
 int z4c_rhs_vjp(z4c_ctx* ctx, const z4c_params* p, const double* u, int nw, const double* w,
                 double* ubar, double* pbar, void* stream);
 
