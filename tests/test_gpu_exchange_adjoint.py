@@ -129,6 +129,40 @@ def test_ml_exchange_vjp_dot_product(case, nvar):
     z.close()
 
 
+def test_ml_exchange_vjp_context_reuse():
+    """One context: the transpose data are built on the first call, serve
+    any field count, and are rebuilt by a second z4c_set_exchange; every
+    result equals a fresh context's bit for bit."""
+    c, geoms = _ml_geoms("open_3level")
+    rng = np.random.default_rng(29)
+    ys = {nv: [rng.standard_normal((nv,) + g.shape) for g in geoms] for nv in (22, 3)}
+
+    def fresh(nv):
+        z = Z4c(geoms, Params())
+        z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+        y = z.upload(ys[nv], nv)
+        z.exchange_vjp(y, nv)
+        torch.cuda.synchronize()
+        out = y.clone()
+        z.close()
+        return out
+
+    ref = {nv: fresh(nv) for nv in (22, 3)}
+    z = Z4c(geoms, Params())
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    for nv in (22, 3, 22):
+        y = z.upload(ys[nv], nv)
+        z.exchange_vjp(y, nv)
+        torch.cuda.synchronize()
+        assert torch.equal(y, ref[nv]), nv
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    y = z.upload(ys[3], 3)
+    z.exchange_vjp(y, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(y, ref[3])
+    z.close()
+
+
 def test_physical_bc_vjp_dot_product():
     """<F x, y> = <x, F^T y> for the physical fill on blocks with outflow
     faces (every face of one block; two faces of a second)."""
