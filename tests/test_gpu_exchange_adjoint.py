@@ -349,3 +349,27 @@ def test_ml_step_ghosts_vjp_finite_differences(case, physical):
         print(f"{name}: fd {fd:.12e}, adjoint {an:.12e}", flush=True)
         assert abs(fd - an) <= 1e-6 * abs(an), (name, fd, an)
     z.close()
+
+
+def test_step_ghosts_vjp_batched_cotangents():
+    """Two cotangents in one z4c_rk4_step_ghosts_vjp call (the update across
+    levels and the outflow fill after every stage) equal two single calls
+    bit for bit, state and parameter cotangents."""
+    c, geoms = _ml_geoms("open_centre")
+    p = Params(**dict(PUNCTURE, eps_floor=-0.5, damp_kappa2=0.05))
+    z = Z4c(geoms, p)
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    u_d = _smooth_state(z, geoms, p)
+    dt = 0.25 * min(g.dx[0] for g in geoms)
+    rng = np.random.default_rng(41)
+    ws = [z.upload([rng.standard_normal((22,) + g.shape) for g in geoms]) for _ in range(2)]
+    both, pb = z.rk4_step_ghosts_vjp(dt, u_d, torch.cat(ws), 2, physical=True)
+    n = ws[0].numel()
+    for q in range(2):
+        one, pb1 = z.rk4_step_ghosts_vjp(dt, u_d, ws[q], 1, physical=True)
+        torch.cuda.synchronize()
+        assert torch.equal(both[q * n:(q + 1) * n], one), q
+        assert torch.equal(pb[q], pb1[0]), q
+    assert not torch.equal(both[:n], both[n:])
+    z.close()
+
