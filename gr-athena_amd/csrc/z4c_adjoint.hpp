@@ -716,28 +716,131 @@ __host__ __device__ __forceinline__ void rhs_vjp_point_b(const RhsParams& o, con
             b_gu[sym(c, d)] -= 0.5 * mult(c, d) * X.d2(V_GXX + ab, sym(c, d)) * r;
             S.d2(V_GXX + ab, sym(c, d), -0.5 * mult(c, d) * gu(sym(c, d)) * r);
           }
-        // the quadratic Christoffel terms: accumulators in the bG column
-        // (inverse-metric part at bG[36 + cd]), loops kept rolled
-#pragma unroll 1
+      }
+    // The quadratic Christoffel terms, sum_{a<=b} R_ab Q_ab with
+    // Q_ab = g^cd (G^e_ca G_b,ed + G^e_cb G_a,ed + G^e_ad G_e,cb).  Q is
+    // symmetric in (a, b), so the sum equals the sum over ordered (a, b) of
+    // w_ab g^cd (2 G^e_ca G_b,ed + G^e_ad G_e,cb), w_aa = R_aa and
+    // w_ab = R_ab / 2, whose derivatives contract through
+    // Acon[c][b][e] = sum_d g^cd G_b,ed and Ncon[a][d][e] = sum_c g^dc G^e_ca.
+    // Cotangents of a symmetric pair add both orders (the packing of bG).
+    {
+      double w[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) w[a][b] = (a == b) ? b_RR[sym(a, a)] : 0.5 * b_RR[sym(a, b)];
+      {
+        // G^e_ca (term 1) and G^e_ad (term 3), through Acon
+        double Acon[3][3][3];
+#pragma unroll
         for (int c = 0; c < 3; ++c)
-#pragma unroll 1
-          for (int d = 0; d < 3; ++d) {
-            const double gcd = gu(sym(c, d));
-            double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int e = 0; e < 3; ++e) {
-              s += Gudd(e, sym(c, a)) * Gddd(b, sym(e, d)) + Gudd(e, sym(c, b)) * Gddd(a, sym(e, d)) +
-                   Gudd(e, sym(a, d)) * Gddd(e, sym(c, b));
-              bG[(6 * e + sym(c, a)) * bgs] += gcd * Gddd(b, sym(e, d)) * r;
-              bG[(18 + 6 * b + sym(e, d)) * bgs] += gcd * Gudd(e, sym(c, a)) * r;
-              bG[(6 * e + sym(c, b)) * bgs] += gcd * Gddd(a, sym(e, d)) * r;
-              bG[(18 + 6 * a + sym(e, d)) * bgs] += gcd * Gudd(e, sym(c, b)) * r;
-              bG[(6 * e + sym(a, d)) * bgs] += gcd * Gddd(e, sym(c, b)) * r;
-              bG[(18 + 6 * e + sym(c, b)) * bgs] += gcd * Gudd(e, sym(a, d)) * r;
+              double t = 0.0;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) t += gu(sym(c, d)) * Gddd(b, sym(e, d));
+              Acon[c][b][e] = t;  // sum_d g^cd G_b,ed
             }
-            bG[(36 + sym(c, d)) * bgs] += s * r;
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              double t1 = 0.0, t3 = 0.0;
+#pragma unroll
+              for (int b = 0; b < 3; ++b) {
+                t1 += w[a][b] * Acon[c][b][e];
+                t3 += w[c][b] * Acon[a][e][b];  // term 3's G^e_ad with (a, d) = (c, a)
+              }
+              bG[(6 * e + sym(c, a)) * bgs] += 2.0 * t1 + t3;
+            }
+      }
+      {
+        // G_b,ed (term 1) and G_e,cb (term 3), through Ncon
+        double Ncon[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+              double t = 0.0;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) t += gu(sym(d, c)) * Gudd(e, sym(c, a));
+              Ncon[a][d][e] = t;  // sum_c g^dc G^e_ca
+            }
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              double t1 = 0.0, t3 = 0.0;
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                t1 += w[a][b] * Ncon[a][d][e];
+                t3 += w[a][d] * Ncon[a][e][b];  // term 3's G_e,cb with (e, c, b) = (b, e, d)
+              }
+              bG[(18 + 6 * b + sym(e, d)) * bgs] += 2.0 * t1 + t3;
+            }
+      }
+      {
+        // g^cd, term 1: 2 sum_{a,e} G^e_ca P[a][e][d], P[a][e][d] = sum_b w_ab G_b,ed
+        double P[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              double t = 0.0;
+#pragma unroll
+              for (int b = 0; b < 3; ++b) t += w[a][b] * Gddd(b, sym(e, d));
+              P[a][e][d] = t;
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+              for (int e = 0; e < 3; ++e) t += Gudd(e, sym(c, a)) * P[a][e][d];
+            bG[(36 + sym(c, d)) * bgs] += 2.0 * t;
           }
       }
+      {
+        // g^cd, term 3: sum_{a,e} G^e_ad Q2[e][c][a], Q2[e][c][a] = sum_b w_ab G_e,cb
+        double Q2[3][3][3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              double t = 0.0;
+#pragma unroll
+              for (int b = 0; b < 3; ++b) t += w[a][b] * Gddd(e, sym(c, b));
+              Q2[e][c][a] = t;
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            double t = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+              for (int e = 0; e < 3; ++e) t += Gudd(e, sym(a, d)) * Q2[e][c][a];
+            bG[(36 + sym(c, d)) * bgs] += t;
+          }
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
