@@ -3416,7 +3416,10 @@ int build_ml_transpose(z4c_ctx* c) {
   };
   int smem = 0;
   HIPCALL(hipDeviceGetAttribute(&smem, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const int lds_max = std::min(9728, smem / (int)sizeof(double));  // doubles, at most 76 KiB
+  // within 76 KiB (two workgroups per CU), else within the device's LDS
+  // (large blocks: the x extent of a tile is the block's coarse x range)
+  const int lds_cap[2] = {std::min(9728, smem / (int)sizeof(double)),
+                          smem / (int)sizeof(double)};
   int best_ty = 0, best_tz = 0;
   long long best = LLONG_MAX;
   int ymax = 1, zmax = 1;
@@ -3425,17 +3428,18 @@ int build_ml_transpose(z4c_ctx* c) {
       ymax = std::max(ymax, bx[3] - bx[2] + 1);
       zmax = std::max(zmax, bx[5] - bx[4] + 1);
     }
-  for (int tz = 1; tz <= std::min(zmax, 8); ++tz)
-    for (int ty = 1; ty <= ymax; ++ty) {
-      long long staged;
-      int lds;
-      make_tiles(ty, tz, nullptr, staged, lds);
-      if (lds <= lds_max && staged < best) {
-        best = staged;
-        best_ty = ty;
-        best_tz = tz;
+  for (int pass = 0; pass < 2 && !best_ty; ++pass)
+    for (int tz = 1; tz <= std::min(zmax, 8); ++tz)
+      for (int ty = 1; ty <= ymax; ++ty) {
+        long long staged;
+        int lds;
+        make_tiles(ty, tz, nullptr, staged, lds);
+        if (lds <= lds_cap[pass] && staged < best) {
+          best = staged;
+          best_ty = ty;
+          best_tz = tz;
+        }
       }
-    }
   std::vector<PrTTile> tiles;
   if (!ent.empty()) {
     if (!best_ty) return fail(Z4C_EINVAL, "exchange_vjp: blocks too large for the prolongation's transpose");

@@ -230,7 +230,9 @@ int z4c_exchange(z4c_ctx* ctx, int nvar, double* u, void* stream);
    writes once however many boxes overlap there.  Deterministic (gathers in
    a fixed order).  The transpose data of the update across levels are
    built on the first call (device memory: about 22 doubles per source
-   vertex and per coarse vertex). */
+   vertex and per coarse vertex).  Across levels the prolongation's
+   transpose stages a block's full fine x range per workgroup, which limits
+   blocks to about 140 cells along x (Z4C_EINVAL beyond). */
 int z4c_exchange_vjp(z4c_ctx* ctx, int nvar, double* ubar, void* stream);
 
 /* Transpose of z4c_physical_bc (no counterpart in the reference): the

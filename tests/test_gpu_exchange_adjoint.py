@@ -129,6 +129,29 @@ def test_ml_exchange_vjp_dot_product(case, nvar):
     z.close()
 
 
+def test_ml_exchange_vjp_dot_product_large_blocks():
+    """<G x, y> = <x, G^T y> across levels on 72^3-cell blocks, whose
+    prolongation transpose tiles need more than 76 KiB of LDS (the chooser's
+    fallback to the device's LDS)."""
+    c, geoms = _ml_geoms("periodic_2level", nx=72)
+    nvar = 3
+    z = Z4c(geoms, Params())
+    z.set_exchange([c["lo"]] * 3, [c["hi"]] * 3, c["periodic"])
+    rng = np.random.default_rng(31)
+    x = z.upload([rng.standard_normal((nvar,) + g.shape) for g in geoms], nvar)
+    y = z.upload([rng.standard_normal((nvar,) + g.shape) for g in geoms], nvar)
+    gx = x.clone()
+    z.exchange(gx, nvar)
+    gty = y.clone()
+    z.exchange_vjp(gty, nvar)
+    torch.cuda.synchronize()
+    lhs, rhs = float((gx * y).sum()), float((x * gty).sum())
+    scale = float((gx.abs() * y.abs()).sum())
+    print(f"{len(geoms)} blocks of 72^3: <Gx, y> = {lhs:.15e}, <x, G^T y> = {rhs:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale, (lhs, rhs)
+    z.close()
+
+
 def test_ml_exchange_vjp_context_reuse():
     """One context: the transpose data are built on the first call, serve
     any field count, and are rebuilt by a second z4c_set_exchange; every
