@@ -187,13 +187,17 @@ def test_ml_exchange_vjp_context_reuse():
 
 
 def test_physical_bc_vjp_dot_product():
-    """<F x, y> = <x, F^T y> for the physical fill on blocks with outflow
-    faces (every face of one block; two faces of a second)."""
+    """<F x, y> = <x, F^T y> for the physical fill: every face of one block
+    outflow, two faces of a second, and a third block with Sommerfeld faces
+    (the fill extrapolates their ghosts the same way) and one outflow face."""
     g0 = MeshBlockGeom.cube(8, 1.0 / 8, (0.0, 0.0, 0.0))
     g0.outflow_faces = 63
     g1 = MeshBlockGeom((10, 8, 12), (0.1, 0.12, 0.08), (2.0, 0.0, 0.0))
     g1.outflow_faces = 2 | 16
-    geoms = [g0, g1]
+    g2 = MeshBlockGeom((8, 10, 8), (0.1, 0.1, 0.1), (-3.0, 0.0, 0.0))
+    g2.sommerfeld_faces = 1 | 8 | 32
+    g2.outflow_faces = 4
+    geoms = [g0, g1, g2]
     z = Z4c(geoms, Params())
     rng = np.random.default_rng(29)
     x = z.upload([rng.standard_normal((22,) + g.shape) for g in geoms])
