@@ -400,3 +400,62 @@ def test_step_ghosts_vjp_batched_cotangents():
     assert not torch.equal(both[:n], both[n:])
     z.close()
 
+
+def test_step_ghosts_vjp_single_level_outflow_finite_differences():
+    """The step VJP with the same-level exchange and the outflow fill after
+    every stage (ghosts = 6 on one spacing): 2 x 2 x 2 blocks of 8^3 cells on
+    an open box, the gauge-wave state with a shift, eps_floor = -0.5 and w, v
+    at interior vertices as in the open tree's check; fourth-order
+    differences of <w, step(u)> in the state and the shift damping."""
+    nb, n, h = 2, 8, 1.0 / 8
+    geoms = []
+    for k in range(nb):
+        for j in range(nb):
+            for i in range(nb):
+                g = MeshBlockGeom.cube(n, h, (i * n * h, j * n * h, k * n * h))
+                f = 0
+                for d, idx in enumerate((i, j, k)):
+                    f |= (1 << (2 * d)) if idx == 0 else (2 << (2 * d))
+                g.outflow_faces = f
+                geoms.append(g)
+    p = Params(**dict(PUNCTURE, eps_floor=-0.5, damp_kappa2=0.05))
+    z = Z4c(geoms, p)
+    z.set_exchange([0.0] * 3, [nb * n * h] * 3, (False, False, False))
+    u_d = _smooth_state(z, geoms, p)
+    dt = 0.25 * h
+    rng = np.random.default_rng(43)
+
+    def field(g):
+        a = rng.standard_normal((22,) + g.shape)
+        m = np.zeros(g.shape, bool)
+        m[4:-4, 4:-4, 4:-4] = True
+        return a * m
+
+    w_d = z.upload([field(g) for g in geoms])
+    v_d = z.upload([field(g) for g in geoms])
+    ubar, pbar = z.rk4_step_ghosts_vjp(dt, u_d, w_d, 1, physical=True)
+    work = z.make_step_work()
+
+    def phi(uu, **kw):
+        saved = z.params
+        z.params = Params(**dict(p.as_dict(), **kw))
+        out = uu.clone()
+        z.rk4_step_fused(dt, out, work, exchange=True, physical=True)
+        z.params = saved
+        return float((out * w_d).sum())
+
+    eps = 1e-6
+    fd = (-phi(u_d + 2 * eps * v_d) + 8 * phi(u_d + eps * v_d) - 8 * phi(u_d - eps * v_d)
+          + phi(u_d - 2 * eps * v_d)) / (12 * eps)
+    an = float((ubar * v_d).sum())
+    print(f"state: fd {fd:.12e}, adjoint {an:.12e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    x = p.shift_eta
+    hh = 1e-2 * max(1.0, abs(x))
+    fd = (-phi(u_d, shift_eta=x + 2 * hh) + 8 * phi(u_d, shift_eta=x + hh)
+          - 8 * phi(u_d, shift_eta=x - hh) + phi(u_d, shift_eta=x - 2 * hh)) / (12 * hh)
+    an = float(pbar[0, 0])
+    print(f"shift_eta: fd {fd:.12e}, adjoint {an:.12e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    z.close()
+
