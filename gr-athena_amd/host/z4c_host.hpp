@@ -178,6 +178,24 @@ class Z4cBatch {
   }
   void Exchange(double* uio) { Check(z4c_exchange(ctx_, Z4C_NVAR, uio, nullptr), "exchange"); }
 
+  // -- adjoints (no counterpart in the reference) ----------------------------
+  // ubar (nw registers) = (d rhs / d u)^T w at uin; pbar (nw * Z4C_NPBAR,
+  // device) the gauge-parameter gradients
+  void RhsVjp(const double* uin, int nw, const double* w, double* ubar, double* pbar) {
+    Check(z4c_rhs_vjp(ctx_, &opt_, uin, nw, w, ubar, pbar, nullptr), "z4c_rhs_vjp");
+  }
+  // ubar <- (the ghost update set by SetExchange)^T ubar
+  void ExchangeVjp(double* ubar) {
+    Check(z4c_exchange_vjp(ctx_, Z4C_NVAR, ubar, nullptr), "z4c_exchange_vjp");
+  }
+  // VJP of the fused bt_rk4 step with the ghost update (ghosts 2 or 6)
+  void StepGhostsVjp(double dt, const double* uin, int nw, const double* ubar_out,
+                     double* ubar_in, double* pbar, int ghosts) {
+    Check(z4c_rk4_step_ghosts_vjp(ctx_, &opt_, dt, uin, nw, ubar_out, ubar_in, pbar, ghosts,
+                                  nullptr),
+          "z4c_rk4_step_ghosts_vjp");
+  }
+
   // -- GR_Z4c: one step of <time>/integrator on u ----------------------------
   // "bt_rk4": Butcher stages (SumBT_ak / SumBT_bk); "vl2", "rk1", "rk2",
   // "rk3", "rk4": low-storage stages (IntegrateZ4c).  `fused`: one call per

@@ -3,7 +3,7 @@
 // evaluates Z4cRHS of the initial state, then runs the GR_Z4c step loop, and
 // writes both results.
 //
-// Usage: z4c_host_demo <input> <output>
+// Usage: z4c_host_demo <input> <output> [<adjoint output>]
 //
 // Input (little-endian, packed):
 //   char   magic[4] = "Z4CH"
@@ -18,6 +18,11 @@
 //   per block: 22 * nverts1 * nverts2 * nverts3 doubles, u(n,k,j,i)
 // Output: per block the right-hand side of the initial state (22 fields,
 // interior written, ghosts 0), then per block the state after nsteps steps.
+// Adjoint output (optional): per block the RHS VJP at the initial state with
+// that right-hand side as the cotangent, then its 3 parameter gradients;
+// with the exchange (ghosts & 2), per block the exchange's transpose of the
+// same VJP and then per block the VJP of one fused bt_rk4 step with the
+// ghost update (cotangent: the initial state), and its 3 gradients.
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -46,7 +51,7 @@ T get(std::ifstream& in) {
   return v;
 }
 
-int run(const char* inpath, const char* outpath) {
+int run(const char* inpath, const char* outpath, const char* adjpath) {
   std::ifstream in(inpath, std::ios::binary);
   if (!in) throw std::runtime_error(std::string("cannot open ") + inpath);
   char magic[4];
@@ -91,6 +96,39 @@ int run(const char* inpath, const char* outpath) {
       batch.Download(b, rhs.get(), Z4C_NVAR, host[b].data());
       out.write(reinterpret_cast<const char*>(host[b].data()), host[b].size() * sizeof(double));
     }
+    if (adjpath) {
+      std::ofstream adj(adjpath, std::ios::binary);
+      z4c_host::DeviceBuffer ubar(Z4C_NVAR * batch.elems()), pbar(Z4C_NPBAR);
+      std::vector<double> hp(Z4C_NPBAR), tmp;
+      auto dump = [&](const double* d) {
+        for (int b = 0; b < nblocks; ++b) {
+          tmp.resize(host[b].size());
+          batch.Download(b, d, Z4C_NVAR, tmp.data());
+          adj.write(reinterpret_cast<const char*>(tmp.data()), tmp.size() * sizeof(double));
+        }
+      };
+      auto dump_p = [&]() {
+        z4c_host::HipCheck(hipMemcpy(hp.data(), pbar.get(), sizeof(double) * Z4C_NPBAR,
+                                     hipMemcpyDeviceToHost),
+                           "pbar");
+        adj.write(reinterpret_cast<const char*>(hp.data()), hp.size() * sizeof(double));
+      };
+      batch.RhsVjp(batch.u.get(), 1, rhs.get(), ubar.get(), pbar.get());
+      batch.Synchronize();
+      dump(ubar.get());
+      dump_p();
+      if (ghosts & z4c_host::kExchange) {
+        batch.ExchangeVjp(ubar.get());
+        batch.Synchronize();
+        dump(ubar.get());
+        batch.StepGhostsVjp(dt, batch.u.get(), 1, batch.u.get(), ubar.get(), pbar.get(),
+                            ghosts & (z4c_host::kExchange | z4c_host::kPhysical));
+        batch.Synchronize();
+        dump(ubar.get());
+        dump_p();
+      }
+      if (!adj) throw std::runtime_error(std::string("cannot write ") + adjpath);
+    }
   }
   for (int s = 0; s < nsteps; ++s) batch.Step(integ, dt, ghosts, fused != 0);
   batch.Synchronize();
@@ -107,12 +145,12 @@ int run(const char* inpath, const char* outpath) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s <input> <output>\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s <input> <output> [<adjoint output>]\n", argv[0]);
     return 2;
   }
   try {
-    return run(argv[1], argv[2]);
+    return run(argv[1], argv[2], argc == 4 ? argv[3] : nullptr);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "z4c_host_demo: %s\n", e.what());
     return 1;

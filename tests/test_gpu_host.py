@@ -129,3 +129,51 @@ def test_host_program_against_oracle(tmp_path, integ, fused):
         r_ = _rel(u_got[b], exp_u[b], 0.01)
         print(f"block {b}: max rel diff after {nsteps} steps {r_.max():.3e}")
         assert np.all(r_ <= 1e-11), (b, r_)
+
+
+def test_host_program_adjoint_equals_python_binding(tmp_path):
+    """The C++ host's adjoint calls (z4c_rhs_vjp, z4c_exchange_vjp,
+    z4c_rk4_step_ghosts_vjp through Z4cBatch) equal the Python binding's on
+    the same blocks, state and cotangents, bit for bit: the RHS VJP with the
+    right-hand side as the cotangent and its parameter gradients, the
+    exchange's transpose of it, and the VJP of one fused step with the
+    exchange after every stage (the state as the cotangent)."""
+    import torch
+    from z4c_amd.z4c import Z4c
+    assert os.path.exists(HOST_DEMO), "build the host program first (__graft_entry__.build())"
+    blocks = _blocks()
+    geoms = [g for _, _, g in blocks]
+    p = Params(**dict(PARAMS, shift_eta=1.5, damp_kappa2=0.05))
+    op = orc.Params(**p.as_dict())
+    ogs = [orc.Geom.block(g.nx, g.dx, g.xmin) for g in geoms]
+    us = _initial(op, ogs, geoms)
+    dt = 0.25 / 16
+    inp, out, adj = tmp_path / "in.bin", tmp_path / "out.bin", tmp_path / "adj.bin"
+    _write_input(inp, blocks, us, p, "bt_rk4", dt, 0, True)
+    r = subprocess.run([HOST_DEMO, str(inp), str(out), str(adj)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(adj, dtype="<f8")
+    n = sum(22 * int(np.prod(g.shape)) for g in geoms)
+    assert got.size == 3 * n + 2 * 3
+    z = Z4c(geoms, p)
+    z.set_exchange(LO, HI, (True, True, True))
+    u = z.upload(us)
+    rhs = z.zeros()
+    z.Z4cRHS(u, None, rhs)
+    z.Z4cBoundaryRHS(u, None, rhs)
+    ubar, pbar = z.rhs_vjp(u, rhs, 1)
+    ex = ubar.clone()
+    z.exchange_vjp(ex)
+    sbar, spbar = z.rk4_step_ghosts_vjp(dt, u, u, 1)
+    torch.cuda.synchronize()
+    host = lambda t: np.concatenate([z.download(t, 22)[b].reshape(-1) for b in range(len(geoms))])
+    assert np.array_equal(got[:n], host(ubar))
+    assert np.array_equal(got[n:n + 3], pbar.cpu().numpy().reshape(-1))
+    assert np.array_equal(got[n + 3:2 * n + 3], host(ex))
+    assert np.array_equal(got[2 * n + 3:3 * n + 3], host(sbar))
+    assert np.array_equal(got[3 * n + 3:], spbar.cpu().numpy().reshape(-1))
+    assert np.abs(got[:n]).max() > 0 and np.abs(got[2 * n + 3:3 * n + 3]).max() > 0
+    z.close()
+
